@@ -1,0 +1,164 @@
+/*
+ * libmpm_hip — C ABI of the MI355X (gfx950) MPM snow-simulation step.
+ *
+ * Drop-in for the simulation path of luoluobuli/WebGPU-MPM-based-Snow-Simulation: the
+ * TypeScript managers that own the WGSL compute passes (src/lib/gpu/mpm/GpuMpmBufferManager.ts,
+ * GpuMpmPipelineManager.ts, src/lib/gpu/uniforms/GpuUniformsBufferManager.ts,
+ * src/lib/gpu/collider/GpuColliderBufferManager.ts) bind these entry points instead of
+ * creating GPU buffers and pipelines. Plain pointers and sizes only; every entry returns
+ * 0 on success or a negative MPM_E_* code, and mpm_last_error() gives the message.
+ *
+ * Threading: a context is not reentrant (one caller at a time) but may be called from
+ * different OS threads; each entry binds the context's device first.
+ */
+#ifndef MPM_HIP_H
+#define MPM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_UNIFORMS_BYTES 37376   /* GpuUniformsBufferManager.ts:17 */
+#define MPM_PARTICLE_BYTES 192     /* shaderPrelude/mpmOps.wgsl:1-12, GpuMpmBufferManager.ts:24 */
+#define MPM_BVH_NODE_BYTES 32      /* mpm/colliderPrelude.wgsl:6-11 */
+
+enum {
+    MPM_OK = 0,
+    MPM_E_INVALID = -1,   /* bad argument / handle / size mismatch */
+    MPM_E_HIP = -2,       /* HIP runtime error */
+    MPM_E_NOMEM = -3,     /* device allocation failed */
+    MPM_E_RANGE = -4,     /* value outside what the context was created for */
+    MPM_E_STATE = -5      /* call not valid in the current state (e.g. no particles yet) */
+};
+
+enum { MPM_METHOD_EXPLICIT = 0, MPM_METHOD_PBMPM = 1 };   /* GpuSimulationMethod.ts:1-4 */
+
+/* Creation-time configuration. Every field that the reference hard-codes in WGSL or TS has
+ * a runtime value here; mpm_config_defaults() fills the reference values. */
+typedef struct mpm_config {
+    uint32_t n_particles;          /* GpuMpmBufferManager.ts:15-21 nParticles */
+    uint32_t grid_res[3];          /* uniforms gridResolution @208 */
+    float    grid_min[3];          /* uniforms gridMinCoords @16 (reference: -5) */
+    float    grid_max[3];          /* uniforms gridMaxCoords @32 (reference: +5) */
+    float    fixed_point_scale;    /* uniforms fixedPointScale @4 (GpuSnowPipelineRunner.svelte.ts:28, 65536) */
+    float    dt;                   /* uniforms simulationTimestep @0 */
+    uint32_t max_blocks;           /* sparseGridPrelude.wgsl:7; 0 = derived from grid and N */
+    uint32_t hash_map_size;        /* sparseGridPrelude.wgsl:6; 0 = derived (prime > 2 max_blocks) */
+    uint32_t max_probes;           /* sparseGridPrelude.wgsl:8 (100) */
+    int32_t  pbmpm_iterations;     /* GpuMpmPipelineManager.ts:445 (3) */
+    float    pbmpm_force_divisor;  /* pbmpm.wgsl:137 (3.0) */
+    float    gravity[3];           /* gridUpdate.cs.wgsl:43, pbmpm.wgsl:101 ((0,0,-9.81)) */
+    double   youngs_modulus;       /* stressTensorOps.wgsl:1 (1.4e5) */
+    double   poissons_ratio;       /* stressTensorOps.wgsl:2 (0.2) */
+    float    hardening_explicit;   /* stressTensorOps.wgsl:44 (5) */
+    float    hardening_pbmpm;      /* pbmpm.wgsl:35 (20) */
+    float    pbmpm_blend;          /* pbmpm.wgsl:26 (0.95) */
+    float    pbmpm_volume_exponent;/* pbmpm.wgsl:23 (-0.3333333) */
+    double   theta_c;              /* plasticityOps.wgsl:18 (2.5e-2) */
+    double   theta_s;              /* plasticityOps.wgsl:19 (7.5e-3) */
+    double   density;              /* particleToGrid.cs.wgsl:36 (400) */
+    float    grad_scale;           /* scale of D written by G2P (1 = reference, 1/h = physical) */
+    float    collider_margin;      /* colliderPrelude.wgsl:158,297,330,351 (0.05) */
+    float    friction_ccd;         /* colliderPrelude.wgsl:308 (0.1) */
+    float    friction_proximity;   /* colliderPrelude.wgsl:339 (0.0) */
+    float    collider_velocity_factor; /* colliderPrelude.wgsl:294 (0.2) */
+    uint32_t wall_mode;            /* 0 = reference clamp+restitution; bit0 = also sticky grid border */
+    uint32_t wall_cells;           /* sticky border thickness in cells (3) */
+    uint32_t chunk_particles;      /* particles per LDS-tiled work item (0 = default 512) */
+    uint32_t grid_workgroups;      /* workgroups of the chunk kernels (0 = derived from CU count) */
+    int32_t  device;               /* HIP device ordinal */
+    uint32_t use_graph;            /* capture one substep in a hipGraph and replay it */
+} mpm_config;
+
+typedef struct mpm_ctx mpm_ctx;
+
+/* Counters for conditions the reference handles silently (mapAffectedBlocks.wgsl:27-31,52;
+ * particleToGrid.cs.wgsl:47-48; countParticlesPerBlock.wgsl:20). Values are of the last substep. */
+typedef struct mpm_stats {
+    uint64_t n_particles;
+    uint64_t particles_binned;
+    uint64_t particles_unbinned;     /* own block not allocated / outside the block grid */
+    uint64_t particle_blocks;        /* distinct own blocks (runs of the sort) */
+    uint64_t work_chunks;
+    uint64_t blocks_allocated;       /* n_allocated_blocks */
+    uint64_t blocks_dropped;         /* beyond max_blocks */
+    uint64_t probe_failures;         /* hash insert exhausted max_probes */
+    uint64_t substeps;
+} mpm_stats;
+
+/* Per-phase device time of the last timed mpm_step (timing enabled with mpm_set_timing). */
+typedef struct mpm_timings {
+    double compute_simulation_step_ns;   /* GpuSnowPipelineRunner.svelte.ts:520-524 computeSimulationStepNs */
+    double binning_ns;
+    double transfer_ns;                  /* P2G / G2P / constraint passes */
+    double integrate_ns;
+    uint64_t substeps;
+} mpm_timings;
+
+void mpm_config_defaults(mpm_config* cfg);
+
+/* new GpuMpmBufferManager({device, nParticles}) + new GpuMpmPipelineManager({...})
+ * (GpuMpmBufferManager.ts:15-80, GpuMpmPipelineManager.ts:42-66). Also writes the uniform
+ * fields the runner's constructor writes (GpuSnowPipelineRunner.svelte.ts:136-144). */
+int  mpm_create(const mpm_config* cfg, mpm_ctx** out);
+void mpm_destroy(mpm_ctx* ctx);
+
+/* device.queue.writeBuffer(uniformsBuffer, offset, bytes) — byte-compatible with the
+ * reference Uniforms layout (shaderPrelude/uniforms.wgsl:8-56, GpuUniformsBufferManager.ts:54-182). */
+int  mpm_write_uniforms(mpm_ctx* ctx, size_t offset, const void* bytes, size_t len);
+int  mpm_read_uniforms(mpm_ctx* ctx, size_t offset, void* bytes, size_t len);
+
+/* colliderDataBuffer + colliderBvhBuffer (GpuColliderBufferManager.ts:96-219): data_words is
+ * the packed u32 collider data (indices in BVH leaf order, then f32 xyz vertices, ...),
+ * bvh_nodes are 32-byte BvhNode records. Also writes uniforms colliderNumIndices @348. */
+int  mpm_set_collider(mpm_ctx* ctx, const uint32_t* data_words, size_t n_words, uint32_t n_indices,
+                      const void* bvh_nodes, size_t n_nodes);
+
+/* GpuColliderBufferManager.buildBvh / buildBvhRecursive (GpuColliderBufferManager.ts:222-350)
+ * on the host: median split along the longest axis by centroid, leaf <= 4 triangles, 32-byte
+ * nodes, left child = node + 1. tri_order receives the leaf-order triangle permutation
+ * (reorderedIndices, :211-219). nodes_cap >= 2*n_tris (or 1 when empty). */
+int  mpm_build_bvh(const float* positions, size_t n_vertices, const uint32_t* indices, size_t n_indices,
+                   void* nodes_out, size_t nodes_cap, size_t* n_nodes, uint32_t* tri_order);
+
+/* queue.writeBuffer(particleDataBuffer, ...) / readback of the 192-byte ParticleData records. */
+int  mpm_upload_particles(mpm_ctx* ctx, const void* aos192, size_t n);
+int  mpm_download_particles(mpm_ctx* ctx, void* aos192, size_t n);
+
+/* addExplicitMpmDispatches / addPbmpmDispatches x n_substeps, then queue.submit
+ * (GpuSnowPipelineRunner.svelte.ts:392-445, GpuMpmPipelineManager.ts:327-521). Enqueues on the
+ * context's stream and returns; mpm_sync waits. */
+int  mpm_step(mpm_ctx* ctx, int method, int n_substeps);
+int  mpm_sync(mpm_ctx* ctx);
+
+/* Test hooks (no reference counterpart). mpm_debug_stage runs one substep prefix:
+ * explicit: 0 binning, 1 +P2G, 2 +grid update, >=3 full; PBMPM: 0 binning, s in 1..k the grid
+ * after the pbmpm dispatch of iteration s-1, > k full. mpm_dump_grid returns every cell of every
+ * allocated block keyed by cell coordinate with its (mass, mx, my, mz) i32 accumulators. */
+int  mpm_debug_stage(mpm_ctx* ctx, int method, int stage);
+int  mpm_dump_grid(mpm_ctx* ctx, int32_t* cell_xyz, int32_t* vals4, size_t cap, size_t* n_out);
+int  mpm_dump_blocks(mpm_ctx* ctx, int32_t* block_xyz, size_t cap, size_t* n_out);
+int  mpm_dump_particle_blocks(mpm_ctx* ctx, int32_t* block_xyz, uint32_t* counts, size_t cap, size_t* n_out);
+
+/* Device evaluation of the step's scalar / 3x3 arithmetic for bitwise comparison with a host
+ * restatement (no reference counterpart). kind: 0 exp(a) 1 log(a) 2 pow(a,b) 3 sqrt(a) 4 a/b
+ * 5 hash3(bits a, bits b, index) 6 det3 7 inverse3 8 polar rotation 9 i32(a); matrix kinds take
+ * and return 9 column-major floats per item. */
+int  mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n);
+
+int  mpm_get_stats(mpm_ctx* ctx, mpm_stats* out);
+int  mpm_set_timing(mpm_ctx* ctx, int enabled);
+int  mpm_get_timings(mpm_ctx* ctx, mpm_timings* out);
+const char* mpm_last_error(mpm_ctx* ctx);
+const char* mpm_version(void);
+
+/* In-process consumers: the device stream the context enqueues on (a hipStream_t). */
+void* mpm_stream(mpm_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM_HIP_H */

@@ -1,0 +1,194 @@
+"""Oracle — TEST INFRASTRUCTURE ONLY (tests/, __graft_entry__.smoke(), bench.py cpu_baseline).
+
+ctypes front end of oracle/mpm_oracle.cpp: the CPU restatement of the reference's MPM
+substep (FP32-faithful, and an FP64 physics variant). Never imported by the product package.
+"""
+import ctypes
+import pathlib
+import subprocess
+
+import numpy as np
+
+ORACLE_DIR = pathlib.Path(__file__).resolve().parent
+LIB = ORACLE_DIR / "_build" / "liborc.so"
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("pbmpm_iterations", ctypes.c_int32),
+        ("pbmpm_force_divisor", ctypes.c_float),
+        ("gravity", ctypes.c_float * 3),
+        ("youngs_modulus", ctypes.c_double),
+        ("poissons_ratio", ctypes.c_double),
+        ("hardening_explicit", ctypes.c_float),
+        ("hardening_pbmpm", ctypes.c_float),
+        ("pbmpm_blend", ctypes.c_float),
+        ("pbmpm_volume_exponent", ctypes.c_float),
+        ("theta_c", ctypes.c_double),
+        ("theta_s", ctypes.c_double),
+        ("density", ctypes.c_double),
+        ("grad_scale", ctypes.c_float),
+        ("collider_margin", ctypes.c_float),
+        ("friction_ccd", ctypes.c_float),
+        ("friction_proximity", ctypes.c_float),
+        ("collider_velocity_factor", ctypes.c_float),
+        ("wall_mode", ctypes.c_uint32),
+        ("wall_cells", ctypes.c_uint32),
+        ("hash_map_size", ctypes.c_uint32),
+        ("max_blocks", ctypes.c_uint32),
+        ("max_probes", ctypes.c_uint32),
+    ]
+
+
+class OrcStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("blocks_allocated", "blocks_dropped", "particles_unbinned", "lookups_missed")]
+
+
+_lib = None
+_P = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = ctypes.CDLL(str(LIB))
+        sig = {
+            "orc_default_params": (None, [ctypes.POINTER(OrcParams)]),
+            "orc_params_size": (ctypes.c_size_t, []),
+            "orc_create": (_P, [ctypes.c_int, ctypes.POINTER(OrcParams), _P, ctypes.c_uint32, _P]),
+            "orc_destroy": (None, [_P]),
+            "orc_set_uniforms": (None, [_P, _P]),
+            "orc_set_collider": (None, [_P, _P, ctypes.c_size_t, _P, ctypes.c_size_t]),
+            "orc_step": (None, [_P, ctypes.c_int, ctypes.c_int]),
+            "orc_run_stage": (None, [_P, ctypes.c_int, ctypes.c_int]),
+            "orc_get_particles": (None, [_P, _P]),
+            "orc_get_particles_f64": (None, [_P, _P]),
+            "orc_set_particles": (None, [_P, _P, ctypes.c_uint32]),
+            "orc_dump_grid": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_size_t]),
+            "orc_active_blocks": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_size_t]),
+            "orc_get_stats": (None, [_P, ctypes.POINTER(OrcStats)]),
+            "orc_hash1": (ctypes.c_uint32, [ctypes.c_uint32]),
+            "orc_hash3": (ctypes.c_uint32, [ctypes.c_uint32] * 3),
+            "orc_hash4": (ctypes.c_uint32, [ctypes.c_uint32] * 4),
+            "orc_exp": (ctypes.c_float, [ctypes.c_float]),
+            "orc_log": (ctypes.c_float, [ctypes.c_float]),
+            "orc_pow": (ctypes.c_float, [ctypes.c_float, ctypes.c_float]),
+            "orc_f2i": (ctypes.c_int32, [ctypes.c_float]),
+            "orc_det3": (ctypes.c_float, [_P]),
+            "orc_inverse3": (None, [_P, _P]),
+            "orc_polar3": (None, [_P, _P]),
+            "orc_mmul3": (None, [_P, _P, _P]),
+            "orc_bspline": (None, [_P, _P, _P]),
+            "orc_closest_point_triangle": (None, [_P, _P, _P, _P, _P]),
+            "orc_intersect_ray_triangle": (ctypes.c_float, [_P, _P, _P, _P, _P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        assert L.orc_params_size() == ctypes.sizeof(OrcParams)
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def params(**kw):
+    p = OrcParams()
+    lib().orc_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        cur = getattr(p, k)
+        if isinstance(cur, ctypes.Array):
+            for i, x in enumerate(v):
+                cur[i] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class OracleSim:
+    """One CPU simulation. mode 0: FP32-faithful restatement; mode 1: FP64 twin with the same
+    fixed-point grid; mode 2: FP64 physics without fixed-point quantization."""
+
+    def __init__(self, particles, uniforms, fp64=False, mode=None, **param_kw):
+        if mode is None:
+            mode = 1 if fp64 else 0
+        self.n = particles.shape[0]
+        self._prm = params(**param_kw)
+        self._uni = np.ascontiguousarray(uniforms, np.uint8)
+        p = np.ascontiguousarray(particles)
+        self.h = lib().orc_create(int(mode), ctypes.byref(self._prm), _ptr(self._uni), self.n, _ptr(p))
+        self._dtype = particles.dtype
+
+    def close(self):
+        if self.h:
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_uniforms(self, uniforms):
+        self._uni = np.ascontiguousarray(uniforms, np.uint8)
+        lib().orc_set_uniforms(self.h, _ptr(self._uni))
+
+    def set_collider(self, data_words, bvh_nodes):
+        d = np.ascontiguousarray(data_words, np.uint32)
+        b = np.ascontiguousarray(bvh_nodes, np.uint32)
+        lib().orc_set_collider(self.h, _ptr(d), d.size, _ptr(b), b.size // 8)
+
+    def step(self, method, n=1):
+        lib().orc_step(self.h, int(method), int(n))
+
+    def run_stage(self, method, stage):
+        lib().orc_run_stage(self.h, int(method), int(stage))
+
+    def particles(self):
+        out = np.zeros(self.n, self._dtype)
+        lib().orc_get_particles(self.h, _ptr(out))
+        return out
+
+    def state_f64(self):
+        out = np.zeros((self.n, 37), np.float64)
+        lib().orc_get_particles_f64(self.h, _ptr(out))
+        return out
+
+    def dump_grid(self):
+        n = lib().orc_dump_grid(self.h, None, None, 0)
+        xyz = np.zeros((n, 3), np.int32)
+        vals = np.zeros((n, 4), np.float64)
+        lib().orc_dump_grid(self.h, _ptr(xyz), _ptr(vals), n)
+        return xyz, vals
+
+    def active_blocks(self):
+        n = lib().orc_active_blocks(self.h, None, None, 0)
+        xyz = np.zeros((n, 3), np.int32)
+        cnt = np.zeros(n, np.uint32)
+        lib().orc_active_blocks(self.h, _ptr(xyz), _ptr(cnt), n)
+        return xyz, cnt
+
+    def stats(self):
+        s = OrcStats()
+        lib().orc_get_stats(self.h, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+
+def f32_vec(fn, *arrays):
+    """Apply a scalar oracle function elementwise over float32 arrays."""
+    arrays = [np.asarray(a, np.float32).ravel() for a in arrays]
+    out = np.empty(arrays[0].size, np.float32)
+    for i in range(out.size):
+        out[i] = fn(*(float(a[i]) for a in arrays))
+    return out

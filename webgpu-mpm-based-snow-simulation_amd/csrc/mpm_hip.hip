@@ -1,0 +1,707 @@
+// libmpm_hip — C ABI and host orchestration of the MPM substep on one MI355X.
+//
+// Host-side counterpart of src/lib/gpu/mpm/GpuMpmBufferManager.ts (buffer sizing),
+// GpuMpmPipelineManager.ts:327-521 (per-substep dispatch schedule),
+// src/lib/gpu/uniforms/GpuUniformsBufferManager.ts (uniform blob) and
+// src/lib/gpu/collider/GpuColliderBufferManager.ts:79-350 (collider packing, BVH build).
+// One substep is a fixed sequence of kernels on the context's stream; with use_graph it is
+// captured once per (method, buffer parity, parameters) into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <cstdlib>
+#include <cmath>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "mpm_kernels.h"
+#include "../../include/mpm_hip.h"
+
+using namespace mpm;
+
+struct mpm_ctx {
+    mpm_config cfg;
+    int device = 0;
+    int n_cu = 256;
+    hipStream_t stream = nullptr;
+    uint8_t uni[MPM_UNIFORMS_BYTES];
+    std::string err;
+    uint32_t n = 0, stride = 0, K = 0, nb[3] = {0, 0, 0}, cap = 0, max_blocks = 0, max_runs = 0, max_chunks = 0, ntiles = 0;
+    uint32_t chunk = 512;
+    float* planes[2] = {nullptr, nullptr};
+    uint32_t* pid[2] = {nullptr, nullptr};
+    int cur = 0;
+    float* relax = nullptr;
+    uint32_t *key = nullptr, *mask = nullptr, *perm = nullptr, *kcount = nullptr, *kmask = nullptr, *kcursor = nullptr;
+    uint32_t *tile_sums = nullptr, *tile_ofs = nullptr;
+    uint32_t *run_key = nullptr, *run_start = nullptr, *run_count = nullptr, *run_mask = nullptr;
+    uint2* chunks = nullptr;
+    uint32_t *table = nullptr, *slot_val = nullptr, *nbr_slot = nullptr, *nbr = nullptr, *block_key = nullptr;
+    int4* grid[3] = {nullptr, nullptr, nullptr};
+    uint32_t* counters = nullptr;
+    uint32_t* coll_data = nullptr;
+    uint4* bvh = nullptr;
+    size_t n_coll_words = 0, n_bvh_nodes = 0;
+    int last_grid = 0;
+    bool has_particles = false;
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    SimParams gparams[2][2];
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    mpm_timings tm{};
+    uint64_t substeps = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+const char* k_version = "libmpm_hip 0.1 (gfx950)";
+
+int fail(mpm_ctx* c, int code, const std::string& m) { if (c) c->err = m; return code; }
+#define HIPCHK(c, x) do { hipError_t _e = (x); if (_e != hipSuccess) return fail((c), MPM_E_HIP, std::string(#x) + ": " + hipGetErrorString(_e)); } while (0)
+
+template <class T> int dalloc(mpm_ctx* c, T** p, size_t count) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (hipMalloc(&q, bytes) != hipSuccess) { (void)hipGetLastError(); return fail(c, MPM_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed"); }
+    c->allocs.push_back(q);
+    *p = (T*)q;
+    return MPM_OK;
+}
+
+uint32_t next_prime(uint32_t v) {
+    if (v < 3) return 3;
+    if (!(v & 1u)) ++v;
+    for (;; v += 2) {
+        bool prime = true;
+        for (uint32_t d = 3; (uint64_t)d * d <= v; d += 2) if (v % d == 0) { prime = false; break; }
+        if (prime) return v;
+    }
+}
+
+template <class T> T rdu(const uint8_t* b, size_t off) { T v; std::memcpy(&v, b + off, sizeof(T)); return v; }
+template <class T> void wru(uint8_t* b, size_t off, T v) { std::memcpy(b + off, &v, sizeof(T)); }
+
+Bufs make_bufs(mpm_ctx* c) {
+    Bufs B;
+    B.cur = c->planes[c->cur]; B.nxt = c->planes[c->cur ^ 1];
+    B.pid_cur = c->pid[c->cur]; B.pid_nxt = c->pid[c->cur ^ 1];
+    B.stride = c->stride;
+    B.relax = c->relax; B.key = c->key; B.mask = c->mask; B.perm = c->perm;
+    B.kcount = c->kcount; B.kmask = c->kmask; B.kcursor = c->kcursor;
+    B.tile_sums = c->tile_sums; B.tile_ofs = c->tile_ofs;
+    B.run_key = c->run_key; B.run_start = c->run_start; B.run_count = c->run_count; B.run_mask = c->run_mask;
+    B.chunks = c->chunks;
+    B.table = c->table; B.slot_val = c->slot_val; B.nbr_slot = c->nbr_slot; B.nbr = c->nbr; B.block_key = c->block_key;
+    B.grid0 = c->grid[0]; B.grid1 = c->grid[1]; B.grid2 = c->grid[2];
+    B.counters = c->counters;
+    B.coll_data = c->coll_data; B.bvh = c->bvh;
+    return B;
+}
+
+// Parse the uniform blob (GpuUniformsBufferManager.ts:54-182 offsets) + creation config.
+int make_params(mpm_ctx* c, SimParams& P) {
+    std::memset(&P, 0, sizeof(P));
+    const uint8_t* u = c->uni;
+    const mpm_config& g = c->cfg;
+    P.dt = rdu<float>(u, 0);
+    P.S = rdu<float>(u, 4);
+    for (int a = 0; a < 3; ++a) {
+        P.gmin[a] = rdu<float>(u, 16 + 4 * a);
+        P.gmax[a] = rdu<float>(u, 32 + 4 * a);
+        P.res[a] = (int32_t)rdu<uint32_t>(u, 208 + 4 * a);
+        P.cd[a] = rdu<float>(u, 352 + 4 * a);
+        P.cmin[a] = rdu<float>(u, 224 + 4 * a);
+        P.cmax[a] = rdu<float>(u, 240 + 4 * a);
+        P.cvel[a] = rdu<float>(u, 320 + 4 * a);
+        P.ip[a] = rdu<float>(u, 496 + 4 * a);
+        P.grav[a] = g.gravity[a];
+        if (P.res[a] <= 0) return fail(c, MPM_E_RANGE, "grid resolution must be positive");
+        P.nb[a] = (P.res[a] + 3) / 4;
+    }
+    uint64_t K = (uint64_t)P.nb[0] * P.nb[1] * P.nb[2];
+    if (K > c->K) return fail(c, MPM_E_RANGE, "uniform grid resolution exceeds the resolution the context was created for");
+    P.K = (uint32_t)K;
+    for (int i = 0; i < 16; ++i) { P.T[i] = rdu<float>(u, 256 + 4 * i); P.Tinv[i] = rdu<float>(u, 432 + 4 * i); }
+    P.coll_num_indices = rdu<uint32_t>(u, 348);
+    P.coll_num_objects = rdu<uint32_t>(u, 364);
+    P.istrength = rdu<float>(u, 508);
+    P.iradius = rdu<float>(u, 512);
+    P.interacting = rdu<uint32_t>(u, 516);
+    P.imode = rdu<uint32_t>(u, 520);
+    P.n = c->n;
+    P.chunk = c->chunk;
+    P.cap = c->cap;
+    P.max_blocks = c->max_blocks;
+    P.max_probes = g.max_probes ? g.max_probes : 100u;
+    P.mu0 = (float)(g.youngs_modulus / (2.0 * (1.0 + g.poissons_ratio)));
+    P.la0 = (float)(g.youngs_modulus * g.poissons_ratio / ((1.0 + g.poissons_ratio) * (1.0 - 2.0 * g.poissons_ratio)));
+    P.xi_e = g.hardening_explicit;
+    P.xi_p = g.hardening_pbmpm;
+    P.blend = g.pbmpm_blend;
+    P.one_minus_blend = 1.0f - g.pbmpm_blend;
+    P.vol_exp = g.pbmpm_volume_exponent;
+    P.plast_lo = (float)(1.0 - g.theta_c);
+    P.plast_hi = (float)(1.0 + g.theta_s);
+    P.inv_density = (float)(1.0 / g.density);
+    P.grad_scale = g.grad_scale;
+    P.force_div = g.pbmpm_force_divisor;
+    P.wall_mode = g.wall_mode;
+    P.wall_cells = g.wall_cells;
+    P.n_bvh_nodes = (uint32_t)c->n_bvh_nodes;
+    P.has_collider = (c->n_bvh_nodes > 0 && c->n_coll_words > 0) ? 1u : 0u;
+    P.margin = g.collider_margin;
+    P.fr_ccd = g.friction_ccd;
+    P.fr_prox = g.friction_proximity;
+    P.vel_factor = g.collider_velocity_factor;
+    if (!(P.dt > 0.0f)) return fail(c, MPM_E_RANGE, "simulationTimestep must be > 0");
+    if (g.pbmpm_iterations < 1) return fail(c, MPM_E_RANGE, "pbmpm_iterations must be >= 1");
+    return MPM_OK;
+}
+
+uint32_t grid_for(uint64_t work, uint32_t cap_wg) {
+    uint64_t g = (work + 255) / 256;
+    if (g < 1) g = 1;
+    if (g > cap_wg) g = cap_wg;
+    return (uint32_t)g;
+}
+
+// One substep (or a debug prefix of it) enqueued on the stream. Returns the stage reached.
+void enqueue_binning(mpm_ctx* c, const SimParams& P, const Bufs& B) {
+    hipStream_t s = c->stream;
+    const uint32_t gk = grid_for(std::max<uint64_t>(c->n, c->cap), 8192);
+    k_keys_count<<<gk, 256, 0, s>>>(P, B);
+    k_scan_tiles<<<c->ntiles, 256, 0, s>>>(P, B);
+    k_scan_top<<<1, 256, 0, s>>>(P, B, c->ntiles);
+    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P, B);
+    k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P, B);
+    const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 4096);
+    k_insert<<<gi, 256, 0, s>>>(P, B);
+    k_compact<<<grid_for(c->cap, 1u << 30), 256, 0, s>>>(P, B);
+    k_nbr<<<gi, 256, 0, s>>>(P, B);
+}
+
+uint32_t chunk_grid(mpm_ctx* c) {
+    uint32_t g = c->cfg.grid_workgroups ? c->cfg.grid_workgroups : (uint32_t)c->n_cu * 4u;
+    return std::min<uint32_t>(g, std::max<uint32_t>(c->max_chunks, 1u));
+}
+
+// stage < 0: full substep. Returns true if the substep completed (buffers swap).
+// `full` records the phase events of mpm_get_timings.
+bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool full = false) {
+    hipStream_t s = c->stream;
+    Bufs B = make_bufs(c);
+    if (full) hipEventRecord(c->ev[0], s);
+    enqueue_binning(c, P, B);
+    if (full) hipEventRecord(c->ev[1], s);
+    if (stage == 0) return false;
+    const uint32_t G = chunk_grid(c);
+    const uint32_t gs = 256;
+    if (method == MPM_METHOD_PBMPM) {
+        const int k = c->cfg.pbmpm_iterations;
+        k_pbmpm<PB_FIRST><<<G, 256, 0, s>>>(P, B, -1, 0, 1);
+        c->last_grid = 0;
+        if (stage == 1) return false;
+        for (int i = 1; i < k; ++i) {
+            k_pbmpm<PB_MID><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3);
+            c->last_grid = i % 3;
+            if (stage == i + 1) return false;
+        }
+        if (full) hipEventRecord(c->ev[2], s);
+        k_pbmpm<PB_LAST><<<G, 256, 0, s>>>(P, B, (k - 1) % 3, -1, -1);
+    } else {
+        k_explicit_p2g<0><<<G, 256, 0, s>>>(P, B);
+        c->last_grid = 0;
+        if (stage == 1) return false;
+        k_grid_update<<<G, 256, 0, s>>>(P, B);
+        if (stage == 2) return false;
+        if (full) hipEventRecord(c->ev[2], s);
+        k_explicit_g2p<0><<<G, 256, 0, s>>>(P, B);
+    }
+    k_straggle<<<gs, 256, 0, s>>>(P, B);
+    if (full) hipEventRecord(c->ev[3], s);
+    return true;
+}
+
+int ensure_graph(mpm_ctx* c, int method, const SimParams& P) {
+    const int par = c->cur;
+    if (c->gexec[method][par] && std::memcmp(&c->gparams[method][par], &P, sizeof(SimParams)) == 0) return MPM_OK;
+    if (c->gexec[method][par]) { hipGraphExecDestroy(c->gexec[method][par]); c->gexec[method][par] = nullptr; }
+    if (c->graph[method][par]) { hipGraphDestroy(c->graph[method][par]); c->graph[method][par] = nullptr; }
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_substep(c, method, P, -1, false);
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (r != hipSuccess) { hipGraphDestroy(g); return fail(c, MPM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(r)); }
+    c->graph[method][par] = g;
+    c->gexec[method][par] = e;
+    c->gparams[method][par] = P;
+    return MPM_OK;
+}
+
+// ------------------------------------------------------------------ BVH (GpuColliderBufferManager.ts:222-350)
+struct BTri { double cen[3], mn[3], mx[3]; };
+struct BNode { double mn[3], mx[3]; uint32_t left, right; bool leaf; };
+
+uint32_t bvh_rec(const std::vector<BTri>& tris, std::vector<uint32_t>& idx, uint32_t start, uint32_t end, std::vector<BNode>& nodes) {
+    const uint32_t ni = (uint32_t)nodes.size();
+    const uint32_t count = end - start;
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = start; i < end; ++i) {
+        const BTri& t = tris[idx[i]];
+        for (int a = 0; a < 3; ++a) { mn[a] = std::min(mn[a], t.mn[a]); mx[a] = std::max(mx[a], t.mx[a]); }
+    }
+    BNode nd;
+    for (int a = 0; a < 3; ++a) { nd.mn[a] = mn[a]; nd.mx[a] = mx[a]; }
+    if (count <= 4) {
+        nd.left = start; nd.right = count; nd.leaf = true;
+        nodes.push_back(nd);
+        return ni;
+    }
+    const double ex = mx[0] - mn[0], ey = mx[1] - mn[1], ez = mx[2] - mn[2];
+    int axis = 0;
+    if (ey > ex && ey > ez) axis = 1;
+    else if (ez > ex && ez > ey) axis = 2;
+    std::stable_sort(idx.begin() + start, idx.begin() + end,
+                     [&](uint32_t a, uint32_t b) { return tris[a].cen[axis] < tris[b].cen[axis]; });
+    const uint32_t mid = start + count / 2;
+    nd.left = 0; nd.right = 0; nd.leaf = false;
+    nodes.push_back(nd);
+    const uint32_t l = bvh_rec(tris, idx, start, mid, nodes);
+    const uint32_t r = bvh_rec(tris, idx, mid, end, nodes);
+    nodes[ni].left = l; nodes[ni].right = r;
+    return ni;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mpm_version(void) { return k_version; }
+
+void mpm_config_defaults(mpm_config* g) {
+    if (!g) return;
+    std::memset(g, 0, sizeof(*g));
+    g->n_particles = 0;
+    g->grid_res[0] = g->grid_res[1] = g->grid_res[2] = 256;   // SimulationState.svelte.ts:22-24
+    for (int a = 0; a < 3; ++a) { g->grid_min[a] = -5.0f; g->grid_max[a] = 5.0f; }   // GpuSnowPipelineRunner.svelte.ts:133-134
+    g->fixed_point_scale = 65536.0f;
+    g->dt = 1.0f / 384.0f;                  // SimulationState.svelte.ts:26 (PBMPM)
+    g->max_blocks = 0;
+    g->hash_map_size = 0;
+    g->max_probes = 100;
+    g->pbmpm_iterations = 3;
+    g->pbmpm_force_divisor = 3.0f;
+    g->gravity[0] = 0.0f; g->gravity[1] = 0.0f; g->gravity[2] = -9.81f;
+    g->youngs_modulus = 1.4e5;
+    g->poissons_ratio = 0.2;
+    g->hardening_explicit = 5.0f;
+    g->hardening_pbmpm = 20.0f;
+    g->pbmpm_blend = 0.95f;
+    g->pbmpm_volume_exponent = -0.3333333f;
+    g->theta_c = 2.5e-2;
+    g->theta_s = 7.5e-3;
+    g->density = 400.0;
+    g->grad_scale = 1.0f;
+    g->collider_margin = 0.05f;
+    g->friction_ccd = 0.1f;
+    g->friction_proximity = 0.0f;
+    g->collider_velocity_factor = 0.2f;
+    g->wall_mode = 0;
+    g->wall_cells = 3;
+    g->chunk_particles = 0;
+    g->grid_workgroups = 0;
+    g->device = 0;
+    g->use_graph = 1;
+}
+
+static void release(mpm_ctx* c) {
+    for (int m = 0; m < 2; ++m) for (int p = 0; p < 2; ++p) {
+        if (c->gexec[m][p]) hipGraphExecDestroy(c->gexec[m][p]);
+        if (c->graph[m][p]) hipGraphDestroy(c->graph[m][p]);
+    }
+    for (int i = 0; i < 4; ++i) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    for (void* p : c->allocs) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+}
+
+int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
+    if (!cfg || !out) return MPM_E_INVALID;
+    *out = nullptr;
+    if (cfg->n_particles == 0) return MPM_E_INVALID;
+    for (int a = 0; a < 3; ++a) if (cfg->grid_res[a] == 0 || cfg->grid_res[a] > 65536 || !(cfg->grid_max[a] > cfg->grid_min[a])) return MPM_E_INVALID;
+    if (cfg->pbmpm_iterations < 1 || cfg->pbmpm_iterations > 64) return MPM_E_INVALID;
+    mpm_ctx* c = new mpm_ctx();
+    c->cfg = *cfg;
+    c->device = cfg->device;
+    if (hipSetDevice(c->device) != hipSuccess) { (void)hipGetLastError(); delete c; return MPM_E_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { (void)hipGetLastError(); delete c; return MPM_E_HIP; }
+    c->n = cfg->n_particles;
+    c->stride = (c->n + 63u) & ~63u;
+    for (int a = 0; a < 3; ++a) c->nb[a] = (cfg->grid_res[a] + 3u) / 4u;
+    uint64_t K = (uint64_t)c->nb[0] * c->nb[1] * c->nb[2];
+    if (K >= 0xFFFFFFF0ull) { release(c); delete c; return MPM_E_RANGE; }
+    c->K = (uint32_t)K;
+    c->chunk = cfg->chunk_particles ? cfg->chunk_particles : 512u;
+    uint64_t mb = cfg->max_blocks ? cfg->max_blocks : std::min<uint64_t>(K, std::max<uint64_t>(8ull * c->n, 64));
+    c->max_blocks = (uint32_t)mb;
+    c->cap = cfg->hash_map_size ? cfg->hash_map_size : next_prime((uint32_t)std::min<uint64_t>(2ull * mb + 1, 0xFFFFFF00ull));
+    c->max_runs = (uint32_t)std::min<uint64_t>(c->n, K);
+    c->max_chunks = (c->n + c->chunk - 1) / c->chunk + c->max_runs;
+    c->ntiles = (uint32_t)((K + 1023) / 1024);
+    int rc = MPM_OK;
+    const size_t ps = (size_t)NPLANES * c->stride;
+    if ((rc = dalloc(c, &c->planes[0], ps)) || (rc = dalloc(c, &c->planes[1], ps)) ||
+        (rc = dalloc(c, &c->pid[0], c->stride)) || (rc = dalloc(c, &c->pid[1], c->stride)) ||
+        (rc = dalloc(c, &c->relax, c->stride)) || (rc = dalloc(c, &c->key, c->stride)) ||
+        (rc = dalloc(c, &c->mask, c->stride)) || (rc = dalloc(c, &c->perm, c->stride)) ||
+        (rc = dalloc(c, &c->kcount, K)) || (rc = dalloc(c, &c->kmask, K)) || (rc = dalloc(c, &c->kcursor, K)) ||
+        (rc = dalloc(c, &c->tile_sums, 3 * (size_t)c->ntiles)) || (rc = dalloc(c, &c->tile_ofs, 3 * (size_t)c->ntiles)) ||
+        (rc = dalloc(c, &c->run_key, c->max_runs)) || (rc = dalloc(c, &c->run_start, c->max_runs)) ||
+        (rc = dalloc(c, &c->run_count, c->max_runs)) || (rc = dalloc(c, &c->run_mask, c->max_runs)) ||
+        (rc = dalloc(c, &c->chunks, c->max_chunks)) ||
+        (rc = dalloc(c, &c->table, c->cap)) || (rc = dalloc(c, &c->slot_val, c->cap)) ||
+        (rc = dalloc(c, &c->nbr_slot, 27 * (size_t)c->max_runs)) || (rc = dalloc(c, &c->nbr, 27 * (size_t)c->max_runs)) ||
+        (rc = dalloc(c, &c->block_key, c->max_blocks)) ||
+        (rc = dalloc(c, &c->grid[0], 64 * (size_t)c->max_blocks)) || (rc = dalloc(c, &c->grid[1], 64 * (size_t)c->max_blocks)) ||
+        (rc = dalloc(c, &c->grid[2], 64 * (size_t)c->max_blocks)) ||
+        (rc = dalloc(c, &c->counters, C_NCOUNTERS))) {
+        release(c); delete c; return rc;
+    }
+    // zero-initialised key-space accumulators and counters (kernels keep them zero afterwards)
+    k_fill_u32<<<grid_for(K, 4096), 256, 0, c->stream>>>(c->kcount, 0u, (uint32_t)K);
+    k_fill_u32<<<grid_for(K, 4096), 256, 0, c->stream>>>(c->kmask, 0u, (uint32_t)K);
+    k_fill_u32<<<1, 256, 0, c->stream>>>(c->counters, 0u, C_NCOUNTERS);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { (void)hipGetLastError(); release(c); delete c; return MPM_E_HIP; }
+    for (int i = 0; i < 4; ++i) hipEventCreate(&c->ev[i]);
+    // uniform fields the runner writes at construction (GpuSnowPipelineRunner.svelte.ts:136-144,171-177)
+    std::memset(c->uni, 0, sizeof(c->uni));
+    wru<float>(c->uni, 0, cfg->dt);
+    wru<float>(c->uni, 4, cfg->fixed_point_scale);
+    for (int a = 0; a < 3; ++a) {
+        wru<float>(c->uni, 16 + 4 * a, cfg->grid_min[a]);
+        wru<float>(c->uni, 32 + 4 * a, cfg->grid_max[a]);
+        wru<uint32_t>(c->uni, 208 + 4 * a, cfg->grid_res[a]);
+        wru<float>(c->uni, 352 + 4 * a, (float)(((double)cfg->grid_max[a] - (double)cfg->grid_min[a]) / (double)cfg->grid_res[a]));
+    }
+    for (int i = 0; i < 4; ++i) { wru<float>(c->uni, 256 + 20 * i, 1.0f); wru<float>(c->uni, 432 + 20 * i, 1.0f); }
+    *out = c;
+    return MPM_OK;
+}
+
+void mpm_destroy(mpm_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    release(c);
+    delete c;
+}
+
+const char* mpm_last_error(mpm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+void* mpm_stream(mpm_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int mpm_write_uniforms(mpm_ctx* c, size_t offset, const void* bytes, size_t len) {
+    if (!c || (!bytes && len)) return MPM_E_INVALID;
+    if (offset > MPM_UNIFORMS_BYTES || len > MPM_UNIFORMS_BYTES - offset) return fail(c, MPM_E_RANGE, "uniform write out of the 37376-byte buffer");
+    std::memcpy(c->uni + offset, bytes, len);
+    return MPM_OK;
+}
+int mpm_read_uniforms(mpm_ctx* c, size_t offset, void* bytes, size_t len) {
+    if (!c || (!bytes && len)) return MPM_E_INVALID;
+    if (offset > MPM_UNIFORMS_BYTES || len > MPM_UNIFORMS_BYTES - offset) return fail(c, MPM_E_RANGE, "uniform read out of the 37376-byte buffer");
+    std::memcpy(bytes, c->uni + offset, len);
+    return MPM_OK;
+}
+
+int mpm_set_collider(mpm_ctx* c, const uint32_t* data, size_t n_words, uint32_t n_indices, const void* nodes, size_t n_nodes) {
+    if (!c) return MPM_E_INVALID;
+    if ((n_words && !data) || (n_nodes && !nodes)) return fail(c, MPM_E_INVALID, "null collider pointer");
+    if (n_indices > n_words || n_indices % 3) return fail(c, MPM_E_INVALID, "collider index count must be a multiple of 3 within the data");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // validate the node array before it can steer a device traversal
+    const uint32_t* w = (const uint32_t*)nodes;
+    const size_t n_tris = n_indices / 3;
+    for (size_t i = 0; i < n_nodes; ++i) {
+        const uint32_t l = w[8 * i + 3], r = w[8 * i + 7];
+        if (r & 0x80000000u) { if ((uint64_t)l + (r & 0x7FFFFFFFu) > n_tris) return fail(c, MPM_E_INVALID, "BVH leaf range outside the triangle list"); }
+        else if (l >= n_nodes || r >= n_nodes) return fail(c, MPM_E_INVALID, "BVH child index out of range");
+    }
+    const size_t n_vert_words = n_words - n_indices;
+    for (size_t i = 0; i < n_indices; ++i) if ((uint64_t)data[i] * 3 + 2 >= n_vert_words) return fail(c, MPM_E_INVALID, "collider index beyond the vertex data");
+    if (c->coll_data) { hipFree(c->coll_data); c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->coll_data)); c->coll_data = nullptr; }
+    if (c->bvh) { hipFree(c->bvh); c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->bvh)); c->bvh = nullptr; }
+    c->n_coll_words = 0; c->n_bvh_nodes = 0;
+    int rc;
+    if (n_words && n_nodes) {
+        if ((rc = dalloc(c, &c->coll_data, n_words))) return rc;
+        if ((rc = dalloc(c, &c->bvh, 2 * n_nodes))) return rc;
+        HIPCHK(c, hipMemcpy(c->coll_data, data, n_words * 4, hipMemcpyHostToDevice));
+        HIPCHK(c, hipMemcpy(c->bvh, nodes, n_nodes * 32, hipMemcpyHostToDevice));
+        c->n_coll_words = n_words; c->n_bvh_nodes = n_nodes;
+    }
+    wru<uint32_t>(c->uni, 348, n_indices);
+    return MPM_OK;
+}
+
+int mpm_build_bvh(const float* pos, size_t n_vertices, const uint32_t* indices, size_t n_indices,
+                  void* nodes_out, size_t nodes_cap, size_t* n_nodes, uint32_t* tri_order) {
+    if (!n_nodes || (n_indices && (!pos || !indices)) || n_indices % 3) return MPM_E_INVALID;
+    const size_t nt = n_indices / 3;
+    std::vector<BNode> nodes;
+    std::vector<uint32_t> idx(nt);
+    if (nt == 0) {
+        BNode d{}; d.leaf = true; d.left = 0; d.right = 0;
+        nodes.push_back(d);
+    } else {
+        std::vector<BTri> tris(nt);
+        for (size_t t = 0; t < nt; ++t) {
+            double v[3][3];
+            for (int k = 0; k < 3; ++k) {
+                uint32_t vi = indices[3 * t + k];
+                if (vi >= n_vertices) return MPM_E_INVALID;
+                for (int a = 0; a < 3; ++a) v[k][a] = (double)pos[3 * (size_t)vi + a];
+            }
+            for (int a = 0; a < 3; ++a) {
+                tris[t].mn[a] = std::min(std::min(v[0][a], v[1][a]), v[2][a]);
+                tris[t].mx[a] = std::max(std::max(v[0][a], v[1][a]), v[2][a]);
+                tris[t].cen[a] = (v[0][a] + v[1][a] + v[2][a]) / 3.0;
+            }
+            idx[t] = (uint32_t)t;
+        }
+        bvh_rec(tris, idx, 0, (uint32_t)nt, nodes);
+    }
+    *n_nodes = nodes.size();
+    if (!nodes_out || nodes_cap < nodes.size()) return MPM_E_RANGE;
+    uint8_t* o = (uint8_t*)nodes_out;
+    for (size_t i = 0; i < nodes.size(); ++i) {
+        const BNode& nd = nodes[i];
+        float f[6] = {(float)nd.mn[0], (float)nd.mn[1], (float)nd.mn[2], (float)nd.mx[0], (float)nd.mx[1], (float)nd.mx[2]};
+        uint32_t r = nd.leaf ? (nd.right | 0x80000000u) : nd.right;
+        std::memcpy(o + 32 * i, f, 12); std::memcpy(o + 32 * i + 12, &nd.left, 4);
+        std::memcpy(o + 32 * i + 16, f + 3, 12); std::memcpy(o + 32 * i + 28, &r, 4);
+    }
+    if (tri_order) for (size_t t = 0; t < nt; ++t) tri_order[t] = idx[t];
+    return MPM_OK;
+}
+
+int mpm_upload_particles(mpm_ctx* c, const void* aos, size_t n) {
+    if (!c || !aos) return MPM_E_INVALID;
+    if (n != c->n) return fail(c, MPM_E_INVALID, "particle count differs from the context's n_particles");
+    HIPCHK(c, hipSetDevice(c->device));
+    float* tmp = nullptr;
+    HIPCHK(c, hipMalloc(&tmp, n * MPM_PARTICLE_BYTES));
+    hipError_t e = hipMemcpyAsync(tmp, aos, n * MPM_PARTICLE_BYTES, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        k_upload<<<grid_for(n, 1u << 30), 256, 0, c->stream>>>(tmp, c->planes[c->cur], c->pid[c->cur], c->n, c->stride);
+        e = hipStreamSynchronize(c->stream);
+    }
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(c, MPM_E_HIP, std::string("particle upload: ") + hipGetErrorString(e));
+    c->has_particles = true;
+    return MPM_OK;
+}
+
+int mpm_download_particles(mpm_ctx* c, void* aos, size_t n) {
+    if (!c || !aos) return MPM_E_INVALID;
+    if (n != c->n) return fail(c, MPM_E_INVALID, "particle count differs from the context's n_particles");
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    float* tmp = nullptr;
+    HIPCHK(c, hipMalloc(&tmp, n * MPM_PARTICLE_BYTES));
+    k_download<<<grid_for(n, 1u << 30), 256, 0, c->stream>>>(c->planes[c->cur], c->pid[c->cur], tmp, c->n, c->stride);
+    hipError_t e = hipMemcpyAsync(aos, tmp, n * MPM_PARTICLE_BYTES, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(c, MPM_E_HIP, std::string("particle download: ") + hipGetErrorString(e));
+    return MPM_OK;
+}
+
+int mpm_step(mpm_ctx* c, int method, int n_substeps) {
+    if (!c) return MPM_E_INVALID;
+    if (method != MPM_METHOD_EXPLICIT && method != MPM_METHOD_PBMPM) return fail(c, MPM_E_INVALID, "unknown method");
+    if (n_substeps < 0) return fail(c, MPM_E_INVALID, "negative substep count");
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    SimParams P;
+    int rc = make_params(c, P);
+    if (rc) return rc;
+    wru<uint32_t>(c->uni, 8, method == MPM_METHOD_PBMPM ? 1u : 0u);   // writeUsePbmpm
+    for (int s = 0; s < n_substeps; ++s) {
+        const bool timed = c->timing && s == n_substeps - 1;
+        if (c->cfg.use_graph && !timed) {
+            if ((rc = ensure_graph(c, method, P))) return rc;
+            HIPCHK(c, hipGraphLaunch(c->gexec[method][c->cur], c->stream));
+        } else {
+            enqueue_substep(c, method, P, -1, timed);
+            HIPCHK(c, hipGetLastError());
+        }
+        c->cur ^= 1;
+        c->substeps++;
+        if (timed) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            float a = 0, b = 0, d = 0;
+            hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+            hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+            hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
+            c->tm.binning_ns = a * 1e6; c->tm.transfer_ns = b * 1e6; c->tm.integrate_ns = d * 1e6;
+            c->tm.compute_simulation_step_ns = (a + b + d) * 1e6;
+            c->tm.substeps = c->substeps;
+        }
+    }
+    return MPM_OK;
+}
+
+int mpm_sync(mpm_ctx* c) {
+    if (!c) return MPM_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+int mpm_debug_stage(mpm_ctx* c, int method, int stage) {
+    if (!c) return MPM_E_INVALID;
+    if (method != MPM_METHOD_EXPLICIT && method != MPM_METHOD_PBMPM) return fail(c, MPM_E_INVALID, "unknown method");
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    SimParams P;
+    int rc = make_params(c, P);
+    if (rc) return rc;
+    bool done = enqueue_substep(c, method, P, stage < 0 ? 1 << 30 : stage);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (done) { c->cur ^= 1; c->substeps++; }
+    return MPM_OK;
+}
+
+static int read_counters(mpm_ctx* c, uint32_t* h) {
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(h, c->counters, C_NCOUNTERS * 4, hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+
+int mpm_dump_grid(mpm_ctx* c, int32_t* xyz, int32_t* vals, size_t cap, size_t* n_out) {
+    if (!c || !n_out) return MPM_E_INVALID;
+    uint32_t h[C_NCOUNTERS];
+    int rc = read_counters(c, h);
+    if (rc) return rc;
+    const uint32_t nb = std::min(h[C_NBLOCKS], c->max_blocks);
+    *n_out = (size_t)nb * 64;
+    if (!xyz || !vals) return MPM_OK;
+    if (cap < (size_t)nb * 64) return fail(c, MPM_E_RANGE, "dump capacity too small");
+    std::vector<uint32_t> keys(nb);
+    std::vector<int4> g((size_t)nb * 64);
+    if (nb) {
+        HIPCHK(c, hipMemcpy(keys.data(), c->block_key, nb * 4, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(g.data(), c->grid[c->last_grid], (size_t)nb * 64 * sizeof(int4), hipMemcpyDeviceToHost));
+    }
+    SimParams P;
+    if ((rc = make_params(c, P))) return rc;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t k = keys[b];
+        const int bx = (int)(k % (uint32_t)P.nb[0]), by = (int)((k / (uint32_t)P.nb[0]) % (uint32_t)P.nb[1]), bz = (int)(k / ((uint32_t)P.nb[0] * (uint32_t)P.nb[1]));
+        for (uint32_t l = 0; l < 64; ++l) {
+            const size_t i = (size_t)b * 64 + l;
+            xyz[3 * i] = 4 * bx + (int)(l & 3u); xyz[3 * i + 1] = 4 * by + (int)((l >> 2) & 3u); xyz[3 * i + 2] = 4 * bz + (int)(l >> 4);
+            vals[4 * i] = g[i].x; vals[4 * i + 1] = g[i].y; vals[4 * i + 2] = g[i].z; vals[4 * i + 3] = g[i].w;
+        }
+    }
+    return MPM_OK;
+}
+
+int mpm_dump_blocks(mpm_ctx* c, int32_t* xyz, size_t cap, size_t* n_out) {
+    if (!c || !n_out) return MPM_E_INVALID;
+    uint32_t h[C_NCOUNTERS];
+    int rc = read_counters(c, h);
+    if (rc) return rc;
+    const uint32_t nb = std::min(h[C_NBLOCKS], c->max_blocks);
+    *n_out = nb;
+    if (!xyz) return MPM_OK;
+    if (cap < nb) return fail(c, MPM_E_RANGE, "dump capacity too small");
+    std::vector<uint32_t> keys(nb);
+    if (nb) HIPCHK(c, hipMemcpy(keys.data(), c->block_key, nb * 4, hipMemcpyDeviceToHost));
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t k = keys[b];
+        xyz[3 * b] = (int)(k % c->nb[0]); xyz[3 * b + 1] = (int)((k / c->nb[0]) % c->nb[1]); xyz[3 * b + 2] = (int)(k / (c->nb[0] * c->nb[1]));
+    }
+    return MPM_OK;
+}
+
+int mpm_dump_particle_blocks(mpm_ctx* c, int32_t* xyz, uint32_t* counts, size_t cap, size_t* n_out) {
+    if (!c || !n_out) return MPM_E_INVALID;
+    uint32_t h[C_NCOUNTERS];
+    int rc = read_counters(c, h);
+    if (rc) return rc;
+    const uint32_t nr = h[C_NRUNS];
+    *n_out = nr;
+    if (!xyz || !counts) return MPM_OK;
+    if (cap < nr) return fail(c, MPM_E_RANGE, "dump capacity too small");
+    std::vector<uint32_t> keys(nr), cnt(nr);
+    if (nr) {
+        HIPCHK(c, hipMemcpy(keys.data(), c->run_key, nr * 4, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(cnt.data(), c->run_count, nr * 4, hipMemcpyDeviceToHost));
+    }
+    for (uint32_t r = 0; r < nr; ++r) {
+        const uint32_t k = keys[r];
+        xyz[3 * r] = (int)(k % c->nb[0]); xyz[3 * r + 1] = (int)((k / c->nb[0]) % c->nb[1]); xyz[3 * r + 2] = (int)(k / (c->nb[0] * c->nb[1]));
+        counts[r] = cnt[r];
+    }
+    return MPM_OK;
+}
+
+int mpm_get_stats(mpm_ctx* c, mpm_stats* out) {
+    if (!c || !out) return MPM_E_INVALID;
+    uint32_t h[C_NCOUNTERS];
+    int rc = read_counters(c, h);
+    if (rc) return rc;
+    std::memset(out, 0, sizeof(*out));
+    out->n_particles = c->n;
+    out->particles_binned = h[C_NVALID];
+    out->particles_unbinned = c->n - h[C_NVALID];
+    out->particle_blocks = h[C_NRUNS];
+    out->work_chunks = h[C_NCHUNKS];
+    out->blocks_allocated = std::min(h[C_NBLOCKS], c->max_blocks);
+    out->blocks_dropped = h[C_DROPS];
+    out->probe_failures = h[C_PROBEFAIL];
+    out->substeps = c->substeps;
+    return MPM_OK;
+}
+
+int mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n) {
+    if (kind < 0 || kind > 9 || !a || !out || n == 0 || n > (1u << 24)) return MPM_E_INVALID;
+    if ((kind == 2 || kind == 4 || kind == 5) && !b) return MPM_E_INVALID;
+    const size_t in_w = (kind >= 6 && kind <= 8) ? 9 : 1;
+    const size_t out_w = (kind == 7 || kind == 8) ? 9 : 1;
+    float *da = nullptr, *db = nullptr, *dout = nullptr;
+    if (hipMalloc(&da, n * in_w * 4) != hipSuccess) return MPM_E_NOMEM;
+    if (b && hipMalloc(&db, n * 4) != hipSuccess) { hipFree(da); return MPM_E_NOMEM; }
+    if (hipMalloc(&dout, n * out_w * 4) != hipSuccess) { hipFree(da); hipFree(db); return MPM_E_NOMEM; }
+    hipError_t e = hipMemcpy(da, a, n * in_w * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && b) e = hipMemcpy(db, b, n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_selftest<<<(uint32_t)((n + 255) / 256), 256>>>(kind, da, db, dout, (uint32_t)n);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, n * out_w * 4, hipMemcpyDeviceToHost);
+    hipFree(da); if (db) hipFree(db); hipFree(dout);
+    return e == hipSuccess ? MPM_OK : MPM_E_HIP;
+}
+
+int mpm_set_timing(mpm_ctx* c, int enabled) {
+    if (!c) return MPM_E_INVALID;
+    c->timing = enabled != 0;
+    return MPM_OK;
+}
+int mpm_get_timings(mpm_ctx* c, mpm_timings* out) {
+    if (!c || !out) return MPM_E_INVALID;
+    *out = c->tm;
+    return MPM_OK;
+}
+
+}  // extern "C"

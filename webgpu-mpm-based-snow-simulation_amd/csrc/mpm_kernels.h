@@ -1,0 +1,1052 @@
+// Hand-written gfx950 kernels of the MPM substep.
+//
+// Data layout in HBM (one GPU, everything resident):
+//   particle state  37 FP32 planes (SoA, plane-major, stride = padded N) kept in *sorted* order:
+//                   pos xyz, mass, vel xyz, dx xyz, F_e[9], F_p[9], D[9] (matrices column-major),
+//                   plus a u32 plane of original particle ids. Two copies (cur / nxt): each
+//                   substep's first transfer pass gathers cur through the sort permutation and
+//                   writes nxt, so every later pass streams contiguous, coalesced planes.
+//   binning         key space = all 4^3 blocks of the grid (linearised block coordinate); per-key
+//                   counts / OR of touched-neighbour masks, one scan over the key space yields
+//                   run (particle-block) descriptors, work chunks and scatter cursors.
+//   block table     open-addressing hash (hash3 of the block coordinate, linear probing,
+//                   atomicCAS insert) compacted to dense block indices.
+//   grid            int4 per cell {mass, px, py, pz} i32 fixed point, 64 cells per block,
+//                   three buffers so PBMPM iteration i reads grid i-1, writes grid i, and clears
+//                   the buffer iteration i+1 will write.
+// Reference passes (src/lib/gpu/mpm/*.wgsl) and the kernels that replace them:
+//   clearHashMap, mapAffectedBlocks                 -> k_keys_count (table clear), k_insert, k_compact
+//   clearBlockParticleCounts, countParticlesPerBlock,
+//   computeBlockOffsets (1 thread), binParticles    -> k_keys_count, k_scan_*, k_scatter
+//   clearMappedBlocks (whole capacity)              -> k_nbr (active blocks only), fused clears
+//   pbmpm (solve+P2G+gravity) and doGridToParticle  -> k_pbmpm<FIRST|MID|LAST> (LDS-tiled)
+//   doParticleToGrid / doGridUpdate / doGridToParticle (explicit) -> k_explicit_p2g / k_grid_update / k_explicit_g2p
+//   integrateParticles                              -> fused into k_pbmpm<LAST>, k_explicit_g2p, k_straggle
+#pragma once
+#include "mpm_math.h"
+
+namespace mpm {
+
+enum : uint32_t { EMPTY = 0xFFFFFFFFu };
+
+enum Plane : int {
+    PX = 0, PY, PZ, PM, VX, VY, VZ, DXX, DXY, DXZ,
+    FE = 10, FP = 19, DD = 28, NPLANES = 37
+};
+
+enum Counter : int {
+    C_NVALID = 0, C_NRUNS, C_NCHUNKS, C_NBLOCKS, C_STRAG, C_DROPS, C_PROBEFAIL, C_SUBSTEPS, C_NCOUNTERS = 16
+};
+
+struct SimParams {
+    float dt, S;
+    float gmin[3], gmax[3], cd[3];
+    int32_t res[3];
+    int32_t nb[3];
+    uint32_t K;                // number of blocks in the block grid (key space)
+    uint32_t n;                // particles
+    uint32_t chunk;            // particles per work chunk
+    uint32_t cap;              // hash table slots
+    uint32_t max_blocks, max_probes;
+    // material / method
+    float mu0, la0, xi_e, xi_p, blend, one_minus_blend, vol_exp, plast_lo, plast_hi, inv_density, grad_scale;
+    float force_div, grav[3];
+    uint32_t wall_mode, wall_cells;
+    // collider (uniforms @224..@432)
+    float cmin[3], cmax[3], T[16], Tinv[16], cvel[3];
+    uint32_t coll_num_indices, coll_num_objects, n_bvh_nodes, has_collider;
+    float margin, fr_ccd, fr_prox, vel_factor;
+    // pointer interaction (uniforms @496..@528)
+    float ip[3], istrength, iradius;
+    uint32_t interacting, imode;
+};
+
+struct Bufs {
+    float* cur; float* nxt; uint32_t* pid_cur; uint32_t* pid_nxt; uint32_t stride;
+    float* relax;
+    uint32_t* key; uint32_t* mask; uint32_t* perm;
+    uint32_t* kcount; uint32_t* kmask; uint32_t* kcursor;
+    uint32_t* tile_sums; uint32_t* tile_ofs;     // 3 per tile
+    uint32_t* run_key; uint32_t* run_start; uint32_t* run_count; uint32_t* run_mask;
+    uint2* chunks;
+    uint32_t* table; uint32_t* slot_val; uint32_t* nbr_slot; uint32_t* nbr; uint32_t* block_key;
+    int4* grid0; int4* grid1; int4* grid2;
+    uint32_t* counters;
+    const uint32_t* coll_data; const uint4* bvh;
+};
+
+MPM_DEV int4* grid_sel(const Bufs& B, int i) { return i == 0 ? B.grid0 : (i == 1 ? B.grid1 : B.grid2); }
+
+// ------------------------------------------------------------------ particle record in registers
+struct Part {
+    v3 pos, vel; float mass;
+    m3 Fe, Fp, D;
+    v3 dx;
+};
+
+MPM_DEV float ld(const float* s, uint32_t stride, int plane, uint32_t i) { return s[(size_t)plane * stride + i]; }
+MPM_DEV void st(float* s, uint32_t stride, int plane, uint32_t i, float v) { s[(size_t)plane * stride + i] = v; }
+MPM_DEV v3 ld3(const float* s, uint32_t stride, int plane, uint32_t i) {
+    return mk(ld(s, stride, plane, i), ld(s, stride, plane + 1, i), ld(s, stride, plane + 2, i));
+}
+MPM_DEV void st3(float* s, uint32_t stride, int plane, uint32_t i, v3 v) {
+    st(s, stride, plane, i, v.x); st(s, stride, plane + 1, i, v.y); st(s, stride, plane + 2, i, v.z);
+}
+MPM_DEV m3 ldm(const float* s, uint32_t stride, int plane, uint32_t i) {
+    m3 m; m.c[0] = ld3(s, stride, plane, i); m.c[1] = ld3(s, stride, plane + 3, i); m.c[2] = ld3(s, stride, plane + 6, i); return m;
+}
+MPM_DEV void stm(float* s, uint32_t stride, int plane, uint32_t i, const m3& m) {
+    st3(s, stride, plane, i, m.c[0]); st3(s, stride, plane + 3, i, m.c[1]); st3(s, stride, plane + 6, i, m.c[2]);
+}
+
+// ------------------------------------------------------------------ mpmOps.wgsl:14-53
+MPM_DEV void cell_number(const SimParams& P, v3 pos, int c[3]) {
+    v3 q = cdiv(pos - mk(P.gmin[0], P.gmin[1], P.gmin[2]), mk(P.cd[0], P.cd[1], P.cd[2]));
+    c[0] = f2i(q.x); c[1] = f2i(q.y); c[2] = f2i(q.z);
+}
+MPM_DEV bool in_range(const SimParams& P, int x, int y, int z) {
+    return x >= 0 && y >= 0 && z >= 0 && x < P.res[0] && y < P.res[1] && z < P.res[2];
+}
+MPM_DEV v3 frac_pos(const SimParams& P, v3 pos, const int c[3]) {
+    v3 cdv = mk(P.cd[0], P.cd[1], P.cd[2]);
+    v3 minPos = mk(P.gmin[0], P.gmin[1], P.gmin[2]) + cmul(cdv, mk((float)c[0], (float)c[1], (float)c[2]));
+    return cdiv(pos - minPos, cdv);
+}
+MPM_DEV void bspline(v3 f, v3 w[3]) {
+    v3 a = mk(1.0f, 1.0f, 1.0f) - f;
+    w[0] = cmul(0.5f * a, a);
+    v3 b = f - mk(0.5f, 0.5f, 0.5f);
+    w[1] = mk(0.75f, 0.75f, 0.75f) - cmul(b, b);
+    w[2] = cmul(0.5f * f, f);
+}
+MPM_DEV void bspline_d(v3 f, v3 d[3]) {
+    d[0] = f - mk(1.0f, 1.0f, 1.0f);
+    d[1] = -2.0f * (f - mk(0.5f, 0.5f, 0.5f));
+    d[2] = f;
+}
+MPM_DEV float comp(v3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// 27-bit set of neighbour-block offsets touched by the in-range 3x3x3 stencil of cell c
+// (bit (ox+1) + 3(oy+1) + 9(oz+1)); same block set mapAffectedBlocks.wgsl:75-84 allocates.
+MPM_DEV uint32_t touched_mask(const SimParams& P, const int c[3]) {
+    uint32_t s[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        uint32_t m = 0;
+        const int own = c[a] >> 2;
+#pragma unroll
+        for (int d = -1; d <= 1; ++d) {
+            int cc = c[a] + d;
+            if (cc >= 0 && cc < P.res[a]) m |= 1u << (((cc >> 2) - own) + 1);
+        }
+        s[a] = m;
+    }
+    uint32_t mask = 0;
+#pragma unroll
+    for (int j = 0; j < 27; ++j) {
+        int ox = j % 3, oy = (j / 3) % 3, oz = j / 9;
+        if (((s[0] >> ox) & 1u) && ((s[1] >> oy) & 1u) && ((s[2] >> oz) & 1u)) mask |= 1u << j;
+    }
+    return mask;
+}
+
+MPM_DEV void block_coords(const SimParams& P, uint32_t key, int b[3]) {
+    b[0] = (int)(key % (uint32_t)P.nb[0]);
+    b[1] = (int)((key / (uint32_t)P.nb[0]) % (uint32_t)P.nb[1]);
+    b[2] = (int)(key / ((uint32_t)P.nb[0] * (uint32_t)P.nb[1]));
+}
+
+// ------------------------------------------------------------------ wave helpers (wave64)
+MPM_DEV int lane_id() { return (int)__lane_id(); }
+MPM_DEV uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+MPM_DEV uint32_t wave_or(uint32_t v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) v |= __shfl_xor(v, o, 64);
+    return v;
+}
+MPM_DEV uint32_t wave_incl_scan(uint32_t v) {
+    const int l = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) { uint32_t t = __shfl_up(v, d, 64); if (l >= d) v += t; }
+    return v;
+}
+// exclusive scan of three u32 streams over a 256-thread workgroup
+MPM_DEV void block_excl_scan3(uint32_t v[3], uint32_t ex[3], uint32_t tot[3], uint32_t (*wt)[3]) {
+    const int w = threadIdx.x >> 6, l = lane_id();
+    uint32_t inc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) inc[c] = wave_incl_scan(v[c]);
+    if (l == 63) { wt[w][0] = inc[0]; wt[w][1] = inc[1]; wt[w][2] = inc[2]; }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        uint32_t pre = 0, all = 0;
+        for (int k = 0; k < 4; ++k) { if (k < w) pre += wt[k][c]; all += wt[k][c]; }
+        ex[c] = pre + inc[c] - v[c];
+        tot[c] = all;
+    }
+    __syncthreads();
+}
+
+// ================================================================== binning
+// Upload: 192-byte ParticleData records (mpmOps.wgsl:1-12) -> SoA planes, ids = record index.
+__global__ void k_upload(const float* __restrict__ aos, float* __restrict__ s, uint32_t* __restrict__ pid, uint32_t n, uint32_t stride) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* q = aos + (size_t)i * 48;
+    st(s, stride, PX, i, q[0]); st(s, stride, PY, i, q[1]); st(s, stride, PZ, i, q[2]);
+    st(s, stride, VX, i, q[4]); st(s, stride, VY, i, q[5]); st(s, stride, VZ, i, q[6]); st(s, stride, PM, i, q[7]);
+    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) {
+        st(s, stride, FE + 3 * c + r, i, q[8 + 4 * c + r]);
+        st(s, stride, FP + 3 * c + r, i, q[20 + 4 * c + r]);
+        st(s, stride, DD + 3 * c + r, i, q[36 + 4 * c + r]);
+    }
+    st(s, stride, DXX, i, q[32]); st(s, stride, DXY, i, q[33]); st(s, stride, DXZ, i, q[34]);
+    pid[i] = i;
+}
+__global__ void k_download(const float* __restrict__ s, const uint32_t* __restrict__ pid, float* __restrict__ aos, uint32_t n, uint32_t stride) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t id = pid[i];
+    if (id >= n) return;
+    float* q = aos + (size_t)id * 48;
+    q[0] = ld(s, stride, PX, i); q[1] = ld(s, stride, PY, i); q[2] = ld(s, stride, PZ, i); q[3] = 1.0f;
+    q[4] = ld(s, stride, VX, i); q[5] = ld(s, stride, VY, i); q[6] = ld(s, stride, VZ, i); q[7] = ld(s, stride, PM, i);
+    for (int c = 0; c < 3; ++c) {
+        for (int r = 0; r < 3; ++r) {
+            q[8 + 4 * c + r] = ld(s, stride, FE + 3 * c + r, i);
+            q[20 + 4 * c + r] = ld(s, stride, FP + 3 * c + r, i);
+            q[36 + 4 * c + r] = ld(s, stride, DD + 3 * c + r, i);
+        }
+        q[11 + 4 * c] = 0.0f; q[23 + 4 * c] = 0.0f; q[39 + 4 * c] = 0.0f;
+    }
+    q[32] = ld(s, stride, DXX, i); q[33] = ld(s, stride, DXY, i); q[34] = ld(s, stride, DXZ, i); q[35] = 0.0f;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Per particle: own-block key, touched-neighbour mask; per key: particle count and OR of masks
+// (countParticlesPerBlock.wgsl:7-23 + the block set of mapAffectedBlocks.wgsl:64-86). Atomics are
+// aggregated per wave over lanes that share a key (input is in the previous substep's sorted order).
+// Also resets the hash table (clearHashMap.wgsl:5-19) and the per-substep counters.
+__global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
+    const uint32_t gsz = gridDim.x * blockDim.x;
+    const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t s = gi; s < P.cap; s += gsz) B.table[s] = EMPTY;
+    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; }
+    const uint32_t n = P.n;
+    const uint32_t nloop = (n + gsz - 1) / gsz;
+    for (uint32_t it = 0; it < nloop; ++it) {
+        uint32_t i = it * gsz + gi;
+        bool valid = false;
+        uint32_t key = EMPTY, mask = 0;
+        if (i < n) {
+            v3 pos = ld3(B.cur, B.stride, PX, i);
+            int c[3]; cell_number(P, pos, c);
+            mask = touched_mask(P, c);
+            int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
+            if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
+                key = (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+                valid = true;
+            }
+            B.key[i] = key; B.mask[i] = mask;
+        }
+        uint64_t active = __ballot(valid);
+        while (active) {
+            int leader = __ffsll((unsigned long long)active) - 1;
+            uint32_t lk = __shfl(key, leader, 64);
+            bool peer = valid && key == lk;
+            uint64_t peers = __ballot(peer);
+            uint32_t m = wave_or(peer ? mask : 0u);
+            if (lane_id() == leader) {
+                atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
+                atomicOr(&B.kmask[lk], m);
+            }
+            active &= ~peers;
+        }
+    }
+}
+
+// Scan over the key space, 1024 keys per tile. Streams: particle count, run flag, chunk count.
+__global__ __launch_bounds__(256) void k_scan_tiles(SimParams P, Bufs B) {
+    __shared__ uint32_t wt[4][3];
+    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t v[3] = {0, 0, 0};
+    for (int q = 0; q < 4; ++q) {
+        uint32_t k = base + q;
+        uint32_t c = k < P.K ? B.kcount[k] : 0u;
+        v[0] += c; v[1] += c ? 1u : 0u; v[2] += (c + P.chunk - 1) / P.chunk;
+    }
+    uint32_t ex[3], tot[3];
+    block_excl_scan3(v, ex, tot, wt);
+    if (threadIdx.x == 0) { B.tile_sums[3 * blockIdx.x] = tot[0]; B.tile_sums[3 * blockIdx.x + 1] = tot[1]; B.tile_sums[3 * blockIdx.x + 2] = tot[2]; }
+}
+__global__ __launch_bounds__(256) void k_scan_top(SimParams P, Bufs B, uint32_t ntiles) {
+    __shared__ uint32_t wt[4][3];
+    const uint32_t per = (ntiles + 255u) / 256u;
+    const uint32_t t0 = threadIdx.x * per;
+    uint32_t v[3] = {0, 0, 0};
+    for (uint32_t t = t0; t < t0 + per && t < ntiles; ++t) { v[0] += B.tile_sums[3 * t]; v[1] += B.tile_sums[3 * t + 1]; v[2] += B.tile_sums[3 * t + 2]; }
+    uint32_t ex[3], tot[3];
+    block_excl_scan3(v, ex, tot, wt);
+    for (uint32_t t = t0; t < t0 + per && t < ntiles; ++t) {
+        B.tile_ofs[3 * t] = ex[0]; B.tile_ofs[3 * t + 1] = ex[1]; B.tile_ofs[3 * t + 2] = ex[2];
+        ex[0] += B.tile_sums[3 * t]; ex[1] += B.tile_sums[3 * t + 1]; ex[2] += B.tile_sums[3 * t + 2];
+    }
+    if (threadIdx.x == 0) { B.counters[C_NVALID] = tot[0]; B.counters[C_NRUNS] = tot[1]; B.counters[C_NCHUNKS] = tot[2]; }
+}
+// Writes run descriptors (particle blocks), work chunks and scatter cursors; zeroes the per-key
+// accumulators for the next substep.
+__global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B) {
+    __shared__ uint32_t wt[4][3];
+    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t cnt[4], v[3] = {0, 0, 0};
+    for (int q = 0; q < 4; ++q) {
+        uint32_t k = base + q;
+        uint32_t c = k < P.K ? B.kcount[k] : 0u;
+        cnt[q] = c;
+        v[0] += c; v[1] += c ? 1u : 0u; v[2] += (c + P.chunk - 1) / P.chunk;
+    }
+    uint32_t ex[3], tot[3];
+    block_excl_scan3(v, ex, tot, wt);
+    uint32_t o0 = ex[0] + B.tile_ofs[3 * blockIdx.x], o1 = ex[1] + B.tile_ofs[3 * blockIdx.x + 1], o2 = ex[2] + B.tile_ofs[3 * blockIdx.x + 2];
+    for (int q = 0; q < 4; ++q) {
+        uint32_t k = base + q;
+        if (k >= P.K) break;
+        uint32_t c = cnt[q];
+        if (c) {
+            uint32_t nch = (c + P.chunk - 1) / P.chunk;
+            B.kcursor[k] = o0;
+            B.run_key[o1] = k; B.run_start[o1] = o0; B.run_count[o1] = c; B.run_mask[o1] = B.kmask[k];
+            for (uint32_t j = 0; j < nch; ++j) B.chunks[o2 + j] = make_uint2(o1, j);
+            B.kcount[k] = 0u; B.kmask[k] = 0u;
+            o0 += c; o1 += 1; o2 += nch;
+        }
+    }
+}
+// binParticles.wgsl:8-25 — scatter of particle indices by key; one cursor atomic per wave and key.
+// Particles with no valid own block go to the tail [n_valid, N).
+__global__ __launch_bounds__(256) void k_scatter(SimParams P, Bufs B) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nvalid = B.counters[C_NVALID];
+    uint32_t key = i < P.n ? B.key[i] : EMPTY;
+    bool valid = i < P.n && key != EMPTY;
+    bool strag = i < P.n && key == EMPTY;
+    uint64_t active = __ballot(valid);
+    while (active) {
+        int leader = __ffsll((unsigned long long)active) - 1;
+        uint32_t lk = __shfl(key, leader, 64);
+        bool peer = valid && key == lk;
+        uint64_t peers = __ballot(peer);
+        uint32_t b = 0;
+        if (lane_id() == leader) b = atomicAdd(&B.kcursor[lk], (uint32_t)__popcll(peers));
+        b = __shfl(b, leader, 64);
+        if (peer) B.perm[b + (uint32_t)__popcll(peers & lanemask_lt())] = i;
+        active &= ~peers;
+    }
+    uint64_t sm = __ballot(strag);
+    if (sm) {
+        int leader = __ffsll((unsigned long long)sm) - 1;
+        uint32_t b = 0;
+        if (lane_id() == leader) b = atomicAdd(&B.counters[C_STRAG], (uint32_t)__popcll(sm));
+        b = __shfl(b, leader, 64);
+        if (strag) B.perm[nvalid + b + (uint32_t)__popcll(sm & lanemask_lt())] = i;
+    }
+}
+
+// Open-addressing insert of one block key (mapAffectedBlocks.wgsl:7-62 allocates the same set;
+// the table stores the key itself so a CAS on one word claims or finds the slot).
+MPM_DEV uint32_t table_insert(const SimParams& P, const Bufs& B, int bx, int by, int bz) {
+    const uint32_t key = (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+    uint32_t s = hash3((uint32_t)bx, (uint32_t)by, (uint32_t)bz) % P.cap;
+    for (uint32_t i = 0; i < P.max_probes; ++i) {
+        uint32_t prev = atomicCAS(&B.table[s], EMPTY, key);
+        if (prev == EMPTY || prev == key) return s;
+        s = (s + 1 >= P.cap) ? 0 : s + 1;
+    }
+    atomicAdd(&B.counters[C_PROBEFAIL], 1u);
+    return EMPTY;
+}
+__global__ __launch_bounds__(256) void k_insert(SimParams P, Bufs B) {
+    const uint32_t nruns = B.counters[C_NRUNS];
+    const uint32_t nvalid = B.counters[C_NVALID];
+    const uint32_t gsz = gridDim.x * blockDim.x;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nruns * 27u; t += gsz) {
+        uint32_t r = t / 27u, j = t % 27u;
+        uint32_t slot = EMPTY;
+        if ((B.run_mask[r] >> j) & 1u) {
+            int b[3]; block_coords(P, B.run_key[r], b);
+            slot = table_insert(P, B, b[0] + (int)(j % 3u) - 1, b[1] + (int)((j / 3u) % 3u) - 1, b[2] + (int)(j / 9u) - 1);
+        }
+        B.nbr_slot[t] = slot;
+    }
+    // unbinned particles still allocate the blocks of their in-range stencil cells
+    for (uint32_t p = nvalid + blockIdx.x * blockDim.x + threadIdx.x; p < P.n; p += gsz) {
+        uint32_t old = B.perm[p];
+        uint32_t m = B.mask[old];
+        if (!m) continue;
+        int c[3]; cell_number(P, ld3(B.cur, B.stride, PX, old), c);
+        for (int j = 0; j < 27; ++j)
+            if ((m >> j) & 1u) table_insert(P, B, (c[0] >> 2) + j % 3 - 1, (c[1] >> 2) + (j / 3) % 3 - 1, (c[2] >> 2) + j / 9 - 1);
+    }
+}
+// Compaction of occupied slots to dense block indices (n_allocated_blocks / mapped_block_indexes).
+__global__ __launch_bounds__(256) void k_compact(SimParams P, Bufs B) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t key = s < P.cap ? B.table[s] : EMPTY;
+    bool occ = key != EMPTY;
+    uint64_t om = __ballot(occ);
+    if (!om) return;
+    int leader = __ffsll((unsigned long long)om) - 1;
+    uint32_t b = 0;
+    if (lane_id() == leader) b = atomicAdd(&B.counters[C_NBLOCKS], (uint32_t)__popcll(om));
+    b = __shfl(b, leader, 64) + (uint32_t)__popcll(om & lanemask_lt());
+    if (occ) {
+        if (b < P.max_blocks) { B.slot_val[s] = b; B.block_key[b] = key; }
+        else { B.slot_val[s] = EMPTY; atomicAdd(&B.counters[C_DROPS], 1u); }
+    }
+}
+MPM_DEV uint32_t n_blocks_of(const SimParams& P, const Bufs& B) {
+    uint32_t nb = B.counters[C_NBLOCKS];
+    return nb < P.max_blocks ? nb : P.max_blocks;
+}
+// Neighbour-block table of every run (27 dense block indices) and the clear of grid buffer 0
+// over the allocated blocks (clearMappedBlocks.wgsl:9-26 restricted to live blocks).
+__global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) {
+    const uint32_t nruns = B.counters[C_NRUNS];
+    const uint32_t gsz = gridDim.x * blockDim.x;
+    const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t t = gi; t < nruns * 27u; t += gsz) {
+        uint32_t s = B.nbr_slot[t];
+        B.nbr[t] = s == EMPTY ? EMPTY : B.slot_val[s];
+    }
+    const uint32_t ncell = n_blocks_of(P, B) * 64u;
+    for (uint32_t c = gi; c < ncell; c += gsz) B.grid0[c] = make_int4(0, 0, 0, 0);
+}
+
+// ================================================================== particle update helpers
+// plasticityOps.wgsl:7-48
+MPM_DEV void plasticity(const SimParams& P, Part& p) {
+    const m3 dE = p.Fe, dP = p.Fp;
+    const m3 rot = polar_rotation(dE);
+    const m3 sc = mmul(transpose(rot), dE);
+    const float lo = P.plast_lo, hi = P.plast_hi;
+    const float s0 = sc.c[0].x, s1 = sc.c[1].y, s2 = sc.c[2].z;
+    if (lo <= s0 && lo <= s1 && lo <= s2 && s0 <= hi && s1 <= hi && s2 <= hi) return;
+    float t0 = s0 < lo ? lo : s0; t0 = hi < t0 ? hi : t0;
+    float t1 = s1 < lo ? lo : s1; t1 = hi < t1 ? hi : t1;
+    float t2 = s2 < lo ? lo : s2; t2 = hi < t2 ? hi : t2;
+    m3 ns = zero3(); ns.c[0].x = t0; ns.c[1].y = t1; ns.c[2].z = t2;
+    const m3 nFe = mmul(rot, ns);
+    const m3 nFeInv = inverse3(nFe);
+    const m3 F = mmul(dE, dP);
+    p.Fe = nFe;
+    p.Fp = mmul(nFeInv, F);
+}
+
+MPM_DEV v3 xform(const float* T, v3 p) {
+    v3 c0 = mk(T[0], T[1], T[2]), c1 = mk(T[4], T[5], T[6]), c2 = mk(T[8], T[9], T[10]), c3 = mk(T[12], T[13], T[14]);
+    return ((c0 * p.x + c1 * p.y) + c2 * p.z) + c3 * 1.0f;
+}
+// colliderPrelude.wgsl:37-78 (Ericson, Real-Time Collision Detection 5.1.5)
+MPM_DEV v3 closest_point_triangle(v3 p, v3 a, v3 b, v3 c) {
+    v3 ab = b - a, ac = c - a, ap = p - a;
+    float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f) return a;
+    v3 bp = p - b;
+    float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3) return b;
+    float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) { float v = d1 / (d1 - d3); return a + v * ab; }
+    v3 cp = p - c;
+    float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6) return c;
+    float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) { float w = d2 / (d2 - d6); return a + w * ac; }
+    float va = d3 * d6 - d5 * d4;
+    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+        float w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        return b + w * (c - b);
+    }
+    float denom = 1.0f / ((va + vb) + vc);
+    float v = vb * denom, w = vc * denom;
+    return (a + v * ab) + w * ac;
+}
+// colliderPrelude.wgsl:80-102 (Möller–Trumbore 1997)
+MPM_DEV float intersect_ray_triangle(v3 o, v3 dir, v3 a, v3 b, v3 c) {
+    const float EPS = 1e-6f;
+    v3 e1 = b - a, e2 = c - a;
+    v3 h = cross(dir, e2);
+    float aa = dot(e1, h);
+    if (aa > -EPS && aa < EPS) return -1.0f;
+    float f = 1.0f / aa;
+    v3 s = o - a;
+    float u = f * dot(s, h);
+    if (u < 0.0f || u > 1.0f) return -1.0f;
+    v3 q = cross(s, e1);
+    float v = f * dot(dir, q);
+    if (v < 0.0f || u + v > 1.0f) return -1.0f;
+    float t = f * dot(e2, q);
+    if (t > EPS) return t;
+    return -1.0f;
+}
+MPM_DEV v3 collider_vertex(const SimParams& P, const uint32_t* cd, uint32_t idx) {
+    uint32_t b = P.coll_num_indices + idx * 3u;
+    return mk(as_f(cd[b]), as_f(cd[b + 1]), as_f(cd[b + 2]));
+}
+MPM_DEV float fsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+// colliderPrelude.wgsl:153-356: BVH depth-first traversal (stack of 32, left pushed first),
+// nearest-point and swept-ray tests per leaf triangle, then the velocity response.
+MPM_DEV void resolve_collision(const SimParams& P, const Bufs& B, Part& pt) {
+    if (!P.has_collider) return;
+    const v3 min_b = xform(P.T, mk(P.cmin[0], P.cmin[1], P.cmin[2]));
+    const v3 max_b = xform(P.T, mk(P.cmax[0], P.cmax[1], P.cmax[2]));
+    const float margin = P.margin;
+    const v3 mg = mk(margin, margin, margin);
+    const v3 smin = vmin(min_b, max_b) - mg, smax = vmax(min_b, max_b) + mg;
+    const v3 cur = pt.pos;
+    const v3 prev = cur - pt.dx;
+    const v3 minp = vmin(cur, prev), maxp = vmax(cur, prev);
+    if (minp.x > smax.x || minp.y > smax.y || minp.z > smax.z || maxp.x < smin.x || maxp.y < smin.y || maxp.z < smin.z) return;
+    if (P.coll_num_objects == 0u) return;
+    float min_d2 = 1e20f;
+    v3 cl_n = mk(0, 0, 1), cl_p = cur;
+    float min_t = 1.0f;
+    v3 hit_n = mk(0, 0, 1), hit_p = cur;
+    bool has_hit = false;
+    const v3 ray = pt.dx;
+    const bool do_ccd = length(ray) > 1e-4f;
+    const v3 cur_l = xform(P.Tinv, cur), prev_l = xform(P.Tinv, prev);
+    const v3 pmin = vmin(cur_l, prev_l), pmax = vmax(cur_l, prev_l);
+    uint32_t stack[32];
+    uint32_t sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const uint32_t ni = stack[--sp];
+        if (ni >= P.n_bvh_nodes) continue;
+        const uint4 n0 = B.bvh[2 * ni], n1 = B.bvh[2 * ni + 1];
+        const v3 bmin = mk(as_f(n0.x), as_f(n0.y), as_f(n0.z)) - mg;
+        const v3 bmax = mk(as_f(n1.x), as_f(n1.y), as_f(n1.z)) + mg;
+        if (pmin.x > bmax.x || pmin.y > bmax.y || pmin.z > bmax.z || pmax.x < bmin.x || pmax.y < bmin.y || pmax.z < bmin.z) continue;
+        if (n1.w & 0x80000000u) {
+            const uint32_t start = n0.w, end = start + (n1.w & 0x7FFFFFFFu);
+            for (uint32_t i = start; i < end; ++i) {
+                const v3 v0 = xform(P.T, collider_vertex(P, B.coll_data, B.coll_data[i * 3u]));
+                const v3 v1 = xform(P.T, collider_vertex(P, B.coll_data, B.coll_data[i * 3u + 1u]));
+                const v3 v2 = xform(P.T, collider_vertex(P, B.coll_data, B.coll_data[i * 3u + 2u]));
+                const v3 cp = closest_point_triangle(cur, v0, v1, v2);
+                const v3 df = cur - cp;
+                const float d2 = dot(df, df);
+                if (d2 < min_d2) {
+                    min_d2 = d2; cl_p = cp;
+                    const v3 nn = cross(v1 - v0, v2 - v0);
+                    const float len = length(nn);
+                    if (len > 1e-6f) cl_n = nn / len;
+                }
+                if (do_ccd) {
+                    const float t = intersect_ray_triangle(prev, ray, v0, v1, v2);
+                    if (t > 0.0f && t < min_t) {
+                        min_t = t; has_hit = true;
+                        hit_p = prev + ray * t;
+                        const v3 fn = normalize(cross(v1 - v0, v2 - v0));
+                        hit_n = fn * fsign(dot(ray, fn));
+                    }
+                }
+            }
+        } else if (sp < 31u) {
+            stack[sp++] = n0.w;
+            stack[sp++] = n1.w;
+        }
+    }
+    const v3 cvel = mk(P.cvel[0], P.cvel[1], P.cvel[2]);
+    const float vsf = P.vel_factor / P.dt;
+    if (has_hit) {
+        pt.pos = hit_p + hit_n * margin;
+        const v3 vr = pt.dx / P.dt - cvel;
+        const float vn = dot(vr, hit_n);
+        const v3 vt = vr - vn * hit_n;
+        const v3 nv = vn < 0.0f ? vt * (1.0f - P.fr_ccd) + cvel * vsf : vr + cvel * vsf;
+        pt.vel = nv;
+        pt.dx = nv * P.dt;
+        return;
+    }
+    const float dist = __builtin_sqrtf(min_d2);
+    const v3 df = cur - cl_p;
+    v3 pd = cl_n;
+    const float ldf = length(df);
+    if (ldf > 1e-6f) pd = df / ldf;
+    if (dist < margin) {
+        const v3 vr = pt.dx / P.dt - cvel;
+        const float vn = dot(vr, pd);
+        const v3 vt = vr - vn * pd;
+        const v3 nv = vn < 0.0f ? vt * (1.0f - P.fr_prox) + cvel * vsf : vr + cvel * vsf;
+        pt.vel = nv;
+        pt.dx = nv * P.dt;
+        pt.pos = cl_p + pd * margin;
+    }
+}
+
+// integrateParticles.wgsl:12-72
+MPM_DEV void integrate(const SimParams& P, const Bufs& B, Part& p) {
+    p.pos = p.pos + p.dx;
+    p.Fe = mmul(madd(ident(), p.D), p.Fe);
+    plasticity(P, p);
+    float* pp = &p.pos.x; float* dd = &p.dx.x; float* vv = &p.vel.x;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (pp[a] < P.gmin[a]) { dd[a] = dd[a] * -0.5f; pp[a] = P.gmin[a]; vv[a] = vv[a] * -0.5f; }
+        if (pp[a] >= P.gmax[a]) { dd[a] = dd[a] * -0.5f; pp[a] = P.gmax[a]; vv[a] = vv[a] * -0.5f; }
+    }
+    resolve_collision(P, B, p);
+}
+
+// pbmpm.wgsl:16-39 (relax = 0.5 + 0.5(1 - exp(-xi / det F_p)) precomputed per substep)
+MPM_DEV void solve_constraints(const SimParams& P, const m3& Fe, float relax, m3& D) {
+    const m3 trial = mmul(madd(ident(), D), Fe);
+    const m3 rot = polar_rotation(trial);
+    const float tv = det3(trial);
+    const float vs = fpow(fabs_(tv), P.vol_exp);
+    const m3 ts = mscale_r(rot, vs);
+    const m3 tb = madd(mscale(P.blend, ts), mscale(P.one_minus_blend, rot));
+    const m3 corr = msub(mmul(tb, inverse3(Fe)), ident());
+    const m3 diff = msub(corr, D);
+    D = madd(D, mscale(relax, diff));
+}
+MPM_DEV float pbmpm_relax(const SimParams& P, const m3& Fp) {
+    const float Jp = det3(Fp);
+    return 0.5f + 0.5f * (1.0f - fexp(-P.xi_p / Jp));
+}
+
+// pbmpm.wgsl:104-134 — pointer force at a grid node (grid-cell units)
+MPM_DEV v3 pointer_force(const SimParams& P, int x, int y, int z) {
+    const v3 node = mk((float)x, (float)y, (float)z);
+    const v3 ip = mk(P.ip[0], P.ip[1], P.ip[2]);
+    const float dist = length(node - ip);
+    v3 applied = mk(0, 0, 0);
+    if (dist < P.iradius) {
+        const v3 off = node - ip;
+        v3 dir = mk(0, 0, 1);
+        if (length(off) > 0.001f) dir = normalize(off);
+        const float fl = dist / P.iradius;
+        const float fall = 1.0f - fl * fl;
+        if (P.imode == 0u) applied = (dir * P.istrength) * fall;
+        else applied = (((-1.0f * dir) * P.istrength) * fall) * (1.0f - fall);
+    }
+    return applied;
+}
+
+MPM_DEV bool sticky_cell(const SimParams& P, int x, int y, int z) {
+    if (!(P.wall_mode & 1u)) return false;
+    const int b = (int)P.wall_cells;
+    return x < b || y < b || z < b || x >= P.res[0] - b || y >= P.res[1] - b || z >= P.res[2] - b;
+}
+
+// ================================================================== LDS-tiled chunk kernels
+// A work chunk is up to P.chunk particles of one particle block (run). All their 3x3x3 stencils
+// fall in the 6^3 node tile [4b-1, 4b+4]^3 of that block, so P2G accumulates in LDS (one LDS
+// atomic per node and channel per particle) and flushes each node with one global atomic per
+// channel, and G2P reads node velocities from LDS.
+struct Tile {
+    float4 vel[216];       // G2P node velocity (w = 1 if the node has mass)
+    int acc[4][216];       // P2G accumulators: mass, px, py, pz
+    int node[216];         // global cell index of the node, -1 if out of range / not allocated
+    uint32_t nbr[27];
+    int base[3];
+    uint32_t start, count, center_ok;
+};
+
+MPM_DEV int tile_of(int tx, int ty, int tz) { return tx + 6 * (ty + 6 * tz); }
+
+MPM_DEV void tile_setup(const SimParams& P, const Bufs& B, Tile& T, uint2 ch) {
+    if (threadIdx.x < 27) T.nbr[threadIdx.x] = B.nbr[ch.x * 27u + threadIdx.x];
+    if (threadIdx.x == 0) {
+        int b[3]; block_coords(P, B.run_key[ch.x], b);
+        T.base[0] = 4 * b[0] - 1; T.base[1] = 4 * b[1] - 1; T.base[2] = 4 * b[2] - 1;
+        const uint32_t rc = B.run_count[ch.x];
+        const uint32_t off = ch.y * P.chunk;
+        T.start = B.run_start[ch.x] + off;
+        T.count = (rc - off) < P.chunk ? (rc - off) : P.chunk;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) T.center_ok = T.nbr[13] != EMPTY;
+    for (int t = threadIdx.x; t < 216; t += blockDim.x) {
+        const int tx = t % 6, ty = (t / 6) % 6, tz = t / 36;
+        const int x = T.base[0] + tx, y = T.base[1] + ty, z = T.base[2] + tz;
+        const int j = (tx == 0 ? 0 : (tx == 5 ? 2 : 1)) + 3 * (ty == 0 ? 0 : (ty == 5 ? 2 : 1)) + 9 * (tz == 0 ? 0 : (tz == 5 ? 2 : 1));
+        const uint32_t blk = T.nbr[j];
+        int idx = -1;
+        if (in_range(P, x, y, z) && blk != EMPTY)
+            idx = (int)(blk * 64u + (uint32_t)(((tx + 3) & 3) + 4 * ((ty + 3) & 3) + 16 * ((tz + 3) & 3)));
+        T.node[t] = idx;
+    }
+}
+// node velocity as gridToParticle.cs.wgsl:43-49 / :102-108 computes it
+MPM_DEV void tile_load_vel(const SimParams& P, Tile& T, const int4* __restrict__ g) {
+    for (int t = threadIdx.x; t < 216; t += blockDim.x) {
+        const int idx = T.node[t];
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (idx >= 0) {
+            const int4 q = g[idx];
+            const float m = (float)q.x / P.S;
+            if (m > 0.0f) {
+                v.x = ((float)q.y / P.S) / m;
+                v.y = ((float)q.z / P.S) / m;
+                v.z = ((float)q.w / P.S) / m;
+                v.w = 1.0f;
+                if (P.wall_mode & 1u) {
+                    const int tx = t % 6, ty = (t / 6) % 6, tz = t / 36;
+                    if (sticky_cell(P, T.base[0] + tx, T.base[1] + ty, T.base[2] + tz)) { v.x = 0.f; v.y = 0.f; v.z = 0.f; }
+                }
+            }
+        }
+        T.vel[t] = v;
+    }
+}
+MPM_DEV void tile_zero_acc(Tile& T) {
+    for (int t = threadIdx.x; t < 216; t += blockDim.x) { T.acc[0][t] = 0; T.acc[1][t] = 0; T.acc[2][t] = 0; T.acc[3][t] = 0; }
+}
+MPM_DEV void tile_flush(Tile& T, int4* __restrict__ g) {
+    for (int t = threadIdx.x; t < 216; t += blockDim.x) {
+        const int idx = T.node[t];
+        if (idx < 0) continue;
+        int* q = reinterpret_cast<int*>(g + idx);
+        const int a0 = T.acc[0][t], a1 = T.acc[1][t], a2 = T.acc[2][t], a3 = T.acc[3][t];
+        if (a0) atomicAdd(q, a0);
+        if (a1) atomicAdd(q + 1, a1);
+        if (a2) atomicAdd(q + 2, a2);
+        if (a3) atomicAdd(q + 3, a3);
+    }
+}
+
+// Stencil geometry of one particle inside its block's tile.
+struct Stencil {
+    int c0[3];
+    int t0[3];      // tile coordinate of the particle's own cell
+    v3 f;           // fractional position
+    v3 w[3];
+};
+MPM_DEV void stencil(const SimParams& P, const Tile& T, v3 pos, Stencil& s) {
+    cell_number(P, pos, s.c0);
+    s.f = frac_pos(P, pos, s.c0);
+    bspline(s.f, s.w);
+    s.t0[0] = s.c0[0] - T.base[0]; s.t0[1] = s.c0[1] - T.base[1]; s.t0[2] = s.c0[2] - T.base[2];
+}
+
+// gridToParticle.cs.wgsl:82-130 from the LDS velocity tile
+MPM_DEV void g2p_pbmpm(const SimParams& P, const Tile& T, const Stencil& s, v3& nv, m3& Bm) {
+    const v3 pcp = (mk((float)s.c0[0], (float)s.c0[1], (float)s.c0[2]) + s.f) - mk(0.5f, 0.5f, 0.5f);
+    nv = mk(0, 0, 0);
+    Bm = zero3();
+    for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
+        const float4 q = T.vel[tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz)];
+        if (q.w == 0.0f) continue;
+        const v3 v = mk(q.x, q.y, q.z);
+        const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
+        nv = nv + cw * v;
+        const v3 dist = mk((float)(s.c0[0] + ox), (float)(s.c0[1] + oy), (float)(s.c0[2] + oz)) - pcp;
+        Bm = madd(Bm, outer(cw * v, dist));
+    }
+}
+// pbmpm.wgsl:41-76 + :79-147 into the LDS accumulators. The two i32 contributions of a node's
+// momentum (transfer and force) are added before the atomic: integer addition is associative,
+// so the node sums are the same bits as the reference's separate atomics.
+MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, float mass, v3 dx, const m3& D) {
+    const v3 pcp = (mk((float)s.c0[0], (float)s.c0[1], (float)s.c0[2]) + s.f) - mk(0.5f, 0.5f, 0.5f);
+    const v3 g0 = mk(P.grav[0], P.grav[1], P.grav[2]);
+    for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
+        const int t = tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz);
+        if (T.node[t] < 0) continue;
+        const int cx = s.c0[0] + ox, cy = s.c0[1] + oy, cz = s.c0[2] + oz;
+        const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
+        const float wm = cw * mass;
+        const v3 off = mk((float)cx, (float)cy, (float)cz) - pcp;
+        const v3 aff = mvmul(D, off);
+        const v3 mom = (wm * (dx + aff)) / P.dt;
+        v3 forces = g0;
+        if (P.interacting) forces = forces + pointer_force(P, cx, cy, cz);
+        forces = forces / P.force_div;
+        const v3 mc = (forces * wm) * P.dt;
+        atomicAdd(&T.acc[0][t], f2i(wm * P.S));
+        atomicAdd(&T.acc[1][t], (int)((uint32_t)f2i(mom.x * P.S) + (uint32_t)f2i(mc.x * P.S)));
+        atomicAdd(&T.acc[2][t], (int)((uint32_t)f2i(mom.y * P.S) + (uint32_t)f2i(mc.y * P.S)));
+        atomicAdd(&T.acc[3][t], (int)((uint32_t)f2i(mom.z * P.S) + (uint32_t)f2i(mc.z * P.S)));
+    }
+}
+
+enum { PB_FIRST = 0, PB_MID = 1, PB_LAST = 2 };
+
+// One PBMPM transfer pass over all work chunks.
+//   FIRST: gather particle state through the sort permutation (cur -> nxt), relax factor,
+//          constraint solve of iteration 0, P2G into grid[gout].
+//   MID:   G2P of iteration i-1 from grid[gin], solve + P2G of iteration i into grid[gout].
+//   LAST:  G2P of iteration k-1 from grid[gin], integrateParticles, write the particle state.
+// Every pass also clears grid[gclr] (the buffer the next pass writes) over the live blocks.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr) {
+    __shared__ Tile T;
+    const uint32_t nchunks = B.counters[C_NCHUNKS];
+    if (gclr >= 0) {
+        int4* gc = grid_sel(B, gclr);
+        const uint32_t ncell = n_blocks_of(P, B) * 64u;
+        for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) gc[c] = make_int4(0, 0, 0, 0);
+    }
+    const int4* gi = MODE != PB_FIRST ? grid_sel(B, gin) : nullptr;
+    int4* go = MODE != PB_LAST ? grid_sel(B, gout) : nullptr;
+    const uint32_t sd = B.stride;
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        tile_setup(P, B, T, B.chunks[ch]);
+        if (MODE != PB_FIRST) tile_load_vel(P, T, gi);
+        if (MODE != PB_LAST) tile_zero_acc(T);
+        __syncthreads();
+        const bool ok = T.center_ok != 0;
+        for (uint32_t i = threadIdx.x; i < T.count; i += blockDim.x) {
+            const uint32_t q = T.start + i;
+            if (MODE == PB_FIRST) {
+                const uint32_t old = B.perm[q];
+                const v3 pos = ld3(B.cur, sd, PX, old);
+                const float mass = ld(B.cur, sd, PM, old);
+                const m3 Fe = ldm(B.cur, sd, FE, old);
+                const m3 Fp = ldm(B.cur, sd, FP, old);
+                m3 D = ldm(B.cur, sd, DD, old);
+                const v3 dx = ld3(B.cur, sd, DXX, old);
+                B.pid_nxt[q] = B.pid_cur[old];
+                st3(B.nxt, sd, PX, q, pos); st(B.nxt, sd, PM, q, mass);
+                stm(B.nxt, sd, FE, q, Fe); stm(B.nxt, sd, FP, q, Fp);
+                const float relax = pbmpm_relax(P, Fp);
+                B.relax[q] = relax;
+                if (ok) {
+                    Stencil s; stencil(P, T, pos, s);
+                    solve_constraints(P, Fe, relax, D);
+                    p2g_pbmpm(P, T, s, mass, dx, D);
+                } else {   // run whose own block is not allocated: carried unchanged to LAST
+                    stm(B.nxt, sd, DD, q, D); st3(B.nxt, sd, DXX, q, dx); st3(B.nxt, sd, VX, q, ld3(B.cur, sd, VX, old));
+                }
+            } else if (MODE == PB_MID) {
+                if (!ok) continue;
+                const v3 pos = ld3(B.nxt, sd, PX, q);
+                const float mass = ld(B.nxt, sd, PM, q);
+                const m3 Fe = ldm(B.nxt, sd, FE, q);
+                const float relax = B.relax[q];
+                Stencil s; stencil(P, T, pos, s);
+                v3 nv; m3 Bm;
+                g2p_pbmpm(P, T, s, nv, Bm);
+                const v3 dx = nv * P.dt;
+                m3 D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
+                solve_constraints(P, Fe, relax, D);
+                p2g_pbmpm(P, T, s, mass, dx, D);
+            } else {
+                Part p;
+                p.pos = ld3(B.nxt, sd, PX, q);
+                p.mass = ld(B.nxt, sd, PM, q);
+                p.Fe = ldm(B.nxt, sd, FE, q);
+                p.Fp = ldm(B.nxt, sd, FP, q);
+                if (ok) {
+                    Stencil s; stencil(P, T, p.pos, s);
+                    v3 nv; m3 Bm;
+                    g2p_pbmpm(P, T, s, nv, Bm);
+                    p.dx = nv * P.dt;
+                    p.D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
+                    p.vel = nv;
+                } else {
+                    p.dx = ld3(B.nxt, sd, DXX, q); p.D = ldm(B.nxt, sd, DD, q); p.vel = ld3(B.nxt, sd, VX, q);
+                }
+                integrate(P, B, p);
+                st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
+                stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
+                st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
+            }
+        }
+        __syncthreads();
+        if (MODE != PB_LAST) tile_flush(T, go);
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ explicit MPM
+// particleToGrid.cs.wgsl:27-79 (hardening stressTensorOps.wgsl:39-56, PK1 stress :9-22)
+template <int DUMMY = 0>
+__global__ __launch_bounds__(256) void k_explicit_p2g(SimParams P, Bufs B) {
+    __shared__ Tile T;
+    const uint32_t nchunks = B.counters[C_NCHUNKS];
+    const uint32_t sd = B.stride;
+    int4* go = B.grid0;
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        tile_setup(P, B, T, B.chunks[ch]);
+        tile_zero_acc(T);
+        __syncthreads();
+        const bool ok = T.center_ok != 0;
+        for (uint32_t i = threadIdx.x; i < T.count; i += blockDim.x) {
+            const uint32_t q = T.start + i;
+            const uint32_t old = B.perm[q];
+            const v3 pos = ld3(B.cur, sd, PX, old);
+            const float mass = ld(B.cur, sd, PM, old);
+            const v3 vel = ld3(B.cur, sd, VX, old);
+            const m3 Fe = ldm(B.cur, sd, FE, old);
+            const m3 Fp = ldm(B.cur, sd, FP, old);
+            B.pid_nxt[q] = B.pid_cur[old];
+            st3(B.nxt, sd, PX, q, pos); st(B.nxt, sd, PM, q, mass);
+            stm(B.nxt, sd, FE, q, Fe); stm(B.nxt, sd, FP, q, Fp);
+            if (!ok) {
+                st3(B.nxt, sd, VX, q, vel); st3(B.nxt, sd, DXX, q, ld3(B.cur, sd, DXX, old)); stm(B.nxt, sd, DD, q, ldm(B.cur, sd, DD, old));
+                continue;
+            }
+            float mu = P.mu0, la = P.la0;
+            const float ef = fexp(P.xi_e * (1.0f - det3(Fp)));
+            mu = mu * ef; la = la * ef;
+            const float J = det3(Fe);
+            const m3 R = polar_rotation(Fe);
+            const m3 Pk = madd(mscale(2.0f * mu, msub(Fe, R)), mscale((la * (J - 1.0f)) * J, transpose(inverse3(Fe))));
+            const m3 negVPT = mscale(-(mass * P.inv_density), transpose(Pk));
+            const v3 pm = mass * vel;
+            Stencil s; stencil(P, T, pos, s);
+            v3 d[3]; bspline_d(s.f, d);
+            const v3 cdv = mk(P.cd[0], P.cd[1], P.cd[2]);
+            for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
+                const int t = tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz);
+                if (T.node[t] < 0) continue;
+                const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
+                v3 g = mk((d[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z,
+                          (s.w[ox + 1].x * d[oy + 1].y) * s.w[oz + 1].z,
+                          (s.w[ox + 1].x * s.w[oy + 1].y) * d[oz + 1].z);
+                g = cdiv(g, cdv);
+                const v3 sm = mvmul(negVPT, g) * P.dt;
+                const v3 mom = cw * pm + sm;
+                atomicAdd(&T.acc[0][t], f2i((cw * mass) * P.S));
+                atomicAdd(&T.acc[1][t], f2i(mom.x * P.S));
+                atomicAdd(&T.acc[2][t], f2i(mom.y * P.S));
+                atomicAdd(&T.acc[3][t], f2i(mom.z * P.S));
+            }
+        }
+        __syncthreads();
+        tile_flush(T, go);
+        __syncthreads();
+    }
+}
+// gridUpdate.cs.wgsl:35-53 over the live blocks
+__global__ __launch_bounds__(256) void k_grid_update(SimParams P, Bufs B) {
+    const uint32_t ncell = n_blocks_of(P, B) * 64u;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
+        int4 q = B.grid0[c];
+        const float m = (float)q.x / P.S;
+        if (m <= 0.0f) continue;
+        v3 v = mk((float)q.y / P.S, (float)q.z / P.S, (float)q.w / P.S) / m;
+        v = v + mk(P.grav[0], P.grav[1], P.grav[2]) * P.dt;
+        if (P.wall_mode & 1u) {
+            int b[3]; block_coords(P, B.block_key[c >> 6], b);
+            const uint32_t l = c & 63u;
+            if (sticky_cell(P, 4 * b[0] + (int)(l & 3u), 4 * b[1] + (int)((l >> 2) & 3u), 4 * b[2] + (int)(l >> 4))) v = mk(0, 0, 0);
+        }
+        const v3 nm = (v * m) * P.S;
+        q.y = f2i(nm.x); q.z = f2i(nm.y); q.w = f2i(nm.z);
+        B.grid0[c] = q;
+    }
+}
+// gridToParticle.cs.wgsl:28-80 + integrateParticles.wgsl:12-72
+template <int DUMMY = 0>
+__global__ __launch_bounds__(256) void k_explicit_g2p(SimParams P, Bufs B) {
+    __shared__ Tile T;
+    const uint32_t nchunks = B.counters[C_NCHUNKS];
+    const uint32_t sd = B.stride;
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+        tile_setup(P, B, T, B.chunks[ch]);
+        tile_load_vel(P, T, B.grid0);
+        __syncthreads();
+        const bool ok = T.center_ok != 0;
+        for (uint32_t i = threadIdx.x; i < T.count; i += blockDim.x) {
+            const uint32_t q = T.start + i;
+            Part p;
+            p.pos = ld3(B.nxt, sd, PX, q);
+            p.mass = ld(B.nxt, sd, PM, q);
+            p.Fe = ldm(B.nxt, sd, FE, q);
+            p.Fp = ldm(B.nxt, sd, FP, q);
+            if (ok) {
+                Stencil s; stencil(P, T, p.pos, s);
+                v3 d[3]; bspline_d(s.f, d);
+                v3 nv = mk(0, 0, 0);
+                m3 G = zero3();
+                for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
+                    const float4 qv = T.vel[tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz)];
+                    if (qv.w == 0.0f) continue;
+                    const v3 v = mk(qv.x, qv.y, qv.z);
+                    const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
+                    nv = nv + cw * v;
+                    const v3 g = mk((d[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z,
+                                    (s.w[ox + 1].x * d[oy + 1].y) * s.w[oz + 1].z,
+                                    (s.w[ox + 1].x * s.w[oy + 1].y) * d[oz + 1].z);
+                    m3 o; o.c[0] = g.x * v; o.c[1] = g.y * v; o.c[2] = g.z * v;
+                    G = madd(G, o);
+                }
+                p.vel = nv;
+                p.dx = nv * P.dt;
+                p.D = mscale_r(mscale_r(G, P.dt), P.grad_scale);
+            } else {
+                p.dx = ld3(B.nxt, sd, DXX, q); p.D = ldm(B.nxt, sd, DD, q); p.vel = ld3(B.nxt, sd, VX, q);
+            }
+            integrate(P, B, p);
+            st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
+            stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
+            st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
+        }
+        __syncthreads();
+    }
+}
+
+// Unbinned particles: copied to the new order and integrated with their stored dx / D
+// (no grid transfer reaches them). Runs over the sorted tail [n_valid, N).
+__global__ __launch_bounds__(256) void k_straggle(SimParams P, Bufs B) {
+    const uint32_t nvalid = B.counters[C_NVALID];
+    const uint32_t sd = B.stride;
+    for (uint32_t q = nvalid + blockIdx.x * blockDim.x + threadIdx.x; q < P.n; q += gridDim.x * blockDim.x) {
+        const uint32_t old = B.perm[q];
+        B.pid_nxt[q] = B.pid_cur[old];
+        Part p;
+        p.pos = ld3(B.cur, sd, PX, old); p.vel = ld3(B.cur, sd, VX, old); p.mass = ld(B.cur, sd, PM, old);
+        p.Fe = ldm(B.cur, sd, FE, old); p.Fp = ldm(B.cur, sd, FP, old); p.D = ldm(B.cur, sd, DD, old);
+        p.dx = ld3(B.cur, sd, DXX, old);
+        st(B.nxt, sd, PM, q, p.mass);
+        integrate(P, B, p);
+        st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
+        stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
+        st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
+    }
+}
+// Copy of the state without integration (debug-stage prefixes leave particles untouched).
+__global__ void k_copy_state(Bufs B, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int pl = 0; pl < NPLANES; ++pl) B.nxt[(size_t)pl * B.stride + i] = B.cur[(size_t)pl * B.stride + i];
+    B.pid_nxt[i] = B.pid_cur[i];
+}
+
+// Elementwise evaluation of the device arithmetic for bitwise checks against the host.
+// kind: 0 exp(a) 1 log(a) 2 pow(a,b) 3 sqrt(a) 4 a/b 5 hash3(bits a, bits b, i) 6 det3
+//       7 inverse3 8 polar rotation 9 f2i(a) (matrices: 9 floats per item, column-major)
+__global__ void k_selftest(int kind, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (kind <= 5 || kind == 9) {
+        const float x = a[i], y = b ? b[i] : 0.0f;
+        float r = 0.0f;
+        switch (kind) {
+            case 0: r = fexp(x); break;
+            case 1: r = flog(x); break;
+            case 2: r = fpow(x, y); break;
+            case 3: r = __builtin_sqrtf(x); break;
+            case 4: r = x / y; break;
+            case 5: r = as_f(hash3(as_u(x), as_u(y), i)); break;
+            default: r = as_f((uint32_t)f2i(x)); break;
+        }
+        out[i] = r;
+        return;
+    }
+    m3 m;
+    for (int c = 0; c < 3; ++c) m.c[c] = mk(a[9 * i + 3 * c], a[9 * i + 3 * c + 1], a[9 * i + 3 * c + 2]);
+    if (kind == 6) { out[i] = det3(m); return; }
+    const m3 r = kind == 7 ? inverse3(m) : polar_rotation(m);
+    for (int c = 0; c < 3; ++c) { out[9 * i + 3 * c] = r.c[c].x; out[9 * i + 3 * c + 1] = r.c[c].y; out[9 * i + 3 * c + 2] = r.c[c].z; }
+}
+
+}  // namespace mpm

@@ -1,0 +1,273 @@
+"""Host-side mirror of the reference's simulation managers over libmpm_hip.so.
+
+Same class and method names as the TypeScript the drop-in replaces, so a caller of the
+reference finds the same surface:
+  GpuUniformsBufferManager   src/lib/gpu/uniforms/GpuUniformsBufferManager.ts:3-183
+  GpuMpmBufferManager        src/lib/gpu/mpm/GpuMpmBufferManager.ts:2-81
+  GpuColliderBufferManager   src/lib/gpu/collider/GpuColliderBufferManager.ts:42-351
+  GpuMpmPipelineManager      src/lib/gpu/mpm/GpuMpmPipelineManager.ts:20-521
+  MpmSimulation              the simulation half of GpuSnowPipelineRunner.svelte.ts:68-445
+All GPU work happens in the HIP library; these classes pack bytes and call the C ABI.
+"""
+import ctypes
+import numpy as np
+
+from . import _lib
+from ._lib import check, METHOD_EXPLICIT, METHOD_PBMPM, UNIFORMS_BYTES
+from .particles import PARTICLE_DTYPE
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class GpuUniformsBufferManager:
+    """Byte-offset writers of the 37,376-byte uniform buffer (GpuUniformsBufferManager.ts:54-182)."""
+
+    def __init__(self, ctx_handle, lib):
+        self._ctx = ctx_handle
+        self._lib = lib
+
+    def _write(self, offset, arr):
+        a = np.ascontiguousarray(arr)
+        check(self._lib, self._ctx, self._lib.mpm_write_uniforms(self._ctx, offset, _ptr(a), a.nbytes))
+
+    def _f(self, offset, vals):
+        self._write(offset, np.asarray(vals, dtype=np.float32).ravel())
+
+    def _u(self, offset, vals):
+        self._write(offset, np.asarray(vals, dtype=np.uint32).ravel())
+
+    def read(self, offset, nbytes):
+        out = np.zeros(nbytes, np.uint8)
+        check(self._lib, self._ctx, self._lib.mpm_read_uniforms(self._ctx, offset, _ptr(out), nbytes))
+        return out
+
+    def writeSimulationTimestepS(self, timestep): self._f(0, [timestep])
+    def writeFixedPointScale(self, scale): self._f(4, [scale])
+    def writeUsePbmpm(self, use_pbmpm): self._u(8, [1 if use_pbmpm else 0])
+    def writeTime(self, time): self._u(12, [int(time) & 0xFFFFFFFF])
+    def writeGridMinCoords(self, mn): self._f(16, mn)
+    def writeGridMaxCoords(self, mx): self._f(32, mx)
+    def writeViewProjMat(self, m): self._f(48, m)
+    def writeViewProjInvMat(self, m): self._f(112, m)
+    def writeMeshMinCoords(self, mn): self._f(176, mn)
+    def writeMeshMaxCoords(self, mx): self._f(192, mx)
+    def writeGridResolution(self, res): self._u(208, res)
+    def writeColliderMinCoords(self, mn): self._f(224, mn)
+    def writeColliderMaxCoords(self, mx): self._f(240, mx)
+    def writeColliderTransformMat(self, m): self._f(256, m)
+    def writeColliderVel(self, v): self._f(320, v)
+    def writeCameraPos(self, p): self._f(336, p)
+    def writeGridCellDims(self, d): self._f(352, d)
+    def writeColliderNumIndices(self, n): self._u(348, [n])
+    def writeColliderNumObjects(self, n): self._u(364, [n])
+    def writeLightViewProjMat(self, m): self._f(368, m)
+    def writeColliderTransformInv(self, m): self._f(432, m)
+    def writeInteractionPos(self, p): self._f(496, p)
+    def writeInteractionStrength(self, s): self._f(508, [s])
+    def writeInteractionRadius(self, r): self._f(512, [r])
+    def writeIsInteracting(self, b): self._u(516, [1 if b else 0])
+    def writeInteractionDir(self, d): self._f(528, d)
+    def writeInteractionMode(self, m): self._u(520, [m])
+
+    def writeColliderObjects(self, objects):
+        count = min(len(objects), 1024)
+        buf = np.zeros(count * 8, np.uint32)
+        f = buf.view(np.float32)
+        for i, o in enumerate(objects[:count]):
+            f[8 * i:8 * i + 3] = o["min"]
+            buf[8 * i + 3] = o["startIndex"]
+            f[8 * i + 4:8 * i + 7] = o["max"]
+            buf[8 * i + 7] = o["countIndices"]
+        if count:
+            self._write(544, buf)
+
+
+def build_bvh(positions, indices):
+    """buildBvh on the host (GpuColliderBufferManager.ts:222-350) via mpm_build_bvh.
+    Returns (nodes uint32 (n,8), triangle leaf order)."""
+    lib = _lib.load_library()
+    pos = np.ascontiguousarray(positions, dtype=np.float32).ravel()
+    idx = np.ascontiguousarray(indices, dtype=np.uint32).ravel()
+    nt = idx.size // 3
+    cap = max(2 * nt, 1)
+    nodes = np.zeros((cap, 8), np.uint32)
+    order = np.zeros(max(nt, 1), np.uint32)
+    n = ctypes.c_size_t(0)
+    rc = lib.mpm_build_bvh(_ptr(pos), pos.size // 3, _ptr(idx), idx.size, _ptr(nodes), cap, ctypes.byref(n), _ptr(order))
+    if rc != 0:
+        raise _lib.MpmError(rc, "mpm_build_bvh")
+    return nodes[:n.value].copy(), order[:nt].copy()
+
+
+class GpuColliderBufferManager:
+    """Packs collider data as colliderDataBuffer does ([indices][vertices][normals][uvs][materials],
+    GpuColliderBufferManager.ts:96-126), builds the BVH and uploads both."""
+
+    def __init__(self, vertices, indices, normals=(), uvs=(), materialIndices=()):
+        v = np.asarray(vertices, np.float32).reshape(-1, 3)
+        self.minCoords = v.min(axis=0).tolist() if len(v) else [np.inf] * 3
+        self.maxCoords = v.max(axis=0).tolist() if len(v) else [-np.inf] * 3
+        idx = np.asarray(indices, np.uint32).ravel()
+        self.numIndices = int(idx.size)
+        self.bvhNodes, tri_order = build_bvh(v, idx)
+        self.numBvhNodes = int(self.bvhNodes.shape[0])
+        reordered = idx.reshape(-1, 3)[tri_order].ravel() if idx.size else idx
+        parts = [reordered.astype(np.uint32), v.ravel().view(np.uint32),
+                 np.asarray(normals, np.float32).ravel().view(np.uint32),
+                 np.asarray(uvs, np.float32).ravel().view(np.uint32),
+                 np.asarray(materialIndices, np.uint32).ravel()]
+        self.colliderData = np.concatenate(parts).astype(np.uint32)
+        self.indicesOffset = 0
+        self.verticesOffset = 4 * idx.size
+
+    def upload(self, sim):
+        lib, ctx = sim.lib, sim.ctx
+        check(lib, ctx, lib.mpm_set_collider(ctx, _ptr(self.colliderData), self.colliderData.size, self.numIndices,
+                                             _ptr(self.bvhNodes), self.numBvhNodes))
+
+
+class GpuMpmBufferManager:
+    """Creation-time sizing (GpuMpmBufferManager.ts:12-13): nParticles plus the sparse-grid
+    capacities the library derived."""
+
+    def __init__(self, nParticles, max_blocks, hash_map_size):
+        self.nParticles = nParticles
+        self.nMaxBlocksInHashMap = max_blocks
+        self.hashMapSize = hash_map_size
+
+
+class GpuMpmPipelineManager:
+    """Per-substep dispatch schedules (GpuMpmPipelineManager.ts:327-521), one mpm_step each."""
+
+    def __init__(self, sim):
+        self._sim = sim
+
+    def addExplicitMpmDispatches(self, nParticles=None, nBlocksInHashMap=None, hashMapSize=None):
+        self._sim.step(METHOD_EXPLICIT, 1)
+
+    def addPbmpmDispatches(self, nParticles=None, nBlocksInHashMap=None, hashMapSize=None):
+        self._sim.step(METHOD_PBMPM, 1)
+
+
+class MpmSimulation:
+    """Headless simulation runner over one libmpm_hip context.
+
+    Mirrors the simulation half of GpuSnowPipelineRunner (constructor uniform writes
+    :133-144/:171-177, updateColliderTransformMat/Vel :383-390, addSimulationStepsComputePass
+    :392-445) without any rendering."""
+
+    def __init__(self, n_particles, grid_res, grid_min=(-5, -5, -5), grid_max=(5, 5, 5), dt=1 / 384, **cfg):
+        self.lib = _lib.load_library()
+        self.config = _lib.default_config(n_particles=n_particles, grid_res=list(grid_res),
+                                          grid_min=list(grid_min), grid_max=list(grid_max), dt=dt, **cfg)
+        h = ctypes.c_void_p()
+        rc = self.lib.mpm_create(ctypes.byref(self.config), ctypes.byref(h))
+        if rc != 0:
+            raise _lib.MpmError(rc, "mpm_create")
+        self.ctx = h
+        self.nParticles = n_particles
+        self.uniformsManager = GpuUniformsBufferManager(self.ctx, self.lib)
+        self.pipelineManager = GpuMpmPipelineManager(self)
+        self.colliderManager = None
+
+    def close(self):
+        if self.ctx:
+            self.lib.mpm_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- collider (GpuSnowPipelineRunner.svelte.ts:162-177, 383-390)
+    def set_collider(self, vertices, indices, transform=None):
+        cm = GpuColliderBufferManager(vertices, indices)
+        cm.upload(self)
+        um = self.uniformsManager
+        um.writeColliderMinCoords(cm.minCoords)
+        um.writeColliderMaxCoords(cm.maxCoords)
+        um.writeColliderNumObjects(1)
+        um.writeColliderObjects([{"min": cm.minCoords, "max": cm.maxCoords, "startIndex": 0, "countIndices": cm.numIndices}])
+        self.updateColliderTransformMat(np.eye(4, dtype=np.float32) if transform is None else transform)
+        um.writeColliderVel([0.0, 0.0, 0.0])
+        self.colliderManager = cm
+        return cm
+
+    def updateColliderTransformMat(self, mat4):
+        m = np.asarray(mat4, np.float64).reshape(4, 4)
+        # column-major storage like wgpu-matrix: element (row r, col c) at [4c + r]
+        self.uniformsManager.writeColliderTransformMat(m.T.ravel().astype(np.float32))
+        self.uniformsManager.writeColliderTransformInv(np.linalg.inv(m).T.ravel().astype(np.float32))
+
+    def updateColliderVel(self, v):
+        self.uniformsManager.writeColliderVel(v)
+
+    # ---- particles
+    def upload(self, particles):
+        p = np.ascontiguousarray(particles)
+        assert p.dtype == PARTICLE_DTYPE and p.shape[0] == self.nParticles
+        check(self.lib, self.ctx, self.lib.mpm_upload_particles(self.ctx, _ptr(p), p.shape[0]))
+
+    def download(self):
+        out = np.zeros(self.nParticles, PARTICLE_DTYPE)
+        check(self.lib, self.ctx, self.lib.mpm_download_particles(self.ctx, _ptr(out), self.nParticles))
+        return out
+
+    # ---- stepping
+    def step(self, method, n_substeps=1):
+        check(self.lib, self.ctx, self.lib.mpm_step(self.ctx, int(method), int(n_substeps)))
+
+    def sync(self):
+        check(self.lib, self.ctx, self.lib.mpm_sync(self.ctx))
+
+    def debug_stage(self, method, stage):
+        check(self.lib, self.ctx, self.lib.mpm_debug_stage(self.ctx, int(method), int(stage)))
+
+    def dump_grid(self):
+        n = ctypes.c_size_t(0)
+        check(self.lib, self.ctx, self.lib.mpm_dump_grid(self.ctx, None, None, 0, ctypes.byref(n)))
+        xyz = np.zeros((n.value, 3), np.int32)
+        vals = np.zeros((n.value, 4), np.int32)
+        check(self.lib, self.ctx, self.lib.mpm_dump_grid(self.ctx, _ptr(xyz), _ptr(vals), n.value, ctypes.byref(n)))
+        return xyz, vals
+
+    def dump_blocks(self):
+        n = ctypes.c_size_t(0)
+        check(self.lib, self.ctx, self.lib.mpm_dump_blocks(self.ctx, None, 0, ctypes.byref(n)))
+        xyz = np.zeros((n.value, 3), np.int32)
+        check(self.lib, self.ctx, self.lib.mpm_dump_blocks(self.ctx, _ptr(xyz), n.value, ctypes.byref(n)))
+        return xyz
+
+    def dump_particle_blocks(self):
+        n = ctypes.c_size_t(0)
+        check(self.lib, self.ctx, self.lib.mpm_dump_particle_blocks(self.ctx, None, None, 0, ctypes.byref(n)))
+        xyz = np.zeros((n.value, 3), np.int32)
+        cnt = np.zeros(n.value, np.uint32)
+        check(self.lib, self.ctx, self.lib.mpm_dump_particle_blocks(self.ctx, _ptr(xyz), _ptr(cnt), n.value, ctypes.byref(n)))
+        return xyz, cnt
+
+    def stats(self):
+        s = _lib.MpmStats()
+        check(self.lib, self.ctx, self.lib.mpm_get_stats(self.ctx, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def set_timing(self, on=True):
+        check(self.lib, self.ctx, self.lib.mpm_set_timing(self.ctx, 1 if on else 0))
+
+    def timings(self):
+        t = _lib.MpmTimings()
+        check(self.lib, self.ctx, self.lib.mpm_get_timings(self.ctx, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def uniform_bytes(self):
+        return self.uniformsManager.read(0, UNIFORMS_BYTES)

@@ -1,0 +1,100 @@
+"""Seeded synthetic scenes for the workloads the step is measured and tested on.
+
+  explicit_small  65,536 particles, 64^3 grid over [0,1]^3, 0.25-side cube at (0.5,0.6,0.5),
+                  v = (0,-1,0), 16 per cell with uniform jitter, dt = 1e-4, sticky 3-cell walls.
+  pbmpm_two_spheres  1,048,576 particles in two r = 0.15 spheres at (0.3,0.5,0.5) and
+                  (0.7,0.5,0.5) moving +/-2 m/s along x, 128^3 grid, k = 4, dt = 1/600,
+                  12-triangle box collider of side 0.2 at the domain centre.
+  explicit_slab_terrain  slab 0.6 x 0.1 x 0.6 at y = 0.7 over a 64 x 64-vertex heightfield
+                  (7,938 triangles), 256^3 grid, dt = 5e-5.
+Gravity is -9.8 along +y (the scenes are y-up). Masses use one unit per particle so the
+2^16 fixed-point grid sums keep their resolution (the reference's random masses in [0,1]
+would truncate to zero at these particle spacings).
+"""
+import numpy as np
+
+from .particles import make_particles
+
+
+def box_mesh(center, side):
+    """12-triangle axis-aligned cube (the geometry of the reference's static/box.glb, scaled)."""
+    c = np.asarray(center, np.float64)
+    h = 0.5 * side
+    v = np.array([[x, y, z] for z in (-h, h) for y in (-h, h) for x in (-h, h)], np.float64) + c
+    # vertex index = x + 2y + 4z
+    quads = [(0, 2, 3, 1), (4, 5, 7, 6), (0, 1, 5, 4), (2, 6, 7, 3), (0, 4, 6, 2), (1, 3, 7, 5)]
+    tris = []
+    for a, b, cc, d in quads:
+        tris += [(a, b, cc), (a, cc, d)]
+    return v.astype(np.float32), np.asarray(tris, np.uint32).ravel()
+
+
+def heightfield_mesh(nx, nz, extent, hmin, hmax, rng):
+    """Seeded low-poly terrain: nx x nz vertices over [0,extent]^2, heights U[hmin,hmax]."""
+    xs = np.linspace(0.0, extent, nx)
+    zs = np.linspace(0.0, extent, nz)
+    X, Z = np.meshgrid(xs, zs, indexing="xy")
+    Y = rng.uniform(hmin, hmax, size=X.shape)
+    v = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1).astype(np.float32)
+    tris = []
+    for j in range(nz - 1):
+        for i in range(nx - 1):
+            a = j * nx + i
+            b, c, d = a + 1, a + nx, a + nx + 1
+            tris += [(a, c, b), (b, c, d)]
+    return v, np.asarray(tris, np.uint32).ravel()
+
+
+def jittered_cells(lo_cell, hi_cell, h, ppc, rng):
+    """ppc particles per cell, uniform jitter inside each cell of the integer box [lo, hi)."""
+    cells = np.stack(np.meshgrid(*[np.arange(lo_cell[a], hi_cell[a]) for a in range(3)], indexing="ij"), -1).reshape(-1, 3)
+    base = np.repeat(cells, ppc, axis=0).astype(np.float64)
+    return ((base + rng.random(base.shape)) * h).astype(np.float32)
+
+
+def sphere_points(center, radius, n, rng):
+    out = np.empty((0, 3))
+    while out.shape[0] < n:
+        q = rng.uniform(-radius, radius, size=(2 * n, 3))
+        q = q[(q * q).sum(1) < radius * radius]
+        out = np.concatenate([out, q])
+    return (out[:n] + np.asarray(center)).astype(np.float32)
+
+
+def scene(name, n_particles=None, seed=None):
+    """Returns dict(particles, sim kwargs, collider (vertices, indices) or None, method, substeps)."""
+    if name == "explicit_small":
+        rng = np.random.default_rng(0 if seed is None else seed)
+        res, h = 64, 1.0 / 64
+        lo = np.array([0.5 - 0.125, 0.6 - 0.125, 0.5 - 0.125])
+        lo_c = np.round(lo / h).astype(int)
+        pos = jittered_cells(lo_c, lo_c + 16, h, 16, rng)
+        if n_particles is not None:
+            pos = pos[rng.permutation(pos.shape[0])[:n_particles]]
+        p = make_particles(pos, vel=(0.0, -1.0, 0.0))
+        sim = dict(grid_res=(res, res, res), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1e-4,
+                   gravity=(0.0, -9.8, 0.0), wall_mode=1, wall_cells=3, hardening_explicit=10.0)
+        return dict(particles=p, sim=sim, collider=None, method=0, substeps=100)
+    if name == "pbmpm_two_spheres":
+        rng = np.random.default_rng(1 if seed is None else seed)
+        n = 1_048_576 if n_particles is None else n_particles
+        a = sphere_points((0.3, 0.5, 0.5), 0.15, n // 2, rng)
+        b = sphere_points((0.7, 0.5, 0.5), 0.15, n - n // 2, rng)
+        vel = np.zeros((n, 3), np.float32)
+        vel[: n // 2, 0] = 2.0
+        vel[n // 2:, 0] = -2.0
+        p = make_particles(np.concatenate([a, b]), vel=vel)
+        sim = dict(grid_res=(128, 128, 128), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1.0 / 600,
+                   gravity=(0.0, -9.8, 0.0), pbmpm_iterations=4, pbmpm_force_divisor=4.0)
+        return dict(particles=p, sim=sim, collider=box_mesh((0.5, 0.5, 0.5), 0.2), method=1, substeps=600)
+    if name == "explicit_slab_terrain":
+        rng = np.random.default_rng(2 if seed is None else seed)
+        res, h = 256, 1.0 / 256
+        n = 4_194_304 if n_particles is None else n_particles
+        lo = np.array([0.2, 0.65, 0.2])
+        pos = (lo + rng.random((n, 3)) * np.array([0.6, 0.1, 0.6])).astype(np.float32)
+        p = make_particles(pos)
+        sim = dict(grid_res=(res, res, res), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=5e-5,
+                   gravity=(0.0, -9.8, 0.0))
+        return dict(particles=p, sim=sim, collider=heightfield_mesh(64, 64, 1.0, 0.05, 0.25, rng), method=0, substeps=2000)
+    raise KeyError(name)
