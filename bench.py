@@ -1,0 +1,176 @@
+"""Benchmark: particle-substeps/s of the 1,048,576-particle PBMPM two-sphere scene on one MI355X.
+
+A step is one substep of the whole scene (binning, k = 4 constraint iterations of
+solve + P2G + G2P, integrate with the box collider), inputs resident in HBM. Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+
+roofline: the kernel with the largest device time per substep, measured with HIP events on
+the library's stream over a separate instrumented pass (eager launches, one event after every
+kernel); achieved = that kernel's compulsory particle-state bytes per launch / its average
+duration. cpu_baseline: the oracle (FP32-faithful CPU restatement of the reference substep,
+one thread) on two substeps of the same 1M-particle scene.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import pathlib
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+PKG_DIR = ROOT / "webgpu-mpm-based-snow-simulation_amd"
+
+HBM_PEAK_GBS = 8000.0
+# Compulsory particle-state bytes per particle for each transfer kernel of the implemented layout
+# (FP32 SoA planes; reads + writes):
+#   k_pbmpm<FIRST>: read perm, id, x, m, F_e, F_p, D, dx (144 B); write id, x, m, F_e, F_p, relax (96 B)
+#   k_pbmpm<MID>:   read x, m, F_e, relax (56 B)
+#   k_pbmpm<LAST>:  read x, m, F_e, F_p (88 B); write x, v, F_e, F_p, dx, D (144 B)
+KERNEL_BYTES_PER_PARTICLE = {
+    "k_pbmpm<FIRST>": 240, "k_pbmpm<MID>": 56, "k_pbmpm<LAST>": 232,
+    "k_explicit_p2g": 4 + 4 + 12 + 4 + 12 + 36 + 36 + 4 + 12 + 4 + 36 + 36,
+    "k_explicit_g2p": 12 + 4 + 36 + 36 + 12 + 12 + 36 + 36 + 12 + 36,
+}
+# Reference formulation (one substep of PBMPM with k iterations): 268 + 172 k bytes per particle.
+STEP_ALGO_BYTES = lambda k: 268 + 172 * k
+
+
+def load_package():
+    spec = importlib.util.spec_from_file_location("mpm_amd", PKG_DIR / "__init__.py",
+                                                  submodule_search_locations=[str(PKG_DIR)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mpm_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def cpu_baseline(mpm, sc, sim, substeps=2):
+    """The oracle timed on the host: FP32-faithful restatement, single thread."""
+    sys.path.insert(0, str(ROOT))
+    from oracle import oracle as orc
+    p = sc["particles"]
+    kw = sc["sim"]
+    o = orc.OracleSim(p, sim.uniform_bytes(), pbmpm_iterations=kw["pbmpm_iterations"],
+                      pbmpm_force_divisor=kw["pbmpm_force_divisor"], gravity=kw["gravity"])
+    o.set_collider(sim.colliderManager.colliderData, sim.colliderManager.bvhNodes)
+    t0 = time.perf_counter()
+    o.step(sc["method"], substeps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": p.shape[0] * substeps / dt, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{substeps} substeps of the full {p.shape[0]}-particle scene ({dt:.1f} s), "
+                      f"oracle FP32 restatement on 1 thread of {cpu_model()} ({os.cpu_count()} logical CPUs)"}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if one exists."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        return d.get(kernel, {}).get("bytes_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--kernel-steps", type=int, default=60)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--particles", type=int, default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on exactly one GPU; --gpus N > 1 is not supported "
+              "(the multi-GPU sharding of the step is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    mpm = load_package()
+    sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=args.particles)
+    p = sc["particles"]
+    n = p.shape[0]
+    k = sc["sim"]["pbmpm_iterations"]
+    sim = mpm.MpmSimulation(n, **sc["sim"])
+    v, idx = sc["collider"]
+    sim.set_collider(v, idx)
+    sim.upload(p)
+    print(f"bench: {n} particles, PBMPM k={k}, warmup {args.warmup}", file=sys.stderr, flush=True)
+
+    sim.step(1, args.warmup)
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(1, args.steps)
+    sim.sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n * args.steps / elapsed
+    print(f"bench: timed {args.steps} substeps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+
+    # per-kernel device time (HIP events on the library's stream)
+    sim.set_kernel_timing(True)
+    sim.step(1, args.kernel_steps)
+    sim.sync()
+    kt = sim.kernel_times()
+    sim.set_kernel_timing(False)
+    per_substep = {name: ms / args.kernel_steps for name, (ms, cnt) in kt.items()}
+    dom = max((nm for nm in per_substep if nm in KERNEL_BYTES_PER_PARTICLE), key=lambda nm: per_substep[nm])
+    dom_ms, dom_cnt = kt[dom]
+    avg_s = dom_ms / dom_cnt * 1e-3
+    stats = sim.stats()
+    algo_bytes = KERNEL_BYTES_PER_PARTICLE[dom] * stats["particles_binned"]
+    achieved = algo_bytes / avg_s / 1e9
+    eager_ms = sum(per_substep.values())
+
+    line = {
+        "metric": "particle-substeps/s, 1M-particle PBMPM",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded two-sphere scene, 1 mass unit per particle)",
+        "config": {"workload": "pbmpm_two_spheres", "particles": n, "grid": "128^3 hashed (4^3 blocks)",
+                   "pbmpm_iterations": k, "dt": sc["sim"]["dt"], "collider": "12-triangle box, BVH",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
+                     "kernel": dom, "avg_launch_us": avg_s * 1e6,
+                     "algorithmic_bytes_per_launch": algo_bytes,
+                     "step_frac_reference_bytes": (STEP_ALGO_BYTES(k) * value / 1e9) / HBM_PEAK_GBS},
+        "kernel_ms_per_substep": {nm: round(v, 5) for nm, v in sorted(per_substep.items(), key=lambda x: -x[1])},
+        "eager_ms_per_substep": eager_ms,
+    }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline (oracle, 2 substeps)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(mpm, sc, sim)
+    else:
+        line["cpu_baseline"] = None
+    sim.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

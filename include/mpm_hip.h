@@ -24,6 +24,7 @@ extern "C" {
 #define MPM_UNIFORMS_BYTES 37376   /* GpuUniformsBufferManager.ts:17 */
 #define MPM_PARTICLE_BYTES 192     /* shaderPrelude/mpmOps.wgsl:1-12, GpuMpmBufferManager.ts:24 */
 #define MPM_BVH_NODE_BYTES 32      /* mpm/colliderPrelude.wgsl:6-11 */
+#define MPM_KERNEL_COUNT 15        /* kernels of one substep, see mpm_kernel_name() */
 
 enum {
     MPM_OK = 0,
@@ -152,6 +153,14 @@ int  mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n
 int  mpm_get_stats(mpm_ctx* ctx, mpm_stats* out);
 int  mpm_set_timing(mpm_ctx* ctx, int enabled);
 int  mpm_get_timings(mpm_ctx* ctx, mpm_timings* out);
+/* Per-kernel device time from HIP events recorded on the context's stream after every kernel
+ * (replaces the reference's whole-pass timestamp slots 0/1, GpuSnowPipelineRunner.svelte.ts:411-420).
+ * While enabled, substeps are launched eagerly instead of through the hipGraph. total_ms and
+ * counts have MPM_KERNEL_COUNT entries; enabling clears them. */
+int  mpm_set_kernel_timing(mpm_ctx* ctx, int enabled);
+int  mpm_get_kernel_times(mpm_ctx* ctx, double* total_ms, uint64_t* counts);
+const char* mpm_kernel_name(int id);
+
 const char* mpm_last_error(mpm_ctx* ctx);
 const char* mpm_version(void);
 

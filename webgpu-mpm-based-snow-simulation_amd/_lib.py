@@ -13,6 +13,7 @@ LIB_PATH = PKG_DIR / "libmpm_hip.so"
 UNIFORMS_BYTES = 37376
 PARTICLE_BYTES = 192
 BVH_NODE_BYTES = 32
+KERNEL_COUNT = 15
 
 MPM_OK = 0
 MPM_E_INVALID = -1
@@ -101,6 +102,9 @@ _SIGS = [
     ("mpm_get_stats", ctypes.c_int, [_P, ctypes.POINTER(MpmStats)]),
     ("mpm_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_get_timings", ctypes.c_int, [_P, ctypes.POINTER(MpmTimings)]),
+    ("mpm_set_kernel_timing", ctypes.c_int, [_P, ctypes.c_int]),
+    ("mpm_get_kernel_times", ctypes.c_int, [_P, _P, _P]),
+    ("mpm_kernel_name", ctypes.c_char_p, [ctypes.c_int]),
     ("mpm_last_error", ctypes.c_char_p, [_P]),
     ("mpm_version", ctypes.c_char_p, []),
     ("mpm_stream", _P, [_P]),

@@ -42,6 +42,7 @@ struct mpm_ctx {
     uint32_t* coll_data = nullptr;
     uint4* bvh = nullptr;
     size_t n_coll_words = 0, n_bvh_nodes = 0;
+    uint32_t coll_num_indices = 0;
     int last_grid = 0;
     bool has_particles = false;
     hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -52,11 +53,44 @@ struct mpm_ctx {
     mpm_timings tm{};
     uint64_t substeps = 0;
     std::vector<void*> allocs;
+    // per-kernel event timing (eager launches, one event after every kernel)
+    bool ktiming = false;
+    std::vector<hipEvent_t> kt_ev;
+    std::vector<int> kt_id;
+    size_t kt_n = 0;
+    double kt_ms[MPM_KERNEL_COUNT] = {};
+    uint64_t kt_cnt[MPM_KERNEL_COUNT] = {};
 };
 
 namespace {
 
 const char* k_version = "libmpm_hip 0.1 (gfx950)";
+const char* k_kernel_names[MPM_KERNEL_COUNT] = {
+    "k_keys_count", "k_scan_tiles", "k_scan_top", "k_scan_apply", "k_scatter", "k_insert", "k_compact", "k_nbr",
+    "k_pbmpm<FIRST>", "k_pbmpm<MID>", "k_pbmpm<LAST>", "k_explicit_p2g", "k_grid_update", "k_explicit_g2p", "k_straggle"};
+enum { KID_KEYS = 0, KID_SCAN_TILES, KID_SCAN_TOP, KID_SCAN_APPLY, KID_SCATTER, KID_INSERT, KID_COMPACT, KID_NBR,
+       KID_PB_FIRST, KID_PB_MID, KID_PB_LAST, KID_EX_P2G, KID_EX_GRID, KID_EX_G2P, KID_STRAGGLE };
+
+// Records an event after a kernel launch when per-kernel timing is on (the first event of a
+// step is the start mark, id -1).
+void kt(mpm_ctx* c, int id) {
+    if (!c->ktiming) return;
+    if (c->kt_n == c->kt_ev.size()) { hipEvent_t e; hipEventCreate(&e); c->kt_ev.push_back(e); c->kt_id.push_back(-1); }
+    c->kt_id[c->kt_n] = id;
+    hipEventRecord(c->kt_ev[c->kt_n], c->stream);
+    c->kt_n++;
+}
+void kt_collect(mpm_ctx* c) {
+    if (!c->ktiming || c->kt_n == 0) return;
+    hipEventSynchronize(c->kt_ev[c->kt_n - 1]);
+    for (size_t i = 1; i < c->kt_n; ++i) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, c->kt_ev[i - 1], c->kt_ev[i]);
+        const int id = c->kt_id[i];
+        if (id >= 0) { c->kt_ms[id] += ms; c->kt_cnt[id] += 1; }
+    }
+    c->kt_n = 0;
+}
 
 int fail(mpm_ctx* c, int code, const std::string& m) { if (c) c->err = m; return code; }
 #define HIPCHK(c, x) do { hipError_t _e = (x); if (_e != hipSuccess) return fail((c), MPM_E_HIP, std::string(#x) + ": " + hipGetErrorString(_e)); } while (0)
@@ -151,6 +185,8 @@ int make_params(mpm_ctx* c, SimParams& P) {
     P.wall_cells = g.wall_cells;
     P.n_bvh_nodes = (uint32_t)c->n_bvh_nodes;
     P.has_collider = (c->n_bvh_nodes > 0 && c->n_coll_words > 0) ? 1u : 0u;
+    if (P.has_collider && P.coll_num_indices != c->coll_num_indices)
+        return fail(c, MPM_E_RANGE, "uniform colliderNumIndices differs from the uploaded collider");
     P.margin = g.collider_margin;
     P.fr_ccd = g.friction_ccd;
     P.fr_prox = g.friction_proximity;
@@ -171,15 +207,16 @@ uint32_t grid_for(uint64_t work, uint32_t cap_wg) {
 void enqueue_binning(mpm_ctx* c, const SimParams& P, const Bufs& B) {
     hipStream_t s = c->stream;
     const uint32_t gk = grid_for(std::max<uint64_t>(c->n, c->cap), 8192);
-    k_keys_count<<<gk, 256, 0, s>>>(P, B);
-    k_scan_tiles<<<c->ntiles, 256, 0, s>>>(P, B);
-    k_scan_top<<<1, 256, 0, s>>>(P, B, c->ntiles);
-    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P, B);
-    k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P, B);
+    kt(c, -1);
+    k_keys_count<<<gk, 256, 0, s>>>(P, B); kt(c, KID_KEYS);
+    k_scan_tiles<<<c->ntiles, 256, 0, s>>>(P, B); kt(c, KID_SCAN_TILES);
+    k_scan_top<<<1, 256, 0, s>>>(P, B, c->ntiles); kt(c, KID_SCAN_TOP);
+    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P, B); kt(c, KID_SCAN_APPLY);
+    k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P, B); kt(c, KID_SCATTER);
     const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 4096);
-    k_insert<<<gi, 256, 0, s>>>(P, B);
-    k_compact<<<grid_for(c->cap, 1u << 30), 256, 0, s>>>(P, B);
-    k_nbr<<<gi, 256, 0, s>>>(P, B);
+    k_insert<<<gi, 256, 0, s>>>(P, B); kt(c, KID_INSERT);
+    k_compact<<<grid_for(c->cap, 1u << 30), 256, 0, s>>>(P, B); kt(c, KID_COMPACT);
+    k_nbr<<<gi, 256, 0, s>>>(P, B); kt(c, KID_NBR);
 }
 
 uint32_t chunk_grid(mpm_ctx* c) {
@@ -200,26 +237,26 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
     const uint32_t gs = 256;
     if (method == MPM_METHOD_PBMPM) {
         const int k = c->cfg.pbmpm_iterations;
-        k_pbmpm<PB_FIRST><<<G, 256, 0, s>>>(P, B, -1, 0, 1);
+        k_pbmpm<PB_FIRST><<<G, 256, 0, s>>>(P, B, -1, 0, 1); kt(c, KID_PB_FIRST);
         c->last_grid = 0;
         if (stage == 1) return false;
         for (int i = 1; i < k; ++i) {
-            k_pbmpm<PB_MID><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3);
+            k_pbmpm<PB_MID><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3); kt(c, KID_PB_MID);
             c->last_grid = i % 3;
             if (stage == i + 1) return false;
         }
         if (full) hipEventRecord(c->ev[2], s);
-        k_pbmpm<PB_LAST><<<G, 256, 0, s>>>(P, B, (k - 1) % 3, -1, -1);
+        k_pbmpm<PB_LAST><<<G, 256, 0, s>>>(P, B, (k - 1) % 3, -1, -1); kt(c, KID_PB_LAST);
     } else {
-        k_explicit_p2g<0><<<G, 256, 0, s>>>(P, B);
+        k_explicit_p2g<0><<<G, 256, 0, s>>>(P, B); kt(c, KID_EX_P2G);
         c->last_grid = 0;
         if (stage == 1) return false;
-        k_grid_update<<<G, 256, 0, s>>>(P, B);
+        k_grid_update<<<G, 256, 0, s>>>(P, B); kt(c, KID_EX_GRID);
         if (stage == 2) return false;
         if (full) hipEventRecord(c->ev[2], s);
-        k_explicit_g2p<0><<<G, 256, 0, s>>>(P, B);
+        k_explicit_g2p<0><<<G, 256, 0, s>>>(P, B); kt(c, KID_EX_G2P);
     }
-    k_straggle<<<gs, 256, 0, s>>>(P, B);
+    k_straggle<<<gs, 256, 0, s>>>(P, B); kt(c, KID_STRAGGLE);
     if (full) hipEventRecord(c->ev[3], s);
     return true;
 }
@@ -324,6 +361,7 @@ static void release(mpm_ctx* c) {
         if (c->graph[m][p]) hipGraphDestroy(c->graph[m][p]);
     }
     for (int i = 0; i < 4; ++i) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    for (hipEvent_t e : c->kt_ev) hipEventDestroy(e);
     for (void* p : c->allocs) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
 }
@@ -445,6 +483,7 @@ int mpm_set_collider(mpm_ctx* c, const uint32_t* data, size_t n_words, uint32_t 
         HIPCHK(c, hipMemcpy(c->bvh, nodes, n_nodes * 32, hipMemcpyHostToDevice));
         c->n_coll_words = n_words; c->n_bvh_nodes = n_nodes;
     }
+    c->coll_num_indices = n_indices;
     wru<uint32_t>(c->uni, 348, n_indices);
     return MPM_OK;
 }
@@ -534,7 +573,7 @@ int mpm_step(mpm_ctx* c, int method, int n_substeps) {
     wru<uint32_t>(c->uni, 8, method == MPM_METHOD_PBMPM ? 1u : 0u);   // writeUsePbmpm
     for (int s = 0; s < n_substeps; ++s) {
         const bool timed = c->timing && s == n_substeps - 1;
-        if (c->cfg.use_graph && !timed) {
+        if (c->cfg.use_graph && !timed && !c->ktiming) {
             if ((rc = ensure_graph(c, method, P))) return rc;
             HIPCHK(c, hipGraphLaunch(c->gexec[method][c->cur], c->stream));
         } else {
@@ -543,6 +582,7 @@ int mpm_step(mpm_ctx* c, int method, int n_substeps) {
         }
         c->cur ^= 1;
         c->substeps++;
+        if (c->ktiming && c->kt_n > 512) kt_collect(c);
         if (timed) {
             HIPCHK(c, hipStreamSynchronize(c->stream));
             float a = 0, b = 0, d = 0;
@@ -554,6 +594,7 @@ int mpm_step(mpm_ctx* c, int method, int n_substeps) {
             c->tm.substeps = c->substeps;
         }
     }
+    kt_collect(c);
     return MPM_OK;
 }
 
@@ -698,6 +739,20 @@ int mpm_set_timing(mpm_ctx* c, int enabled) {
     c->timing = enabled != 0;
     return MPM_OK;
 }
+int mpm_set_kernel_timing(mpm_ctx* c, int enabled) {
+    if (!c) return MPM_E_INVALID;
+    kt_collect(c);
+    c->ktiming = enabled != 0;
+    for (int i = 0; i < MPM_KERNEL_COUNT; ++i) { c->kt_ms[i] = 0.0; c->kt_cnt[i] = 0; }
+    return MPM_OK;
+}
+int mpm_get_kernel_times(mpm_ctx* c, double* total_ms, uint64_t* counts) {
+    if (!c || !total_ms || !counts) return MPM_E_INVALID;
+    for (int i = 0; i < MPM_KERNEL_COUNT; ++i) { total_ms[i] = c->kt_ms[i]; counts[i] = c->kt_cnt[i]; }
+    return MPM_OK;
+}
+const char* mpm_kernel_name(int id) { return (id >= 0 && id < MPM_KERNEL_COUNT) ? k_kernel_names[id] : ""; }
+
 int mpm_get_timings(mpm_ctx* c, mpm_timings* out) {
     if (!c || !out) return MPM_E_INVALID;
     *out = c->tm;

@@ -269,5 +269,16 @@ class MpmSimulation:
         check(self.lib, self.ctx, self.lib.mpm_get_timings(self.ctx, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
 
+    def set_kernel_timing(self, on=True):
+        check(self.lib, self.ctx, self.lib.mpm_set_kernel_timing(self.ctx, 1 if on else 0))
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} accumulated since set_kernel_timing(True)."""
+        ms = np.zeros(_lib.KERNEL_COUNT, np.float64)
+        cnt = np.zeros(_lib.KERNEL_COUNT, np.uint64)
+        check(self.lib, self.ctx, self.lib.mpm_get_kernel_times(self.ctx, _ptr(ms), _ptr(cnt)))
+        return {self.lib.mpm_kernel_name(i).decode(): (float(ms[i]), int(cnt[i]))
+                for i in range(_lib.KERNEL_COUNT) if cnt[i]}
+
     def uniform_bytes(self):
         return self.uniformsManager.read(0, UNIFORMS_BYTES)
