@@ -24,7 +24,7 @@ extern "C" {
 #define MPM_UNIFORMS_BYTES 37376   /* GpuUniformsBufferManager.ts:17 */
 #define MPM_PARTICLE_BYTES 192     /* shaderPrelude/mpmOps.wgsl:1-12, GpuMpmBufferManager.ts:24 */
 #define MPM_BVH_NODE_BYTES 32      /* mpm/colliderPrelude.wgsl:6-11 */
-#define MPM_KERNEL_COUNT 15        /* kernels of one substep, see mpm_kernel_name() */
+#define MPM_KERNEL_COUNT 16        /* kernels of one substep, see mpm_kernel_name() */
 
 enum {
     MPM_OK = 0,
@@ -149,6 +149,10 @@ int  mpm_dump_particle_blocks(mpm_ctx* ctx, int32_t* block_xyz, uint32_t* counts
  * 5 hash3(bits a, bits b, index) 6 det3 7 inverse3 8 polar rotation 9 i32(a); matrix kinds take
  * and return 9 column-major floats per item. */
 int  mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n);
+/* Whether P2G forms i32((x / d) * scale) with the reciprocal + FMA quotient: it does only after
+ * a device check over all 2^32 float x found it equal to the IEEE-division form (no reference
+ * counterpart; test hook). */
+int  mpm_divisor_fast(mpm_ctx* ctx, float d, float scale, int* fast);
 
 int  mpm_get_stats(mpm_ctx* ctx, mpm_stats* out);
 int  mpm_set_timing(mpm_ctx* ctx, int enabled);

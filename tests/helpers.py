@@ -79,3 +79,22 @@ def uniforms_blob(dt, grid_min, grid_max, res, S=65536.0, collider=None, interac
         pos, strength, radius, mode = interaction
         f(496, pos); f(508, [strength]); f(512, [radius]); u(516, [1]); u(520, [mode])
     return b
+
+
+def particles_in_spheres(n, rng, vx=2.0):
+    """n particles (unit mass, F = I) uniformly in the two r = 0.15 spheres at (0.3,0.5,0.5) and
+    (0.7,0.5,0.5) of [0,1]^3, moving +vx / -vx along x."""
+    import __graft_entry__
+    m = __graft_entry__.load_package()
+    pts = []
+    for c in ((0.3, 0.5, 0.5), (0.7, 0.5, 0.5)):
+        q = np.empty((0, 3))
+        k = n // 2 if c[0] < 0.5 else n - n // 2
+        while q.shape[0] < k:
+            r = rng.uniform(-0.15, 0.15, (4 * k + 8, 3))
+            q = np.concatenate([q, r[(r * r).sum(1) < 0.15 ** 2]])
+        pts.append(q[:k] + np.asarray(c))
+    vel = np.zeros((n, 3), np.float32)
+    vel[: n // 2, 0] = vx
+    vel[n // 2:, 0] = -vx
+    return m.make_particles(np.concatenate(pts).astype(np.float32), vel=vel)

@@ -1,0 +1,162 @@
+"""CPU checks of the C ABI (no GPU calls): the library loads, exports every entry point that
+include/mpm_hip.h declares, fills the reference defaults, and builds the collider BVH exactly as
+GpuColliderBufferManager.buildBvh does; the BVH traversal equals a brute-force scan."""
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    txt = (ROOT / "include" / "mpm_hip.h").read_text()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(mpm_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol(mpm):
+    lib = mpm.load_library()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in mpm_hip.h but not exported"
+    assert set(names) == set(mpm.EXPORTED), "ctypes binding and header disagree"
+    assert lib.mpm_version().startswith(b"libmpm_hip")
+
+
+def test_config_defaults_are_reference_values(mpm):
+    c = mpm.default_config()
+    assert c.pbmpm_iterations == 3 and c.pbmpm_force_divisor == 3.0          # GpuMpmPipelineManager.ts:445
+    assert tuple(c.gravity) == (0.0, 0.0, np.float32(-9.81))                  # gridUpdate.cs.wgsl:43
+    assert c.youngs_modulus == 1.4e5 and c.poissons_ratio == 0.2             # stressTensorOps.wgsl:1-2
+    assert c.hardening_explicit == 5.0 and c.hardening_pbmpm == 20.0
+    assert c.pbmpm_blend == np.float32(0.95) and c.pbmpm_volume_exponent == np.float32(-0.3333333)
+    assert c.theta_c == 2.5e-2 and c.theta_s == 7.5e-3 and c.density == 400.0
+    assert c.fixed_point_scale == 65536.0                                    # GpuSnowPipelineRunner.svelte.ts:28
+    assert tuple(c.grid_min) == (-5.0, -5.0, -5.0) and tuple(c.grid_max) == (5.0, 5.0, 5.0)
+    assert c.collider_margin == np.float32(0.05) and c.friction_ccd == np.float32(0.1)
+    assert c.max_probes == 100
+
+
+def _js_bvh(vertices, indices):
+    """buildBvhRecursive (GpuColliderBufferManager.ts:276-350) restated for the test: doubles,
+    stable sort by centroid on the longest axis, median split, leaves of <= 4 triangles."""
+    v = np.asarray(vertices, np.float64).reshape(-1, 3)
+    t = np.asarray(indices).reshape(-1, 3)
+    tri_min = v[t].min(1)
+    tri_max = v[t].max(1)
+    cen = (v[t[:, 0]] + v[t[:, 1]] + v[t[:, 2]]) / 3.0
+    order = list(range(len(t)))
+    nodes = []
+
+    def rec(s, e):
+        ni = len(nodes)
+        idx = order[s:e]
+        mn, mx = tri_min[idx].min(0), tri_max[idx].max(0)
+        if e - s <= 4:
+            nodes.append([mn, mx, s, e - s, True])
+            return ni
+        ext = mx - mn
+        ax = 0
+        if ext[1] > ext[0] and ext[1] > ext[2]:
+            ax = 1
+        elif ext[2] > ext[0] and ext[2] > ext[1]:
+            ax = 2
+        order[s:e] = sorted(idx, key=lambda i: cen[i, ax])
+        mid = s + (e - s) // 2
+        nodes.append([mn, mx, 0, 0, False])
+        left = rec(s, mid)
+        right = rec(mid, e)
+        nodes[ni][2], nodes[ni][3] = left, right
+        return ni
+
+    rec(0, len(t))
+    return nodes, order
+
+
+def test_bvh_build_matches_the_reference_builder(mpm):
+    rng = np.random.default_rng(0)
+    for mesh in (mpm.scenes.box_mesh((0.5, 0.5, 0.5), 0.2),
+                 mpm.scenes.heightfield_mesh(16, 12, 1.0, 0.05, 0.25, rng)):
+        v, idx = mesh
+        nodes, order = mpm.build_bvh(v, idx)
+        want, worder = _js_bvh(v, idx)
+        assert nodes.shape[0] == len(want)
+        assert order.tolist() == worder
+        f = nodes.view(np.float32)
+        for i, (mn, mx, a, b, leaf) in enumerate(want):
+            assert np.array_equal(f[i, 0:3], mn.astype(np.float32)) and np.array_equal(f[i, 4:7], mx.astype(np.float32))
+            assert nodes[i, 3] == a
+            assert nodes[i, 7] == (b | 0x80000000 if leaf else b)
+            if not leaf:
+                assert a == i + 1                                   # left child follows its parent
+
+
+def test_bvh_invariants_on_terrain(mpm):
+    rng = np.random.default_rng(1)
+    v, idx = mpm.scenes.heightfield_mesh(64, 64, 1.0, 0.05, 0.25, rng)
+    assert idx.size // 3 == 7938
+    nodes, order = mpm.build_bvh(v, idx)
+    assert sorted(order.tolist()) == list(range(7938))
+    f = nodes.view(np.float32)
+    covered = np.zeros(7938, int)
+    for i in range(nodes.shape[0]):
+        if nodes[i, 7] & 0x80000000:
+            s, c = nodes[i, 3], nodes[i, 7] & 0x7FFFFFFF
+            assert 1 <= c <= 4
+            covered[s:s + c] += 1
+            tri = np.asarray(idx).reshape(-1, 3)[order[s:s + c]]
+            pts = v[tri.ravel()]
+            assert np.all(pts >= f[i, 0:3] - 0) and np.all(pts <= f[i, 4:7] + 0)
+        else:
+            for ch in (nodes[i, 3], nodes[i, 7]):
+                assert np.all(f[ch, 0:3] >= f[i, 0:3]) and np.all(f[ch, 4:7] <= f[i, 4:7])
+    assert np.all(covered == 1)
+
+
+def test_bvh_culling_changes_no_result(mpm, orc):
+    """colliderPrelude.wgsl:215-291: nodes are skipped when the particle's path box, grown by the
+    margin, misses them; such triangles can neither be hit by the swept ray nor lie within the
+    response distance, so the BVH gives the same particles as one leaf holding every triangle in
+    the traversal's own visiting order (depth first, right child popped first)."""
+    from helpers import uniforms_blob
+    rng = np.random.default_rng(3)
+    for mesh in (mpm.scenes.box_mesh((0.5, 0.5, 0.5), 0.2), mpm.scenes.heightfield_mesh(12, 12, 1.0, 0.3, 0.6, rng)):
+        v, idx = mesh
+        cm = mpm.GpuColliderBufferManager(v, idx)
+        nodes = cm.bvhNodes
+        visit, stack = [], [0]
+        while stack:                                    # traversal order with nothing culled
+            i = stack.pop()
+            if nodes[i, 7] & 0x80000000:
+                s, c = int(nodes[i, 3]), int(nodes[i, 7] & 0x7FFFFFFF)
+                visit += list(range(s, s + c))
+            else:
+                stack += [int(nodes[i, 3]), int(nodes[i, 7])]
+        n_idx = cm.numIndices
+        tri = cm.colliderData[:n_idx].reshape(-1, 3)[visit].ravel()
+        flat_data = cm.colliderData.copy()
+        flat_data[:n_idx] = tri
+        flat = np.zeros((1, 8), np.uint32)
+        flat.view(np.float32)[0, 0:3] = np.min(v, 0)
+        flat.view(np.float32)[0, 4:7] = np.max(v, 0)
+        flat[0, 7] = (n_idx // 3) | 0x80000000
+        n = 4000
+        lo, hi = np.min(v, 0) - 0.08, np.max(v, 0) + 0.08
+        pos = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+        p = mpm.make_particles(pos, vel=rng.uniform(-3, 3, (n, 3)).astype(np.float32))
+        p["dx"][:, :3] = rng.uniform(-0.02, 0.02, (n, 3))
+        uni = uniforms_blob(np.float32(1.0 / 600), (-1, -1, -1), (2, 2, 2), (32, 32, 32),
+                            collider=(np.min(v, 0), np.max(v, 0), n_idx, None))
+        outs = []
+        for data, bvh in ((cm.colliderData, nodes), (flat_data, flat)):
+            o = orc.OracleSim(p, uni, gravity=(0.0, 0.0, 0.0), youngs_modulus=0.0)
+            o.set_collider(data, bvh)
+            o.step(0, 1)
+            outs.append(o.particles())
+        a, b = outs
+        acted = np.abs(a["vel"] - p["vel"]).max(1) > 1e-3
+        assert acted.sum() > 100
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))

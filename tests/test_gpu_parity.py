@@ -135,3 +135,18 @@ def test_graph_replay_matches_eager(mpm, orc):
         x, y = a.download(), b.download()
         assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
     a.close(); b.close()
+
+
+def test_divisor_fast_path_verified(mpm):
+    """P2G's reciprocal+FMA quotient is enabled only for (dt, scale) pairs whose exhaustive device
+    check (all 2^32 float numerators) matched i32((x / dt) * scale); the scene time steps pass."""
+    sim = mpm.MpmSimulation(64, grid_res=(128, 128, 128), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1.0 / 600)
+    lib = sim.lib
+    fast = {}
+    for d in (1.0 / 600, 1.0 / 384, 1.0 / 192, 1e-4, 5e-5, 1.0 / 128):
+        f = ctypes.c_int(-1)
+        assert lib.mpm_divisor_fast(sim.ctx, ctypes.c_float(d), ctypes.c_float(65536.0), ctypes.byref(f)) == 0
+        fast[d] = f.value
+        print("divisor", d, "fast" if f.value else "ieee")
+    assert fast[1.0 / 600] == 1 and fast[1e-4] == 1
+    sim.close()

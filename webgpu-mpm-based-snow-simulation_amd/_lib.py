@@ -13,7 +13,7 @@ LIB_PATH = PKG_DIR / "libmpm_hip.so"
 UNIFORMS_BYTES = 37376
 PARTICLE_BYTES = 192
 BVH_NODE_BYTES = 32
-KERNEL_COUNT = 15
+KERNEL_COUNT = 16
 
 MPM_OK = 0
 MPM_E_INVALID = -1
@@ -99,6 +99,7 @@ _SIGS = [
     ("mpm_dump_blocks", ctypes.c_int, [_P, _P, _SZ, ctypes.POINTER(_SZ)]),
     ("mpm_dump_particle_blocks", ctypes.c_int, [_P, _P, _P, _SZ, ctypes.POINTER(_SZ)]),
     ("mpm_selftest", ctypes.c_int, [ctypes.c_int, _P, _P, _P, _SZ]),
+    ("mpm_divisor_fast", ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_int)]),
     ("mpm_get_stats", ctypes.c_int, [_P, ctypes.POINTER(MpmStats)]),
     ("mpm_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_get_timings", ctypes.c_int, [_P, ctypes.POINTER(MpmTimings)]),

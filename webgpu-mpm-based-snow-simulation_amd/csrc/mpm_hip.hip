@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 #include <algorithm>
+#include <map>
 #include "mpm_kernels.h"
 #include "../../include/mpm_hip.h"
 
@@ -39,6 +40,7 @@ struct mpm_ctx {
     uint32_t *table = nullptr, *slot_val = nullptr, *nbr_slot = nullptr, *nbr = nullptr, *block_key = nullptr;
     int4* grid[3] = {nullptr, nullptr, nullptr};
     uint32_t* counters = nullptr;
+    uint32_t* cqueue = nullptr;
     uint32_t* coll_data = nullptr;
     uint4* bvh = nullptr;
     size_t n_coll_words = 0, n_bvh_nodes = 0;
@@ -53,6 +55,8 @@ struct mpm_ctx {
     mpm_timings tm{};
     uint64_t substeps = 0;
     std::vector<void*> allocs;
+    std::map<uint64_t, bool> div_verified;   // (divisor, scale) bits -> fixed_div exact for all inputs
+    uint32_t* d_scratch = nullptr;           // one counter for k_verify_div
     // per-kernel event timing (eager launches, one event after every kernel)
     bool ktiming = false;
     std::vector<hipEvent_t> kt_ev;
@@ -67,9 +71,10 @@ namespace {
 const char* k_version = "libmpm_hip 0.1 (gfx950)";
 const char* k_kernel_names[MPM_KERNEL_COUNT] = {
     "k_keys_count", "k_scan_tiles", "k_scan_top", "k_scan_apply", "k_scatter", "k_insert", "k_compact", "k_nbr",
-    "k_pbmpm<FIRST>", "k_pbmpm<MID>", "k_pbmpm<LAST>", "k_explicit_p2g", "k_grid_update", "k_explicit_g2p", "k_straggle"};
+    "k_pbmpm<FIRST>", "k_pbmpm<MID>", "k_pbmpm<LAST>", "k_explicit_p2g", "k_grid_update", "k_explicit_g2p", "k_straggle",
+    "k_collide"};
 enum { KID_KEYS = 0, KID_SCAN_TILES, KID_SCAN_TOP, KID_SCAN_APPLY, KID_SCATTER, KID_INSERT, KID_COMPACT, KID_NBR,
-       KID_PB_FIRST, KID_PB_MID, KID_PB_LAST, KID_EX_P2G, KID_EX_GRID, KID_EX_G2P, KID_STRAGGLE };
+       KID_PB_FIRST, KID_PB_MID, KID_PB_LAST, KID_EX_P2G, KID_EX_GRID, KID_EX_G2P, KID_STRAGGLE, KID_COLLIDE };
 
 // Records an event after a kernel launch when per-kernel timing is on (the first event of a
 // step is the start mark, id -1).
@@ -130,8 +135,28 @@ Bufs make_bufs(mpm_ctx* c) {
     B.table = c->table; B.slot_val = c->slot_val; B.nbr_slot = c->nbr_slot; B.nbr = c->nbr; B.block_key = c->block_key;
     B.grid0 = c->grid[0]; B.grid1 = c->grid[1]; B.grid2 = c->grid[2];
     B.counters = c->counters;
+    B.cqueue = c->cqueue;
     B.coll_data = c->coll_data; B.bvh = c->bvh;
     return B;
+}
+
+// Checks on the device, once per (divisor, fixed-point scale) pair, that fixed_div's FMA form
+// gives i32((x / d) * S) for every float x; the kernels use it only for pairs that pass.
+int verify_divisor(mpm_ctx* c, float d, float S, uint32_t* ok) {
+    uint32_t db, sb; std::memcpy(&db, &d, 4); std::memcpy(&sb, &S, 4);
+    const uint64_t key = ((uint64_t)db << 32) | sb;
+    auto it = c->div_verified.find(key);
+    if (it != c->div_verified.end()) { *ok = it->second ? 1u : 0u; return MPM_OK; }
+    *ok = 0;
+    if (!(d > 0.0f) || d > 1e30f || d < 1e-30f || !(S > 0.0f) || S > 1e30f) { c->div_verified[key] = false; return MPM_OK; }
+    HIPCHK(c, hipMemsetAsync(c->d_scratch, 0, 4, c->stream));
+    k_verify_fixed_div<<<8192, 256, 0, c->stream>>>(d, 1.0f / d, S, c->d_scratch);
+    uint32_t bad = 1;
+    HIPCHK(c, hipMemcpyAsync(&bad, c->d_scratch, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->div_verified[key] = bad == 0;
+    *ok = bad == 0 ? 1u : 0u;
+    return MPM_OK;
 }
 
 // Parse the uniform blob (GpuUniformsBufferManager.ts:54-182 offsets) + creation config.
@@ -181,6 +206,11 @@ int make_params(mpm_ctx* c, SimParams& P) {
     P.inv_density = (float)(1.0 / g.density);
     P.grad_scale = g.grad_scale;
     P.force_div = g.pbmpm_force_divisor;
+    for (int a = 0; a < 3; ++a) P.grav_div[a] = P.grav[a] / P.force_div;
+    for (int a = 0; a < 3; ++a) P.grav_nz[a] = P.grav_div[a] != 0.0f ? 1u : 0u;
+    P.rdt = 1.0f / P.dt;
+    int vr;
+    if ((vr = verify_divisor(c, P.dt, P.S, &P.fast_dt))) return vr;
     P.wall_mode = g.wall_mode;
     P.wall_cells = g.wall_cells;
     P.n_bvh_nodes = (uint32_t)c->n_bvh_nodes;
@@ -256,6 +286,7 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
         if (full) hipEventRecord(c->ev[2], s);
         k_explicit_g2p<0><<<G, 256, 0, s>>>(P, B); kt(c, KID_EX_G2P);
     }
+    if (P.has_collider) { k_collide<<<gs, 256, 0, s>>>(P, B); kt(c, KID_COLLIDE); }
     k_straggle<<<gs, 256, 0, s>>>(P, B); kt(c, KID_STRAGGLE);
     if (full) hipEventRecord(c->ev[3], s);
     return true;
@@ -408,7 +439,7 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
         (rc = dalloc(c, &c->block_key, c->max_blocks)) ||
         (rc = dalloc(c, &c->grid[0], 64 * (size_t)c->max_blocks)) || (rc = dalloc(c, &c->grid[1], 64 * (size_t)c->max_blocks)) ||
         (rc = dalloc(c, &c->grid[2], 64 * (size_t)c->max_blocks)) ||
-        (rc = dalloc(c, &c->counters, C_NCOUNTERS))) {
+        (rc = dalloc(c, &c->counters, C_NCOUNTERS)) || (rc = dalloc(c, &c->cqueue, c->stride)) || (rc = dalloc(c, &c->d_scratch, 4))) {
         release(c); delete c; return rc;
     }
     // zero-initialised key-space accumulators and counters (kernels keep them zero afterwards)
@@ -712,6 +743,15 @@ int mpm_get_stats(mpm_ctx* c, mpm_stats* out) {
     out->probe_failures = h[C_PROBEFAIL];
     out->substeps = c->substeps;
     return MPM_OK;
+}
+
+int mpm_divisor_fast(mpm_ctx* c, float d, float scale, int* fast) {
+    if (!c || !fast) return MPM_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    uint32_t ok = 0;
+    int rc = verify_divisor(c, d, scale, &ok);
+    *fast = (int)ok;
+    return rc;
 }
 
 int mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n) {

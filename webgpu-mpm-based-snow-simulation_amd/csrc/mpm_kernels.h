@@ -35,7 +35,7 @@ enum Plane : int {
 };
 
 enum Counter : int {
-    C_NVALID = 0, C_NRUNS, C_NCHUNKS, C_NBLOCKS, C_STRAG, C_DROPS, C_PROBEFAIL, C_SUBSTEPS, C_NCOUNTERS = 16
+    C_NVALID = 0, C_NRUNS, C_NCHUNKS, C_NBLOCKS, C_STRAG, C_DROPS, C_PROBEFAIL, C_SUBSTEPS, C_NCOLL, C_NCOUNTERS = 16
 };
 
 struct SimParams {
@@ -51,6 +51,10 @@ struct SimParams {
     // material / method
     float mu0, la0, xi_e, xi_p, blend, one_minus_blend, vol_exp, plast_lo, plast_hi, inv_density, grad_scale;
     float force_div, grav[3];
+    float grav_div[3];         // grav / force_div (the per-node force of pbmpm.wgsl:137 without a pointer)
+    float rdt;                 // RN(1/dt) for fixed_div
+    uint32_t fast_dt;          // fixed_div(., dt, rdt, S) verified equal to i32((x/dt) S) for all x
+    uint32_t grav_nz[3];       // grav_div component is nonzero (zero ones contribute i32 0 exactly)
     uint32_t wall_mode, wall_cells;
     // collider (uniforms @224..@432)
     float cmin[3], cmax[3], T[16], Tinv[16], cvel[3];
@@ -72,6 +76,7 @@ struct Bufs {
     uint32_t* table; uint32_t* slot_val; uint32_t* nbr_slot; uint32_t* nbr; uint32_t* block_key;
     int4* grid0; int4* grid1; int4* grid2;
     uint32_t* counters;
+    uint32_t* cqueue;          // sorted indices of particles whose path reaches the collider's box
     const uint32_t* coll_data; const uint4* bvh;
 };
 
@@ -235,7 +240,7 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
     const uint32_t gsz = gridDim.x * blockDim.x;
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t s = gi; s < P.cap; s += gsz) B.table[s] = EMPTY;
-    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; }
+    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; }
     const uint32_t n = P.n;
     const uint32_t nloop = (n + gsz - 1) / gsz;
     for (uint32_t it = 0; it < nloop; ++it) {
@@ -498,20 +503,28 @@ MPM_DEV v3 collider_vertex(const SimParams& P, const uint32_t* cd, uint32_t idx)
 }
 MPM_DEV float fsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
 
-// colliderPrelude.wgsl:153-356: BVH depth-first traversal (stack of 32, left pushed first),
-// nearest-point and swept-ray tests per leaf triangle, then the velocity response.
-MPM_DEV void resolve_collision(const SimParams& P, const Bufs& B, Part& pt) {
-    if (!P.has_collider) return;
+// colliderPrelude.wgsl:154-172: broad phase — the particle's path box against the collider's
+// transformed bounds grown by the margin. Particles that fail it leave resolveParticleCollision
+// unchanged.
+MPM_DEV bool collision_candidate(const SimParams& P, v3 pos, v3 dx) {
+    if (!P.has_collider) return false;
     const v3 min_b = xform(P.T, mk(P.cmin[0], P.cmin[1], P.cmin[2]));
     const v3 max_b = xform(P.T, mk(P.cmax[0], P.cmax[1], P.cmax[2]));
+    const v3 mg = mk(P.margin, P.margin, P.margin);
+    const v3 smin = vmin(min_b, max_b) - mg, smax = vmax(min_b, max_b) + mg;
+    const v3 prev = pos - dx;
+    const v3 minp = vmin(pos, prev), maxp = vmax(pos, prev);
+    if (minp.x > smax.x || minp.y > smax.y || minp.z > smax.z || maxp.x < smin.x || maxp.y < smin.y || maxp.z < smin.z) return false;
+    return P.coll_num_objects != 0u;
+}
+
+// colliderPrelude.wgsl:174-356 for a broad-phase candidate: BVH depth-first traversal (stack of
+// 32, left pushed first), nearest-point and swept-ray tests per leaf triangle, velocity response.
+MPM_DEV void resolve_collision_narrow(const SimParams& P, const Bufs& B, Part& pt) {
     const float margin = P.margin;
     const v3 mg = mk(margin, margin, margin);
-    const v3 smin = vmin(min_b, max_b) - mg, smax = vmax(min_b, max_b) + mg;
     const v3 cur = pt.pos;
     const v3 prev = cur - pt.dx;
-    const v3 minp = vmin(cur, prev), maxp = vmax(cur, prev);
-    if (minp.x > smax.x || minp.y > smax.y || minp.z > smax.z || maxp.x < smin.x || maxp.y < smin.y || maxp.z < smin.z) return;
-    if (P.coll_num_objects == 0u) return;
     float min_d2 = 1e20f;
     v3 cl_n = mk(0, 0, 1), cl_p = cur;
     float min_t = 1.0f;
@@ -589,8 +602,9 @@ MPM_DEV void resolve_collision(const SimParams& P, const Bufs& B, Part& pt) {
     }
 }
 
-// integrateParticles.wgsl:12-72
-MPM_DEV void integrate(const SimParams& P, const Bufs& B, Part& p) {
+// integrateParticles.wgsl:12-66 — everything before resolveParticleCollision. Returns whether the
+// particle is a collider candidate (its narrow phase then runs in k_collide).
+MPM_DEV bool integrate_body(const SimParams& P, Part& p) {
     p.pos = p.pos + p.dx;
     p.Fe = mmul(madd(ident(), p.D), p.Fe);
     plasticity(P, p);
@@ -600,7 +614,34 @@ MPM_DEV void integrate(const SimParams& P, const Bufs& B, Part& p) {
         if (pp[a] < P.gmin[a]) { dd[a] = dd[a] * -0.5f; pp[a] = P.gmin[a]; vv[a] = vv[a] * -0.5f; }
         if (pp[a] >= P.gmax[a]) { dd[a] = dd[a] * -0.5f; pp[a] = P.gmax[a]; vv[a] = vv[a] * -0.5f; }
     }
-    resolve_collision(P, B, p);
+    return collision_candidate(P, p.pos, p.dx);
+}
+// integrateParticles.wgsl:12-72 in one piece.
+MPM_DEV void integrate(const SimParams& P, const Bufs& B, Part& p) {
+    if (integrate_body(P, p)) resolve_collision_narrow(P, B, p);
+}
+// Appends sorted index q to the collider queue (one atomic per wave).
+MPM_DEV void queue_collider(const Bufs& B, bool cand, uint32_t q) {
+    const uint64_t m = __ballot(cand);
+    if (!m) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t b = 0;
+    if (lane_id() == leader) b = atomicAdd(&B.counters[C_NCOLL], (uint32_t)__popcll(m));
+    b = __shfl(b, leader, 64);
+    if (cand) B.cqueue[b + (uint32_t)__popcll(m & lanemask_lt())] = q;
+}
+// resolveParticleCollision (colliderPrelude.wgsl:174-356) for the queued particles: reads and
+// writes x, v, dx of the new-order state.
+__global__ __launch_bounds__(256) void k_collide(SimParams P, Bufs B) {
+    const uint32_t nq = B.counters[C_NCOLL];
+    const uint32_t sd = B.stride;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+        const uint32_t q = B.cqueue[i];
+        Part p;
+        p.pos = ld3(B.nxt, sd, PX, q); p.dx = ld3(B.nxt, sd, DXX, q); p.vel = ld3(B.nxt, sd, VX, q);
+        resolve_collision_narrow(P, B, p);
+        st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, DXX, q, p.dx); st3(B.nxt, sd, VX, q, p.vel);
+    }
 }
 
 // pbmpm.wgsl:16-39 (relax = 0.5 + 0.5(1 - exp(-xi / det F_p)) precomputed per substep)
@@ -650,8 +691,9 @@ MPM_DEV bool sticky_cell(const SimParams& P, int x, int y, int z) {
 // atomic per node and channel per particle) and flushes each node with one global atomic per
 // channel, and G2P reads node velocities from LDS.
 struct Tile {
-    float4 vel[216];       // G2P node velocity (w = 1 if the node has mass)
-    int acc[4][216];       // P2G accumulators: mass, px, py, pz
+    float4 vel[216];       // node velocity; w = 0 node outside the grid or not allocated,
+                           // 1 allocated without mass, 2 allocated with mass (G2P reads it)
+    int acc[4][217];       // P2G accumulators: mass, px, py, pz (+ one sink entry for skipped nodes)
     int node[216];         // global cell index of the node, -1 if out of range / not allocated
     uint32_t nbr[27];
     int base[3];
@@ -681,13 +723,14 @@ MPM_DEV void tile_setup(const SimParams& P, const Bufs& B, Tile& T, uint2 ch) {
         if (in_range(P, x, y, z) && blk != EMPTY)
             idx = (int)(blk * 64u + (uint32_t)(((tx + 3) & 3) + 4 * ((ty + 3) & 3) + 16 * ((tz + 3) & 3)));
         T.node[t] = idx;
+        T.vel[t] = make_float4(0.f, 0.f, 0.f, idx >= 0 ? 1.0f : 0.0f);
     }
 }
 // node velocity as gridToParticle.cs.wgsl:43-49 / :102-108 computes it
 MPM_DEV void tile_load_vel(const SimParams& P, Tile& T, const int4* __restrict__ g) {
     for (int t = threadIdx.x; t < 216; t += blockDim.x) {
         const int idx = T.node[t];
-        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 v = make_float4(0.f, 0.f, 0.f, idx >= 0 ? 1.0f : 0.0f);
         if (idx >= 0) {
             const int4 q = g[idx];
             const float m = (float)q.x / P.S;
@@ -695,7 +738,7 @@ MPM_DEV void tile_load_vel(const SimParams& P, Tile& T, const int4* __restrict__
                 v.x = ((float)q.y / P.S) / m;
                 v.y = ((float)q.z / P.S) / m;
                 v.z = ((float)q.w / P.S) / m;
-                v.w = 1.0f;
+                v.w = 2.0f;
                 if (P.wall_mode & 1u) {
                     const int tx = t % 6, ty = (t / 6) % 6, tz = t / 36;
                     if (sticky_cell(P, T.base[0] + tx, T.base[1] + ty, T.base[2] + tz)) { v.x = 0.f; v.y = 0.f; v.z = 0.f; }
@@ -706,7 +749,7 @@ MPM_DEV void tile_load_vel(const SimParams& P, Tile& T, const int4* __restrict__
     }
 }
 MPM_DEV void tile_zero_acc(Tile& T) {
-    for (int t = threadIdx.x; t < 216; t += blockDim.x) { T.acc[0][t] = 0; T.acc[1][t] = 0; T.acc[2][t] = 0; T.acc[3][t] = 0; }
+    for (int t = threadIdx.x; t < 217; t += blockDim.x) { T.acc[0][t] = 0; T.acc[1][t] = 0; T.acc[2][t] = 0; T.acc[3][t] = 0; }
 }
 MPM_DEV void tile_flush(Tile& T, int4* __restrict__ g) {
     for (int t = threadIdx.x; t < 216; t += blockDim.x) {
@@ -735,44 +778,100 @@ MPM_DEV void stencil(const SimParams& P, const Tile& T, v3 pos, Stencil& s) {
     s.t0[0] = s.c0[0] - T.base[0]; s.t0[1] = s.c0[1] - T.base[1]; s.t0[2] = s.c0[2] - T.base[2];
 }
 
-// gridToParticle.cs.wgsl:82-130 from the LDS velocity tile
-MPM_DEV void g2p_pbmpm(const SimParams& P, const Tile& T, const Stencil& s, v3& nv, m3& Bm) {
+// Per-particle stencil offsets in node units: off[a][o] = f32(c0 + o - 1) - (c0 + f - 1/2), the
+// cell_particle_offset of pbmpm.wgsl:64 and dist of gridToParticle.cs.wgsl:117 (same values).
+MPM_DEV void stencil_offsets(const Stencil& s, float off[3][3]) {
     const v3 pcp = (mk((float)s.c0[0], (float)s.c0[1], (float)s.c0[2]) + s.f) - mk(0.5f, 0.5f, 0.5f);
-    nv = mk(0, 0, 0);
-    Bm = zero3();
-    for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
-        const float4 q = T.vel[tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz)];
-        if (q.w == 0.0f) continue;
-        const v3 v = mk(q.x, q.y, q.z);
-        const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
-        nv = nv + cw * v;
-        const v3 dist = mk((float)(s.c0[0] + ox), (float)(s.c0[1] + oy), (float)(s.c0[2] + oz)) - pcp;
-        Bm = madd(Bm, outer(cw * v, dist));
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        off[0][o] = (float)(s.c0[0] + o - 1) - pcp.x;
+        off[1][o] = (float)(s.c0[1] + o - 1) - pcp.y;
+        off[2][o] = (float)(s.c0[2] + o - 1) - pcp.z;
     }
 }
+
+// gridToParticle.cs.wgsl:82-130 from the LDS velocity tile. Nodes are visited in the reference
+// order (z, y, x) so the FP32 sums round identically. Returns the 27-bit set of nodes P2G may
+// deposit to (allocated and in range).
+MPM_DEV uint32_t g2p_pbmpm(const Tile& T, const Stencil& s, const float off[3][3], v3& nv, m3& Bm) {
+    nv = mk(0, 0, 0);
+    Bm = zero3();
+    uint32_t ok = 0;
+    const int tb = tile_of(s.t0[0] - 1, s.t0[1] - 1, s.t0[2] - 1);
+#pragma unroll
+    for (int oz = 0; oz < 3; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox) {
+        const float4 q = T.vel[tb + ox + 6 * oy + 36 * oz];
+        if (q.w != 0.0f) ok |= 1u << (ox + 3 * oy + 9 * oz);
+        if (q.w == 2.0f) {
+            const float cw = (s.w[ox].x * s.w[oy].y) * s.w[oz].z;
+            const v3 cv = cw * mk(q.x, q.y, q.z);
+            nv = nv + cv;
+            Bm.c[0] = Bm.c[0] + cv * off[0][ox];
+            Bm.c[1] = Bm.c[1] + cv * off[1][oy];
+            Bm.c[2] = Bm.c[2] + cv * off[2][oz];
+        }
+    }
+    return ok;
+}
+// Deposit set of a particle from the tile flags (when no G2P precedes the P2G).
+MPM_DEV uint32_t deposit_mask(const Tile& T, const Stencil& s) {
+    uint32_t ok = 0;
+    const int tb = tile_of(s.t0[0] - 1, s.t0[1] - 1, s.t0[2] - 1);
+#pragma unroll
+    for (int j = 0; j < 27; ++j)
+        if (T.vel[tb + (j % 3) + 6 * ((j / 3) % 3) + 36 * (j / 9)].w != 0.0f) ok |= 1u << j;
+    return ok;
+}
+
 // pbmpm.wgsl:41-76 + :79-147 into the LDS accumulators. The two i32 contributions of a node's
 // momentum (transfer and force) are added before the atomic: integer addition is associative,
-// so the node sums are the same bits as the reference's separate atomics.
-MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, float mass, v3 dx, const m3& D) {
-    const v3 pcp = (mk((float)s.c0[0], (float)s.c0[1], (float)s.c0[2]) + s.f) - mk(0.5f, 0.5f, 0.5f);
+// so the node sums are the same bits as the reference's separate atomics. Each product keeps the
+// reference operand order; the per-axis factors D.c[a] * off[a][o] are shared by the nodes that
+// use them, which changes no rounding.
+MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, const float off[3][3], uint32_t ok,
+                       float mass, v3 dx, const m3& D) {
+    v3 ax[3], by[3], cz[3];
+#pragma unroll
+    for (int o = 0; o < 3; ++o) { ax[o] = D.c[0] * off[0][o]; by[o] = D.c[1] * off[1][o]; cz[o] = D.c[2] * off[2][o]; }
+    const v3 gdiv = mk(P.grav_div[0], P.grav_div[1], P.grav_div[2]);
     const v3 g0 = mk(P.grav[0], P.grav[1], P.grav[2]);
-    for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
-        const int t = tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz);
-        if (T.node[t] < 0) continue;
-        const int cx = s.c0[0] + ox, cy = s.c0[1] + oy, cz = s.c0[2] + oz;
-        const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
+    const bool fast = P.fast_dt != 0;
+    const bool inter = P.interacting != 0;
+    const int tb = tile_of(s.t0[0] - 1, s.t0[1] - 1, s.t0[2] - 1);
+#pragma unroll
+    for (int oz = 0; oz < 3; ++oz)
+#pragma unroll
+    for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox) {
+        const int j = ox + 3 * oy + 9 * oz;
+        const int t = ((ok >> j) & 1u) ? tb + ox + 6 * oy + 36 * oz : 216;
+        const float cw = (s.w[ox].x * s.w[oy].y) * s.w[oz].z;
         const float wm = cw * mass;
-        const v3 off = mk((float)cx, (float)cy, (float)cz) - pcp;
-        const v3 aff = mvmul(D, off);
-        const v3 mom = (wm * (dx + aff)) / P.dt;
-        v3 forces = g0;
-        if (P.interacting) forces = forces + pointer_force(P, cx, cy, cz);
-        forces = forces / P.force_div;
-        const v3 mc = (forces * wm) * P.dt;
+        const v3 aff = (ax[ox] + by[oy]) + cz[oz];
+        const v3 num = wm * (dx + aff);
+        int32_t px = fixed_div(num.x, P.dt, P.rdt, P.S, fast);
+        int32_t py = fixed_div(num.y, P.dt, P.rdt, P.S, fast);
+        int32_t pz = fixed_div(num.z, P.dt, P.rdt, P.S, fast);
+        if (inter) {
+            const v3 forces = (g0 + pointer_force(P, s.c0[0] + ox - 1, s.c0[1] + oy - 1, s.c0[2] + oz - 1)) / P.force_div;
+            const v3 mc = (forces * wm) * P.dt;
+            px = (int32_t)((uint32_t)px + (uint32_t)f2i(mc.x * P.S));
+            py = (int32_t)((uint32_t)py + (uint32_t)f2i(mc.y * P.S));
+            pz = (int32_t)((uint32_t)pz + (uint32_t)f2i(mc.z * P.S));
+        } else {
+            if (P.grav_nz[0]) px = (int32_t)((uint32_t)px + (uint32_t)f2i(((gdiv.x * wm) * P.dt) * P.S));
+            if (P.grav_nz[1]) py = (int32_t)((uint32_t)py + (uint32_t)f2i(((gdiv.y * wm) * P.dt) * P.S));
+            if (P.grav_nz[2]) pz = (int32_t)((uint32_t)pz + (uint32_t)f2i(((gdiv.z * wm) * P.dt) * P.S));
+        }
         atomicAdd(&T.acc[0][t], f2i(wm * P.S));
-        atomicAdd(&T.acc[1][t], (int)((uint32_t)f2i(mom.x * P.S) + (uint32_t)f2i(mc.x * P.S)));
-        atomicAdd(&T.acc[2][t], (int)((uint32_t)f2i(mom.y * P.S) + (uint32_t)f2i(mc.y * P.S)));
-        atomicAdd(&T.acc[3][t], (int)((uint32_t)f2i(mom.z * P.S) + (uint32_t)f2i(mc.z * P.S)));
+        atomicAdd(&T.acc[1][t], px);
+        atomicAdd(&T.acc[2][t], py);
+        atomicAdd(&T.acc[3][t], pz);
     }
 }
 
@@ -785,7 +884,7 @@ enum { PB_FIRST = 0, PB_MID = 1, PB_LAST = 2 };
 //   LAST:  G2P of iteration k-1 from grid[gin], integrateParticles, write the particle state.
 // Every pass also clears grid[gclr] (the buffer the next pass writes) over the live blocks.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr) {
+__global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr) {
     __shared__ Tile T;
     const uint32_t nchunks = B.counters[C_NCHUNKS];
     if (gclr >= 0) {
@@ -802,9 +901,13 @@ __global__ __launch_bounds__(256) void k_pbmpm(SimParams P, Bufs B, int gin, int
         if (MODE != PB_LAST) tile_zero_acc(T);
         __syncthreads();
         const bool ok = T.center_ok != 0;
-        for (uint32_t i = threadIdx.x; i < T.count; i += blockDim.x) {
+        const uint32_t rounds = (T.count + blockDim.x - 1) / blockDim.x;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            const uint32_t i = r * blockDim.x + threadIdx.x;
             const uint32_t q = T.start + i;
-            if (MODE == PB_FIRST) {
+            bool cand = false;
+            if (i >= T.count) {
+            } else if (MODE == PB_FIRST) {
                 const uint32_t old = B.perm[q];
                 const v3 pos = ld3(B.cur, sd, PX, old);
                 const float mass = ld(B.cur, sd, PM, old);
@@ -819,24 +922,27 @@ __global__ __launch_bounds__(256) void k_pbmpm(SimParams P, Bufs B, int gin, int
                 B.relax[q] = relax;
                 if (ok) {
                     Stencil s; stencil(P, T, pos, s);
+                    float off[3][3]; stencil_offsets(s, off);
                     solve_constraints(P, Fe, relax, D);
-                    p2g_pbmpm(P, T, s, mass, dx, D);
+                    p2g_pbmpm(P, T, s, off, deposit_mask(T, s), mass, dx, D);
                 } else {   // run whose own block is not allocated: carried unchanged to LAST
                     stm(B.nxt, sd, DD, q, D); st3(B.nxt, sd, DXX, q, dx); st3(B.nxt, sd, VX, q, ld3(B.cur, sd, VX, old));
                 }
             } else if (MODE == PB_MID) {
-                if (!ok) continue;
+                if (ok) {
                 const v3 pos = ld3(B.nxt, sd, PX, q);
                 const float mass = ld(B.nxt, sd, PM, q);
                 const m3 Fe = ldm(B.nxt, sd, FE, q);
                 const float relax = B.relax[q];
                 Stencil s; stencil(P, T, pos, s);
+                float off[3][3]; stencil_offsets(s, off);
                 v3 nv; m3 Bm;
-                g2p_pbmpm(P, T, s, nv, Bm);
+                const uint32_t dep = g2p_pbmpm(T, s, off, nv, Bm);
                 const v3 dx = nv * P.dt;
                 m3 D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                 solve_constraints(P, Fe, relax, D);
-                p2g_pbmpm(P, T, s, mass, dx, D);
+                p2g_pbmpm(P, T, s, off, dep, mass, dx, D);
+                }
             } else {
                 Part p;
                 p.pos = ld3(B.nxt, sd, PX, q);
@@ -845,19 +951,21 @@ __global__ __launch_bounds__(256) void k_pbmpm(SimParams P, Bufs B, int gin, int
                 p.Fp = ldm(B.nxt, sd, FP, q);
                 if (ok) {
                     Stencil s; stencil(P, T, p.pos, s);
+                    float off[3][3]; stencil_offsets(s, off);
                     v3 nv; m3 Bm;
-                    g2p_pbmpm(P, T, s, nv, Bm);
+                    g2p_pbmpm(T, s, off, nv, Bm);
                     p.dx = nv * P.dt;
                     p.D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                     p.vel = nv;
                 } else {
                     p.dx = ld3(B.nxt, sd, DXX, q); p.D = ldm(B.nxt, sd, DD, q); p.vel = ld3(B.nxt, sd, VX, q);
                 }
-                integrate(P, B, p);
+                cand = integrate_body(P, p);
                 st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
                 stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
                 st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
             }
+            if (MODE == PB_LAST) queue_collider(B, cand, q);
         }
         __syncthreads();
         if (MODE != PB_LAST) tile_flush(T, go);
@@ -955,8 +1063,12 @@ __global__ __launch_bounds__(256) void k_explicit_g2p(SimParams P, Bufs B) {
         tile_load_vel(P, T, B.grid0);
         __syncthreads();
         const bool ok = T.center_ok != 0;
-        for (uint32_t i = threadIdx.x; i < T.count; i += blockDim.x) {
+        const uint32_t rounds = (T.count + blockDim.x - 1) / blockDim.x;
+        for (uint32_t r = 0; r < rounds; ++r) {
+            const uint32_t i = r * blockDim.x + threadIdx.x;
             const uint32_t q = T.start + i;
+            bool cand = false;
+            if (i < T.count) {
             Part p;
             p.pos = ld3(B.nxt, sd, PX, q);
             p.mass = ld(B.nxt, sd, PM, q);
@@ -969,7 +1081,7 @@ __global__ __launch_bounds__(256) void k_explicit_g2p(SimParams P, Bufs B) {
                 m3 G = zero3();
                 for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
                     const float4 qv = T.vel[tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz)];
-                    if (qv.w == 0.0f) continue;
+                    if (qv.w != 2.0f) continue;
                     const v3 v = mk(qv.x, qv.y, qv.z);
                     const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
                     nv = nv + cw * v;
@@ -985,10 +1097,12 @@ __global__ __launch_bounds__(256) void k_explicit_g2p(SimParams P, Bufs B) {
             } else {
                 p.dx = ld3(B.nxt, sd, DXX, q); p.D = ldm(B.nxt, sd, DD, q); p.vel = ld3(B.nxt, sd, VX, q);
             }
-            integrate(P, B, p);
+            cand = integrate_body(P, p);
             st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
             stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
             st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
+            }
+            queue_collider(B, cand, q);
         }
         __syncthreads();
     }
@@ -1019,6 +1133,18 @@ __global__ void k_copy_state(Bufs B, uint32_t n) {
     if (i >= n) return;
     for (int pl = 0; pl < NPLANES; ++pl) B.nxt[(size_t)pl * B.stride + i] = B.cur[(size_t)pl * B.stride + i];
     B.pid_nxt[i] = B.pid_cur[i];
+}
+
+// Counts the float inputs x (all 2^32 bit patterns) where fixed_div(x, d, rd, S, true) differs
+// from the reference's i32((x / d) * S).
+__global__ __launch_bounds__(256) void k_verify_fixed_div(float d, float rd, float S, uint32_t* bad) {
+    uint32_t cnt = 0;
+    const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < (1ull << 32); u += gsz) {
+        const float x = as_f((uint32_t)u);
+        if (fixed_div(x, d, rd, S, false) != fixed_div(x, d, rd, S, true)) ++cnt;
+    }
+    if (cnt) atomicAdd(bad, cnt);
 }
 
 // Elementwise evaluation of the device arithmetic for bitwise checks against the host.

@@ -56,11 +56,27 @@ MPM_DEV uint32_t hash4(uint32_t x, uint32_t y, uint32_t z, uint32_t w) {
 MPM_DEV float as_f(uint32_t u) { return __uint_as_float(u); }
 MPM_DEV uint32_t as_u(float f) { return __float_as_uint(f); }
 
+// i32(f32) with WGSL semantics: v_cvt_i32_f32 truncates toward zero, saturates out-of-range
+// values and converts NaN to 0 in hardware (a plain C cast leaves those cases undefined).
 MPM_DEV int32_t f2i(float x) {
-    if (x != x) return 0;
-    if (x >= 2147483648.0f) return 2147483647;
-    if (x <= -2147483648.0f) return (int32_t)0x80000000u;
-    return (int32_t)x;
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// Fixed-point transfer term i32((x / d) * S) for a divisor d fixed for the whole launch, with
+// the quotient formed from rd = RN(1/d) as q0 = RN(xc rd), e = xc - q0 d (exact by FMA),
+// q = RN(q0 + e rd), where xc is x clamped to +-2^100 (NaN passes through). For |x| in
+// [2^-80, 2^80] q is the IEEE quotient; outside it both forms truncate to 0 or saturate. The
+// host uses this form for a (d, S) pair only after a device check over all 2^32 float inputs
+// found it equal to i32((x / d) * S) everywhere (k_verify_fixed_div); otherwise it divides.
+MPM_DEV int32_t fixed_div(float x, float d, float rd, float S, bool fast) {
+    if (!fast) return f2i((x / d) * S);
+    const float ax = __uint_as_float(__float_as_uint(x) & 0x7fffffffu);
+    const float xc = ax > 0x1p100f ? __uint_as_float((__float_as_uint(x) & 0x80000000u) | 0x71800000u) : x;
+    const float q0 = xc * rd;
+    const float e = __builtin_fmaf(-q0, d, xc);
+    return f2i(__builtin_fmaf(e, rd, q0) * S);
 }
 
 MPM_DEV float scale_pow2(float p, int n) {
