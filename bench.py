@@ -77,6 +77,18 @@ def cpu_baseline(mpm, sc, sim, substeps=2):
                       f"oracle FP32 restatement on 1 thread of {cpu_model()} ({os.cpu_count()} logical CPUs)"}
 
 
+def copy_calibration(mpm, nbytes):
+    """One plain device copy kernel of nbytes (k_selftest kind 10): what FETCH_SIZE / WRITE_SIZE
+    report for it under rocprofv3 --pmc calibrates the counters against a copy of known size."""
+    import ctypes
+    lib = mpm.load_library()
+    n = nbytes // 4
+    a = np.ones(n, np.float32)
+    out = np.zeros(n, np.float32)
+    rc = lib.mpm_selftest(10, a.ctypes.data_as(ctypes.c_void_p), None, out.ctypes.data_as(ctypes.c_void_p), n)
+    assert rc == 0 and out[-1] == 1.0
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if one exists."""
     f = ROOT / "profiles" / "traffic.json"
@@ -97,6 +109,8 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--particles", type=int, default=None)
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus N > 1 is not supported "
@@ -163,6 +177,8 @@ def main():
         "kernel_ms_per_substep": {nm: round(v, 5) for nm, v in sorted(per_substep.items(), key=lambda x: -x[1])},
         "eager_ms_per_substep": eager_ms,
     }
+    if args.calibrate_copy_mb:
+        copy_calibration(mpm, args.calibrate_copy_mb << 20)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (oracle, 2 substeps)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(mpm, sc, sim)

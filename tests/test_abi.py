@@ -160,3 +160,13 @@ def test_bvh_culling_changes_no_result(mpm, orc):
         acted = np.abs(a["vel"] - p["vel"]).max(1) > 1e-3
         assert acted.sum() > 100
         assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+
+
+def test_deno_shim_config_offsets_match_the_c_struct(mpm):
+    """The mpm_config byte offsets written by ts/MpmHipBackend.ts equal the C layout."""
+    src = (ROOT / "webgpu-mpm-based-snow-simulation_amd" / "ts" / "MpmHipBackend.ts").read_text()
+    block = src[src.index("const CFG = {"):src.index("} as const;", src.index("const CFG = {"))]
+    pairs = dict((k, int(v)) for k, v in re.findall(r"(\w+): (\d+)", block))
+    assert pairs.pop("size") == ctypes.sizeof(mpm.MpmConfig)
+    for name, off in pairs.items():
+        assert getattr(mpm.MpmConfig, name).offset == off, name

@@ -755,7 +755,7 @@ int mpm_divisor_fast(mpm_ctx* c, float d, float scale, int* fast) {
 }
 
 int mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n) {
-    if (kind < 0 || kind > 9 || !a || !out || n == 0 || n > (1u << 24)) return MPM_E_INVALID;
+    if (kind < 0 || kind > 10 || !a || !out || n == 0 || n > (1u << 25)) return MPM_E_INVALID;
     if ((kind == 2 || kind == 4 || kind == 5) && !b) return MPM_E_INVALID;
     const size_t in_w = (kind >= 6 && kind <= 8) ? 9 : 1;
     const size_t out_w = (kind == 7 || kind == 8) ? 9 : 1;

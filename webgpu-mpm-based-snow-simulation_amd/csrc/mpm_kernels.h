@@ -1149,10 +1149,11 @@ __global__ __launch_bounds__(256) void k_verify_fixed_div(float d, float rd, flo
 
 // Elementwise evaluation of the device arithmetic for bitwise checks against the host.
 // kind: 0 exp(a) 1 log(a) 2 pow(a,b) 3 sqrt(a) 4 a/b 5 hash3(bits a, bits b, i) 6 det3
-//       7 inverse3 8 polar rotation 9 f2i(a) (matrices: 9 floats per item, column-major)
+//       7 inverse3 8 polar rotation 9 f2i(a) 10 copy (matrices: 9 floats per item, column-major)
 __global__ void k_selftest(int kind, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (kind == 10) { out[i] = a[i]; return; }
     if (kind <= 5 || kind == 9) {
         const float x = a[i], y = b ? b[i] : 0.0f;
         float r = 0.0f;
