@@ -37,7 +37,7 @@ struct mpm_ctx {
     uint32_t *tile_sums = nullptr, *tile_ofs = nullptr;
     uint32_t *run_key = nullptr, *run_start = nullptr, *run_count = nullptr, *run_mask = nullptr;
     uint2* chunks = nullptr;
-    uint32_t *table = nullptr, *slot_val = nullptr, *nbr_slot = nullptr, *nbr = nullptr, *block_key = nullptr;
+    uint32_t *table = nullptr, *slot_val = nullptr, *nbr = nullptr, *block_key = nullptr;
     int4* grid[3] = {nullptr, nullptr, nullptr};
     uint32_t* counters = nullptr;
     uint32_t* cqueue = nullptr;
@@ -132,7 +132,7 @@ Bufs make_bufs(mpm_ctx* c) {
     B.tile_sums = c->tile_sums; B.tile_ofs = c->tile_ofs;
     B.run_key = c->run_key; B.run_start = c->run_start; B.run_count = c->run_count; B.run_mask = c->run_mask;
     B.chunks = c->chunks;
-    B.table = c->table; B.slot_val = c->slot_val; B.nbr_slot = c->nbr_slot; B.nbr = c->nbr; B.block_key = c->block_key;
+    B.table = c->table; B.slot_val = c->slot_val; B.nbr = c->nbr; B.block_key = c->block_key;
     B.grid0 = c->grid[0]; B.grid1 = c->grid[1]; B.grid2 = c->grid[2];
     B.counters = c->counters;
     B.cqueue = c->cqueue;
@@ -435,7 +435,7 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
         (rc = dalloc(c, &c->run_count, c->max_runs)) || (rc = dalloc(c, &c->run_mask, c->max_runs)) ||
         (rc = dalloc(c, &c->chunks, c->max_chunks)) ||
         (rc = dalloc(c, &c->table, c->cap)) || (rc = dalloc(c, &c->slot_val, c->cap)) ||
-        (rc = dalloc(c, &c->nbr_slot, 27 * (size_t)c->max_runs)) || (rc = dalloc(c, &c->nbr, 27 * (size_t)c->max_runs)) ||
+        (rc = dalloc(c, &c->nbr, 27 * (size_t)c->max_runs)) ||
         (rc = dalloc(c, &c->block_key, c->max_blocks)) ||
         (rc = dalloc(c, &c->grid[0], 64 * (size_t)c->max_blocks)) || (rc = dalloc(c, &c->grid[1], 64 * (size_t)c->max_blocks)) ||
         (rc = dalloc(c, &c->grid[2], 64 * (size_t)c->max_blocks)) ||

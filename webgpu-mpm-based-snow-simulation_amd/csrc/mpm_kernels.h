@@ -73,7 +73,7 @@ struct Bufs {
     uint32_t* tile_sums; uint32_t* tile_ofs;     // 3 per tile
     uint32_t* run_key; uint32_t* run_start; uint32_t* run_count; uint32_t* run_mask;
     uint2* chunks;
-    uint32_t* table; uint32_t* slot_val; uint32_t* nbr_slot; uint32_t* nbr; uint32_t* block_key;
+    uint32_t* table; uint32_t* slot_val; uint32_t* nbr; uint32_t* block_key;
     int4* grid0; int4* grid1; int4* grid2;
     uint32_t* counters;
     uint32_t* cqueue;          // sorted indices of particles whose path reaches the collider's box
@@ -379,13 +379,11 @@ __global__ __launch_bounds__(256) void k_insert(SimParams P, Bufs B) {
     const uint32_t nvalid = B.counters[C_NVALID];
     const uint32_t gsz = gridDim.x * blockDim.x;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nruns * 27u; t += gsz) {
-        uint32_t r = t / 27u, j = t % 27u;
-        uint32_t slot = EMPTY;
+        const uint32_t r = t / 27u, j = t % 27u;
         if ((B.run_mask[r] >> j) & 1u) {
             int b[3]; block_coords(P, B.run_key[r], b);
-            slot = table_insert(P, B, b[0] + (int)(j % 3u) - 1, b[1] + (int)((j / 3u) % 3u) - 1, b[2] + (int)(j / 9u) - 1);
+            table_insert(P, B, b[0] + (int)(j % 3u) - 1, b[1] + (int)((j / 3u) % 3u) - 1, b[2] + (int)(j / 9u) - 1);
         }
-        B.nbr_slot[t] = slot;
     }
     // unbinned particles still allocate the blocks of their in-range stencil cells
     for (uint32_t p = nvalid + blockIdx.x * blockDim.x + threadIdx.x; p < P.n; p += gsz) {
@@ -417,15 +415,31 @@ MPM_DEV uint32_t n_blocks_of(const SimParams& P, const Bufs& B) {
     uint32_t nb = B.counters[C_NBLOCKS];
     return nb < P.max_blocks ? nb : P.max_blocks;
 }
-// Neighbour-block table of every run (27 dense block indices) and the clear of grid buffer 0
-// over the allocated blocks (clearMappedBlocks.wgsl:9-26 restricted to live blocks).
+// Lookup of a block key after all inserts (sparseGridPrelude.wgsl:35-52 probe sequence on the
+// table's own hash): dense block index, or EMPTY if the block is not allocated.
+MPM_DEV uint32_t table_lookup(const SimParams& P, const Bufs& B, int bx, int by, int bz) {
+    if (bx < 0 || by < 0 || bz < 0 || bx >= P.nb[0] || by >= P.nb[1] || bz >= P.nb[2]) return EMPTY;
+    const uint32_t key = (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+    uint32_t s = hash3((uint32_t)bx, (uint32_t)by, (uint32_t)bz) % P.cap;
+    for (uint32_t i = 0; i < P.max_probes; ++i) {
+        const uint32_t k = B.table[s];
+        if (k == EMPTY) return EMPTY;
+        if (k == key) return B.slot_val[s];
+        s = (s + 1 >= P.cap) ? 0 : s + 1;
+    }
+    return EMPTY;
+}
+// Neighbour-block table of every run (27 dense block indices, allocated by any particle) and the
+// clear of grid buffer 0 over the allocated blocks (clearMappedBlocks.wgsl:9-26 restricted to
+// live blocks).
 __global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) {
     const uint32_t nruns = B.counters[C_NRUNS];
     const uint32_t gsz = gridDim.x * blockDim.x;
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t t = gi; t < nruns * 27u; t += gsz) {
-        uint32_t s = B.nbr_slot[t];
-        B.nbr[t] = s == EMPTY ? EMPTY : B.slot_val[s];
+        const uint32_t r = t / 27u, j = t % 27u;
+        int b[3]; block_coords(P, B.run_key[r], b);
+        B.nbr[t] = table_lookup(P, B, b[0] + (int)(j % 3u) - 1, b[1] + (int)((j / 3u) % 3u) - 1, b[2] + (int)(j / 9u) - 1);
     }
     const uint32_t ncell = n_blocks_of(P, B) * 64u;
     for (uint32_t c = gi; c < ncell; c += gsz) B.grid0[c] = make_int4(0, 0, 0, 0);
