@@ -24,7 +24,7 @@ extern "C" {
 #define MPM_UNIFORMS_BYTES 37376   /* GpuUniformsBufferManager.ts:17 */
 #define MPM_PARTICLE_BYTES 192     /* shaderPrelude/mpmOps.wgsl:1-12, GpuMpmBufferManager.ts:24 */
 #define MPM_BVH_NODE_BYTES 32      /* mpm/colliderPrelude.wgsl:6-11 */
-#define MPM_KERNEL_COUNT 16        /* kernels of one substep, see mpm_kernel_name() */
+#define MPM_KERNEL_COUNT 17        /* kernels of one substep, see mpm_kernel_name() */
 
 enum {
     MPM_OK = 0,
@@ -88,6 +88,7 @@ typedef struct mpm_stats {
     uint64_t blocks_dropped;         /* beyond max_blocks */
     uint64_t probe_failures;         /* hash insert exhausted max_probes */
     uint64_t substeps;
+    uint64_t mlp_fallbacks;          /* particles whose warm-start prediction was dropped (last substep) */
 } mpm_stats;
 
 /* Per-phase device time of the last timed mpm_step (timing enabled with mpm_set_timing). */
@@ -137,8 +138,10 @@ int  mpm_sync(mpm_ctx* ctx);
 
 /* Test hooks (no reference counterpart). mpm_debug_stage runs one substep prefix:
  * explicit: 0 binning, 1 +P2G, 2 +grid update, >=3 full; PBMPM: 0 binning, s in 1..k the grid
- * after the pbmpm dispatch of iteration s-1, > k full. mpm_dump_grid returns every cell of every
+ * after the pbmpm dispatch of iteration s-1, > k full, MPM_STAGE_WARM_START binning and the learned
+ * warm start only (the particle state then holds the warm-started D). mpm_dump_grid returns every cell of every
  * allocated block keyed by cell coordinate with its (mass, mx, my, mz) i32 accumulators. */
+#define MPM_STAGE_WARM_START (-2)
 int  mpm_debug_stage(mpm_ctx* ctx, int method, int stage);
 int  mpm_dump_grid(mpm_ctx* ctx, int32_t* cell_xyz, int32_t* vals4, size_t cap, size_t* n_out);
 int  mpm_dump_blocks(mpm_ctx* ctx, int32_t* block_xyz, size_t cap, size_t* n_out);
@@ -154,6 +157,36 @@ int  mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n
  * a device check over all 2^32 float x found it equal to the IEEE-division form (no reference
  * counterpart; test hook). */
 int  mpm_divisor_fast(mpm_ctx* ctx, float d, float scale, int* fast);
+
+/* PBMPM iteration count and per-iteration force divisor of the following substeps
+ * (GpuMpmPipelineManager.ts:445 nSolveConstraintIterations, pbmpm.wgsl:137 "/ 3.0"). */
+int  mpm_set_iterations(mpm_ctx* ctx, int k, float force_divisor);
+
+/* Learned PBMPM warm start (no reference counterpart; the north_star addition). A 15-64-64-9
+ * GELU MLP with BF16 weights on the matrix cores maps a particle's state at the start of a
+ * substep to the correction of D that k_ref iterations converge to.
+ *  - mpm_learn_collect: n_substeps PBMPM substeps with k_ref iterations (force divisor k_ref),
+ *    logging (features, target) pairs of every particle every `every` substeps, up to max_pairs.
+ *  - mpm_learn_train: Adam on minibatches of `batch` pairs (multiple of 256) drawn with replacement;
+ *    mean_loss = mean squared error of the standardised targets over the last 50 steps.
+ *  - mpm_learn_get/set_params: 6,288 FP32 parameters (W1[16][64] b1 W2[64][64] b2 W3[64][16] b3)
+ *    and 64 standardisation values (feature mean, std, target mean, std; 16 each).
+ *  - mpm_learn_enable: apply the warm start before every PBMPM substep; a particle whose
+ *    standardised prediction exceeds `limit` (or is not finite) keeps the plain D (fallback). */
+int  mpm_learn_collect(mpm_ctx* ctx, int n_substeps, int k_ref, int every, uint64_t max_pairs, uint64_t* n_pairs);
+int  mpm_learn_train(mpm_ctx* ctx, int steps, int batch, float lr, uint32_t seed, float* mean_loss);
+int  mpm_learn_get_params(mpm_ctx* ctx, float* params, float* norm);
+int  mpm_learn_set_params(mpm_ctx* ctx, const float* params, const float* norm);
+int  mpm_learn_enable(mpm_ctx* ctx, int on, float limit);
+/* Gradient of the MSE loss over n samples (n multiple of 16) computed by the MFMA forward and
+ * backward kernels, for checking against a host computation (test hook). */
+int  mpm_mlp_grad(const float* params, const float* norm, const float* feats, const float* targs, uint32_t n,
+                  float* grads, float* y, float* loss);
+
+/* Per-iteration PBMPM residual, r_i = max_particles |dx_i - dx_{i-1}|_inf / largest grid extent,
+ * dx_i from the G2P of iteration i. mean / peak over the substeps since tracking was enabled. */
+int  mpm_set_residual_tracking(mpm_ctx* ctx, int on);
+int  mpm_get_residuals(mpm_ctx* ctx, float* mean, float* peak, int cap, uint64_t* substeps);
 
 int  mpm_get_stats(mpm_ctx* ctx, mpm_stats* out);
 int  mpm_set_timing(mpm_ctx* ctx, int enabled);

@@ -1,0 +1,148 @@
+"""GPU checks of the learned PBMPM warm start (no reference counterpart): the matrix-core forward
+and backward against the numpy oracle (oracle/mlp.py, bfloat16 emulated), the warm-start
+inference applied to particle records, the per-particle fallback, and on-device training."""
+import numpy as np
+import pytest
+
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(seed, out_scale=1.0):
+    p = mlp.init_params(seed, out_scale=out_scale)
+    norm = np.zeros((4, 16), np.float32)
+    norm[0, :15] = np.linspace(-0.5, 0.5, 15)
+    norm[1, :15] = np.linspace(0.5, 2.0, 15)
+    norm[1, 15] = 1.0
+    norm[2, :9] = np.linspace(-1e-3, 1e-3, 9)
+    norm[3, :9] = np.linspace(1e-3, 4e-3, 9)
+    norm[3, 9:] = 1.0
+    return p, norm.ravel()
+
+
+def test_mlp_gradient_matches_oracle(mpm):
+    """Gradient of the MSE loss over 1,024 samples: every layer's weights and biases, the
+    standardised outputs and the loss, against the bfloat16-emulating oracle. Tolerances: outputs
+    2% of their largest magnitude, gradient 2% in norm and 5% of the largest entry per entry
+    (single bfloat16 rounding flips of intermediate activations are 2^-8 relative)."""
+    rng = np.random.default_rng(11)
+    n = 1024
+    p, norm = _net(3)
+    nm = norm.reshape(4, 16)
+    feats = (nm[0] + nm[1] * rng.standard_normal((n, 16))).astype(np.float32)
+    feats[:, 15] = 0
+    targs = (nm[2] + nm[3] * rng.standard_normal((n, 16))).astype(np.float32)
+    targs[:, 9:] = 0
+    g, y, loss = mpm.mlp_grad(p, norm, feats, targs)
+    go, yo, lo = mlp.forward_backward(p, norm, feats, targs, emulate=True)
+    assert np.abs(y - yo).max() <= 0.02 * np.abs(yo).max()
+    assert abs(loss - lo) <= 0.01 * lo
+    for name, a, b in (("W1", mlp.W1, mlp.B1), ("b1", mlp.B1, mlp.W2), ("W2", mlp.W2, mlp.B2),
+                       ("b2", mlp.B2, mlp.W3), ("W3", mlp.W3, mlp.B3), ("b3", mlp.B3, mlp.NPARAM)):
+        ga, gb = g[a:b], go[a:b]
+        assert np.linalg.norm(ga - gb) <= 0.02 * np.linalg.norm(gb) + 1e-12, name
+        assert np.abs(ga - gb).max() <= 0.05 * np.abs(gb).max() + 1e-12, name
+    assert np.all(g[mlp.W1 + 15 * 64:mlp.B1] == 0)
+    assert np.all(g[mlp.W3:mlp.B3].reshape(64, 16)[:, mlp.NOUT:] == 0)
+
+
+def _records(mpm, n, rng):
+    """Particles in a 64^3 unit grid with random near-rotation F_e, F_p, D, velocity, mass."""
+    pos = rng.uniform(0.2, 0.8, (n, 3)).astype(np.float32)
+    p = mpm.make_particles(pos, vel=rng.standard_normal((n, 3)).astype(np.float32),
+                           mass=rng.uniform(0.5, 1.5, n).astype(np.float32))
+    fe = np.empty((n, 3, 3))
+    for i in range(n):
+        q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        q *= np.sign(np.linalg.det(q))
+        fe[i] = q @ np.diag(rng.uniform(0.9, 1.1, 3))
+    fp = np.eye(3) + 0.05 * rng.standard_normal((n, 3, 3))
+    d = 2e-3 * rng.standard_normal((n, 3, 3))
+    p["Fe"][:, :, :3] = np.swapaxes(fe, 1, 2)           # column-major storage
+    p["Fp"][:, :, :3] = np.swapaxes(fp, 1, 2)
+    p["D"][:, :, :3] = np.swapaxes(d, 1, 2)
+    return p, fe, fp, d
+
+
+def test_warmstart_inference_matches_oracle(mpm):
+    """One warm start over 4,000 particles: D + destandardised prediction per particle, against
+    the oracle network on features from an exact polar decomposition (SVD)."""
+    rng = np.random.default_rng(12)
+    n = 4000
+    p, fe, fp, d = _records(mpm, n, rng)
+    params, norm = _net(4, out_scale=0.5)
+    with mpm.MpmSimulation(n, grid_res=(64, 64, 64), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1 / 600,
+                           pbmpm_iterations=2, pbmpm_force_divisor=2.0) as sim:
+        sim.upload(p)
+        sim.set_learn_params(params, norm)
+        sim.learn_enable(True, limit=1e6)
+        sim.debug_stage(1, -2)                           # MPM_STAGE_WARM_START
+        out = sim.download()
+        assert sim.stats()["mlp_fallbacks"] == 0
+    feats = np.stack([mlp.features(fe[i], fp[i], d[i], p["vel"][i], p["mass"][i]) for i in range(n)]).astype(np.float32)
+    _, y, _ = mlp.forward_backward(params, norm, feats, np.zeros((n, 16), np.float32), emulate=True)
+    nm = norm.reshape(4, 16)
+    want = d + (y[:, :9] * nm[3, :9] + nm[2, :9]).reshape(n, 3, 3).transpose(0, 2, 1)   # column-major outputs
+    got = mpm.matrix(out["D"])
+    err = np.abs(got - want).reshape(n, 9) / nm[3, :9]
+    assert np.quantile(err, 0.99) < 0.02 and err.max() < 0.1
+    # everything but D is untouched
+    assert np.array_equal(out["pos"], p["pos"]) and np.array_equal(out["Fe"], p["Fe"])
+
+
+def test_fallback_keeps_plain_pbmpm_bitexact(mpm):
+    """With a limit no prediction meets, every particle falls back: the substeps equal plain PBMPM
+    bit for bit and the fallback count is the particle count."""
+    sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=20000, seed=31)
+    params, norm = _net(5)
+    outs = []
+    for learned in (False, True):
+        with mpm.MpmSimulation(20000, **sc["sim"]) as sim:
+            sim.set_collider(*sc["collider"])
+            sim.upload(sc["particles"])
+            if learned:
+                sim.set_learn_params(params, norm)
+                sim.learn_enable(True, limit=1e-30)
+            sim.step(1, 5)
+            outs.append(sim.download())
+            if learned:
+                assert sim.stats()["mlp_fallbacks"] == 20000
+    assert np.array_equal(outs[0].view(np.uint8), outs[1].view(np.uint8))
+
+
+def test_collect_and_train_reduces_loss(mpm):
+    """Pairs logged from k = 16 substeps, then Adam on the matrix cores: the standardised loss
+    (about 1 per output for an untrained network) falls; the part of the target a per-particle
+    network can explain on this scene is about a third of its variance."""
+    sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=32768, seed=32)
+    with mpm.MpmSimulation(32768, **sc["sim"]) as sim:
+        sim.set_collider(*sc["collider"])
+        sim.upload(sc["particles"])
+        sim.step(1, 20)
+        npairs = sim.learn_collect(40, k_ref=16, every=2)
+        assert npairs == 20 * 32768
+        first = sim.learn_train(50, batch=4096, lr=1e-3, seed=1)
+        later = sim.learn_train(1500, batch=4096, lr=1e-3, seed=2)
+        assert np.isfinite(first) and np.isfinite(later)
+        assert later < 0.85 * first and later < 0.8
+        params, norm = sim.learn_params()
+        assert np.all(np.isfinite(params)) and np.all(norm.reshape(4, 16)[[1, 3]] > 0)
+        sim.learn_enable(True)
+        sim.step(1, 10)
+        assert np.isfinite(mpm.state_columns(sim.download())).all()
+
+
+def test_residual_tracking(mpm):
+    """Per-iteration residuals of plain PBMPM fall with the iteration index on the two-sphere scene."""
+    sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=32768, seed=33)
+    with mpm.MpmSimulation(32768, **sc["sim"]) as sim:
+        sim.set_collider(*sc["collider"])
+        sim.upload(sc["particles"])
+        sim.set_iterations(8)
+        sim.set_residual_tracking(True)
+        sim.step(1, 10)
+        mean, peak, ns = sim.residuals()
+        assert ns == 10
+        assert mean[0] == 0 and np.all(mean[1:8] > 0) and np.all(mean[8:] == 0)
+        assert mean[7] < mean[1] and np.all(peak[1:8] >= mean[1:8])

@@ -13,7 +13,8 @@ LIB_PATH = PKG_DIR / "libmpm_hip.so"
 UNIFORMS_BYTES = 37376
 PARTICLE_BYTES = 192
 BVH_NODE_BYTES = 32
-KERNEL_COUNT = 16
+KERNEL_COUNT = 17
+MLP_NPARAM = 6288
 
 MPM_OK = 0
 MPM_E_INVALID = -1
@@ -66,7 +67,7 @@ class MpmConfig(ctypes.Structure):
 class MpmStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "n_particles", "particles_binned", "particles_unbinned", "particle_blocks", "work_chunks",
-        "blocks_allocated", "blocks_dropped", "probe_failures", "substeps")]
+        "blocks_allocated", "blocks_dropped", "probe_failures", "substeps", "mlp_fallbacks")]
 
 
 class MpmTimings(ctypes.Structure):
@@ -100,6 +101,15 @@ _SIGS = [
     ("mpm_dump_particle_blocks", ctypes.c_int, [_P, _P, _P, _SZ, ctypes.POINTER(_SZ)]),
     ("mpm_selftest", ctypes.c_int, [ctypes.c_int, _P, _P, _P, _SZ]),
     ("mpm_divisor_fast", ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_int)]),
+    ("mpm_set_iterations", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float]),
+    ("mpm_learn_collect", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    ("mpm_learn_train", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]),
+    ("mpm_learn_get_params", ctypes.c_int, [_P, _P, _P]),
+    ("mpm_learn_set_params", ctypes.c_int, [_P, _P, _P]),
+    ("mpm_learn_enable", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float]),
+    ("mpm_mlp_grad", ctypes.c_int, [_P, _P, _P, _P, ctypes.c_uint32, _P, _P, _P]),
+    ("mpm_set_residual_tracking", ctypes.c_int, [_P, ctypes.c_int]),
+    ("mpm_get_residuals", ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     ("mpm_get_stats", ctypes.c_int, [_P, ctypes.POINTER(MpmStats)]),
     ("mpm_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_get_timings", ctypes.c_int, [_P, ctypes.POINTER(MpmTimings)]),

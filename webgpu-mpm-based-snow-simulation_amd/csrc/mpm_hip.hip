@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <map>
 #include "mpm_kernels.h"
+#include "mpm_mlp.h"
 #include "../../include/mpm_hip.h"
 
 using namespace mpm;
@@ -55,6 +56,24 @@ struct mpm_ctx {
     mpm_timings tm{};
     uint64_t substeps = 0;
     std::vector<void*> allocs;
+    int k_cur = 3;                 // PBMPM iterations of the following substeps
+    float force_div_cur = 3.0f;
+    // learned warm start: MLP parameters, Adam state, standardisation, logged dataset
+    float *mlp_params = nullptr, *mlp_grads = nullptr, *mlp_m1 = nullptr, *mlp_m2 = nullptr, *mlp_norm = nullptr;
+    short* mlp_wbf = nullptr;
+    double* mlp_colsum = nullptr;
+    float* mlp_loss = nullptr;
+    float *ds_feats = nullptr, *ds_targs = nullptr;
+    uint64_t ds_cap = 0, ds_n = 0;
+    bool mlp_ready = false, learn_on = false;
+    float learn_limit = 8.0f;
+    uint64_t adam_t = 0;
+    // per-iteration residual tracking
+    float* resid_dx = nullptr;
+    uint32_t *resid_max = nullptr, *resid_peak = nullptr;
+    float* resid_sum = nullptr;
+    bool resid_on = false;
+    uint64_t resid_substeps = 0;
     std::map<uint64_t, bool> div_verified;   // (divisor, scale) bits -> fixed_div exact for all inputs
     uint32_t* d_scratch = nullptr;           // one counter for k_verify_div
     // per-kernel event timing (eager launches, one event after every kernel)
@@ -72,9 +91,9 @@ const char* k_version = "libmpm_hip 0.1 (gfx950)";
 const char* k_kernel_names[MPM_KERNEL_COUNT] = {
     "k_keys_count", "k_scan_tiles", "k_scan_top", "k_scan_apply", "k_scatter", "k_insert", "k_compact", "k_nbr",
     "k_pbmpm<FIRST>", "k_pbmpm<MID>", "k_pbmpm<LAST>", "k_explicit_p2g", "k_grid_update", "k_explicit_g2p", "k_straggle",
-    "k_collide"};
+    "k_collide", "k_mlp_warmstart"};
 enum { KID_KEYS = 0, KID_SCAN_TILES, KID_SCAN_TOP, KID_SCAN_APPLY, KID_SCATTER, KID_INSERT, KID_COMPACT, KID_NBR,
-       KID_PB_FIRST, KID_PB_MID, KID_PB_LAST, KID_EX_P2G, KID_EX_GRID, KID_EX_G2P, KID_STRAGGLE, KID_COLLIDE };
+       KID_PB_FIRST, KID_PB_MID, KID_PB_LAST, KID_EX_P2G, KID_EX_GRID, KID_EX_G2P, KID_STRAGGLE, KID_COLLIDE, KID_MLP };
 
 // Records an event after a kernel launch when per-kernel timing is on (the first event of a
 // step is the start mark, id -1).
@@ -122,6 +141,12 @@ uint32_t next_prime(uint32_t v) {
 template <class T> T rdu(const uint8_t* b, size_t off) { T v; std::memcpy(&v, b + off, sizeof(T)); return v; }
 template <class T> void wru(uint8_t* b, size_t off, T v) { std::memcpy(b + off, &v, sizeof(T)); }
 
+MlpBufs mlp_bufs(mpm_ctx* c) {
+    MlpBufs M;
+    M.params = c->mlp_params; M.wbf = c->mlp_wbf; M.grads = c->mlp_grads; M.norm = c->mlp_norm;
+    return M;
+}
+
 Bufs make_bufs(mpm_ctx* c) {
     Bufs B;
     B.cur = c->planes[c->cur]; B.nxt = c->planes[c->cur ^ 1];
@@ -137,6 +162,9 @@ Bufs make_bufs(mpm_ctx* c) {
     B.counters = c->counters;
     B.cqueue = c->cqueue;
     B.coll_data = c->coll_data; B.bvh = c->bvh;
+    B.resid_dx = c->resid_on ? c->resid_dx : nullptr;
+    B.resid_max = c->resid_max;
+    B.resid_sum = c->resid_sum;
     return B;
 }
 
@@ -205,7 +233,15 @@ int make_params(mpm_ctx* c, SimParams& P) {
     P.plast_hi = (float)(1.0 + g.theta_s);
     P.inv_density = (float)(1.0 / g.density);
     P.grad_scale = g.grad_scale;
-    P.force_div = g.pbmpm_force_divisor;
+    P.force_div = c->force_div_cur;
+    P.k = c->k_cur;
+    P.learn_flags = (c->learn_on && c->mlp_ready ? 1u : 0u) | (c->resid_on ? 2u : 0u);
+    P.learn_limit = c->learn_limit;
+    {
+        float ext = 0.0f;
+        for (int a = 0; a < 3; ++a) ext = std::max(ext, P.gmax[a] - P.gmin[a]);
+        P.inv_domain = 1.0f / ext;
+    }
     for (int a = 0; a < 3; ++a) P.grav_div[a] = P.grav[a] / P.force_div;
     for (int a = 0; a < 3; ++a) P.grav_nz[a] = P.grav_div[a] != 0.0f ? 1u : 0u;
     P.rdt = 1.0f / P.dt;
@@ -222,8 +258,16 @@ int make_params(mpm_ctx* c, SimParams& P) {
     P.fr_prox = g.friction_proximity;
     P.vel_factor = g.collider_velocity_factor;
     if (!(P.dt > 0.0f)) return fail(c, MPM_E_RANGE, "simulationTimestep must be > 0");
-    if (g.pbmpm_iterations < 1) return fail(c, MPM_E_RANGE, "pbmpm_iterations must be >= 1");
+    if (c->k_cur < 1 || c->k_cur > 64) return fail(c, MPM_E_RANGE, "pbmpm iterations must be in 1..64");
     return MPM_OK;
+}
+
+uint32_t hash_host(uint32_t n) {
+    uint32_t h = n + 374761393u;
+    h = 668265263u * ((h << 17) | (h >> 15));
+    h = 2246822519u * (h ^ (h >> 15));
+    h = 3266489917u * (h ^ (h >> 13));
+    return h ^ (h >> 16);
 }
 
 uint32_t grid_for(uint64_t work, uint32_t cap_wg) {
@@ -266,17 +310,24 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
     const uint32_t G = chunk_grid(c);
     const uint32_t gs = 256;
     if (method == MPM_METHOD_PBMPM) {
-        const int k = c->cfg.pbmpm_iterations;
-        k_pbmpm<PB_FIRST><<<G, 256, 0, s>>>(P, B, -1, 0, 1); kt(c, KID_PB_FIRST);
+        const int k = P.k;
+        if (P.learn_flags & 1u) {
+            k_mlp_warmstart<<<std::min<uint32_t>(grid_for(c->n, 1u << 30), (uint32_t)c->n_cu * 2u), 256, 0, s>>>(
+                mlp_bufs(c), B, c->n, P.learn_limit, c->counters + C_FALLBACK);
+            kt(c, KID_MLP);
+        }
+        if (stage == MPM_STAGE_WARM_START) return false;
+        k_pbmpm<PB_FIRST><<<G, 256, 0, s>>>(P, B, -1, 0, 1, -1); kt(c, KID_PB_FIRST);
         c->last_grid = 0;
         if (stage == 1) return false;
         for (int i = 1; i < k; ++i) {
-            k_pbmpm<PB_MID><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3); kt(c, KID_PB_MID);
+            k_pbmpm<PB_MID><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3, i - 1); kt(c, KID_PB_MID);
             c->last_grid = i % 3;
             if (stage == i + 1) return false;
         }
         if (full) hipEventRecord(c->ev[2], s);
-        k_pbmpm<PB_LAST><<<G, 256, 0, s>>>(P, B, (k - 1) % 3, -1, -1); kt(c, KID_PB_LAST);
+        k_pbmpm<PB_LAST><<<G, 256, 0, s>>>(P, B, (k - 1) % 3, -1, -1, k - 1); kt(c, KID_PB_LAST);
+        if (P.learn_flags & 2u) k_resid_roll<<<1, 64, 0, s>>>(B, k, c->resid_peak);
     } else {
         k_explicit_p2g<0><<<G, 256, 0, s>>>(P, B); kt(c, KID_EX_P2G);
         c->last_grid = 0;
@@ -417,6 +468,8 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     if (K >= 0xFFFFFFF0ull) { release(c); delete c; return MPM_E_RANGE; }
     c->K = (uint32_t)K;
     c->chunk = cfg->chunk_particles ? cfg->chunk_particles : 512u;
+    c->k_cur = cfg->pbmpm_iterations;
+    c->force_div_cur = cfg->pbmpm_force_divisor;
     uint64_t mb = cfg->max_blocks ? cfg->max_blocks : std::min<uint64_t>(K, std::max<uint64_t>(8ull * c->n, 64));
     c->max_blocks = (uint32_t)mb;
     c->cap = cfg->hash_map_size ? cfg->hash_map_size : next_prime((uint32_t)std::min<uint64_t>(2ull * mb + 1, 0xFFFFFF00ull));
@@ -641,10 +694,12 @@ int mpm_debug_stage(mpm_ctx* c, int method, int stage) {
     if (method != MPM_METHOD_EXPLICIT && method != MPM_METHOD_PBMPM) return fail(c, MPM_E_INVALID, "unknown method");
     if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
     HIPCHK(c, hipSetDevice(c->device));
+    if (stage == MPM_STAGE_WARM_START && (method != MPM_METHOD_PBMPM || !c->learn_on))
+        return fail(c, MPM_E_STATE, "the warm-start stage needs PBMPM with the learned warm start enabled");
     SimParams P;
     int rc = make_params(c, P);
     if (rc) return rc;
-    bool done = enqueue_substep(c, method, P, stage < 0 ? 1 << 30 : stage);
+    bool done = enqueue_substep(c, method, P, stage == MPM_STAGE_WARM_START ? stage : stage < 0 ? 1 << 30 : stage);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (done) { c->cur ^= 1; c->substeps++; }
@@ -742,6 +797,7 @@ int mpm_get_stats(mpm_ctx* c, mpm_stats* out) {
     out->blocks_dropped = h[C_DROPS];
     out->probe_failures = h[C_PROBEFAIL];
     out->substeps = c->substeps;
+    out->mlp_fallbacks = h[C_FALLBACK];
     return MPM_OK;
 }
 
@@ -771,6 +827,225 @@ int mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n)
     }
     if (e == hipSuccess) e = hipMemcpy(out, dout, n * out_w * 4, hipMemcpyDeviceToHost);
     hipFree(da); if (db) hipFree(db); hipFree(dout);
+    return e == hipSuccess ? MPM_OK : MPM_E_HIP;
+}
+
+
+// ------------------------------------------------------------------ learned warm start / residuals
+int mpm_set_iterations(mpm_ctx* c, int k, float force_divisor) {
+    if (!c) return MPM_E_INVALID;
+    if (k < 1 || k > 64 || !(force_divisor > 0.0f)) return fail(c, MPM_E_RANGE, "iterations must be in 1..64, divisor > 0");
+    c->k_cur = k;
+    c->force_div_cur = force_divisor;
+    return MPM_OK;
+}
+
+static int ensure_mlp(mpm_ctx* c) {
+    if (c->mlp_params) return MPM_OK;
+    int rc;
+    if ((rc = dalloc(c, &c->mlp_params, MLP_NPARAM)) || (rc = dalloc(c, &c->mlp_grads, MLP_NPARAM)) ||
+        (rc = dalloc(c, &c->mlp_m1, MLP_NPARAM)) || (rc = dalloc(c, &c->mlp_m2, MLP_NPARAM)) ||
+        (rc = dalloc(c, &c->mlp_norm, 64)) || (rc = dalloc(c, &c->mlp_wbf, 16 * 64 + 64 * 64 + 64 * 16)) ||
+        (rc = dalloc(c, &c->mlp_colsum, 64)) || (rc = dalloc(c, &c->mlp_loss, 1)))
+        return rc;
+    HIPCHK(c, hipMemset(c->mlp_grads, 0, MLP_NPARAM * 4));
+    HIPCHK(c, hipMemset(c->mlp_m1, 0, MLP_NPARAM * 4));
+    HIPCHK(c, hipMemset(c->mlp_m2, 0, MLP_NPARAM * 4));
+    return MPM_OK;
+}
+
+// Standard-normal initialisation scaled by 1/sqrt(fan_in) from a seeded host generator.
+static void init_params(std::vector<float>& w, uint32_t seed) {
+    w.assign(MLP_NPARAM, 0.0f);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ seed;
+    auto uni = [&]() { st = st * 6364136223846793005ull + 1442695040888963407ull; return ((st >> 11) + 0.5) * (1.0 / 9007199254740992.0); };
+    auto nrm = [&]() { return std::sqrt(-2.0 * std::log(uni())) * std::cos(6.283185307179586 * uni()); };
+    for (int f = 0; f < MLP_NIN; ++f) for (int h = 0; h < 64; ++h) w[MLP_W1 + f * 64 + h] = (float)(nrm() / std::sqrt((double)MLP_NIN));
+    for (int a = 0; a < 64; ++a) for (int b = 0; b < 64; ++b) w[MLP_W2 + a * 64 + b] = (float)(nrm() / 8.0);
+    for (int a = 0; a < 64; ++a) for (int o = 0; o < MLP_NOUT; ++o) w[MLP_W3 + a * 16 + o] = (float)(nrm() / 8.0 * 0.1);
+}
+
+int mpm_learn_collect(mpm_ctx* c, int n_substeps, int k_ref, int every, uint64_t max_pairs, uint64_t* n_pairs) {
+    if (!c || n_substeps < 0 || k_ref < 1 || k_ref > 64 || every < 1) return MPM_E_INVALID;
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (max_pairs > c->ds_cap) {
+        if (c->ds_n) return fail(c, MPM_E_STATE, "dataset capacity is fixed by the first collection");
+        int rc;
+        if ((rc = dalloc(c, &c->ds_feats, max_pairs * 16)) || (rc = dalloc(c, &c->ds_targs, max_pairs * 16))) return rc;
+        c->ds_cap = max_pairs;
+    }
+    const int k_save = c->k_cur; const float fd_save = c->force_div_cur; const bool learn_save = c->learn_on;
+    c->k_cur = k_ref; c->force_div_cur = (float)k_ref; c->learn_on = false;
+    SimParams P;
+    int rc = make_params(c, P);
+    if (rc == MPM_OK) {
+        for (int s = 0; s < n_substeps; ++s) {
+            const bool log = (s % every == 0) && c->ds_n + c->n <= c->ds_cap;
+            if (log) { Bufs B = make_bufs(c); k_mlp_log_features<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(B, c->n, c->ds_feats, c->ds_n); }
+            enqueue_substep(c, MPM_METHOD_PBMPM, P, -1, false);
+            c->cur ^= 1; c->substeps++;
+            if (log) {
+                Bufs B = make_bufs(c);
+                k_mlp_log_targets<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(B, c->n, c->ds_feats, c->ds_targs, c->ds_n);
+                c->ds_n += c->n;
+            }
+        }
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail(c, MPM_E_HIP, std::string("collect: ") + hipGetErrorString(e));
+    }
+    c->k_cur = k_save; c->force_div_cur = fd_save; c->learn_on = learn_save;
+    if (n_pairs) *n_pairs = c->ds_n;
+    return rc;
+}
+
+int mpm_learn_train(mpm_ctx* c, int steps, int batch, float lr, uint32_t seed, float* mean_loss) {
+    if (!c || steps < 0 || batch < 256 || batch % 256 || !(lr > 0.0f)) return MPM_E_INVALID;
+    if (c->ds_n < 16) return fail(c, MPM_E_STATE, "no training pairs collected");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = ensure_mlp(c))) return rc;
+    if (!c->mlp_ready) {
+        // standardisation from the collected pairs, then initial weights
+        HIPCHK(c, hipMemsetAsync(c->mlp_colsum, 0, 64 * 8, c->stream));
+        k_mlp_colstats<<<1024, 256, 0, c->stream>>>(c->ds_feats, c->ds_targs, c->ds_n, c->mlp_colsum);
+        double cs[64];
+        HIPCHK(c, hipMemcpyAsync(cs, c->mlp_colsum, sizeof(cs), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        float norm[64];
+        for (int k = 0; k < 16; ++k) {
+            const double n = (double)c->ds_n;
+            const double mx = cs[k] / n, vx = std::max(cs[16 + k] / n - mx * mx, 0.0);
+            const double mt = cs[32 + k] / n, vt = std::max(cs[48 + k] / n - mt * mt, 0.0);
+            const bool fx = k < MLP_NIN, ft = k < MLP_NOUT;
+            norm[k] = fx ? (float)mx : 0.0f;
+            norm[16 + k] = fx ? (float)std::max(std::sqrt(vx), 1e-12) : 1.0f;
+            norm[32 + k] = ft ? (float)mt : 0.0f;
+            norm[48 + k] = ft ? (float)std::max(std::sqrt(vt), 1e-20) : 1.0f;
+        }
+        HIPCHK(c, hipMemcpy(c->mlp_norm, norm, sizeof(norm), hipMemcpyHostToDevice));
+        std::vector<float> w;
+        init_params(w, seed);
+        HIPCHK(c, hipMemcpy(c->mlp_params, w.data(), MLP_NPARAM * 4, hipMemcpyHostToDevice));
+        k_mlp_refresh_bf16<<<(MLP_NPARAM + 255) / 256, 256, 0, c->stream>>>(c->mlp_params, c->mlp_wbf);
+        c->adam_t = 0;
+        c->mlp_ready = true;
+    }
+    const uint32_t tiles = 4, nwg = (uint32_t)batch / (4u * 16u * tiles);
+    MlpTrainArgs A;
+    A.feats = c->ds_feats; A.targs = c->ds_targs; A.n = c->ds_n; A.tiles = tiles; A.inv_batch = 1.0f / (float)batch;
+    A.loss_sum = c->mlp_loss;
+    HIPCHK(c, hipMemsetAsync(c->mlp_loss, 0, 4, c->stream));
+    const int tail = std::min(steps, 50);
+    for (int it = 0; it < steps; ++it) {
+        if (it == steps - tail) HIPCHK(c, hipMemsetAsync(c->mlp_loss, 0, 4, c->stream));
+        A.seed = hash_host(seed * 2654435761u + (uint32_t)c->adam_t);
+        k_mlp_train<<<nwg, 256, 0, c->stream>>>(mlp_bufs(c), A);
+        c->adam_t++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.0f - (float)std::pow((double)b1, (double)c->adam_t), c2 = 1.0f - (float)std::pow((double)b2, (double)c->adam_t);
+        k_mlp_adam<<<(MLP_NPARAM + 255) / 256, 256, 0, c->stream>>>(c->mlp_params, c->mlp_grads, c->mlp_m1, c->mlp_m2, c->mlp_wbf,
+                                                                    lr, b1, b2, 1e-8f, c1, c2);
+    }
+    float loss = 0.0f;
+    HIPCHK(c, hipMemcpyAsync(&loss, c->mlp_loss, 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (mean_loss) *mean_loss = tail ? loss / ((float)tail * (float)batch * (float)MLP_NOUT) : 0.0f;
+    return MPM_OK;
+}
+
+int mpm_learn_get_params(mpm_ctx* c, float* params, float* norm) {
+    if (!c || !params || !norm) return MPM_E_INVALID;
+    if (!c->mlp_ready) return fail(c, MPM_E_STATE, "no trained network");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(params, c->mlp_params, MLP_NPARAM * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(norm, c->mlp_norm, 64 * 4, hipMemcpyDeviceToHost));
+    return MPM_OK;
+}
+int mpm_learn_set_params(mpm_ctx* c, const float* params, const float* norm) {
+    if (!c || !params || !norm) return MPM_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    if ((rc = ensure_mlp(c))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->mlp_params, params, MLP_NPARAM * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->mlp_norm, norm, 64 * 4, hipMemcpyHostToDevice));
+    k_mlp_refresh_bf16<<<(MLP_NPARAM + 255) / 256, 256, 0, c->stream>>>(c->mlp_params, c->mlp_wbf);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->mlp_ready = true;
+    c->adam_t = 0;
+    HIPCHK(c, hipMemset(c->mlp_m1, 0, MLP_NPARAM * 4));
+    HIPCHK(c, hipMemset(c->mlp_m2, 0, MLP_NPARAM * 4));
+    return MPM_OK;
+}
+int mpm_learn_enable(mpm_ctx* c, int on, float limit) {
+    if (!c || !(limit > 0.0f)) return MPM_E_INVALID;
+    if (on && !c->mlp_ready) return fail(c, MPM_E_STATE, "no trained network");
+    c->learn_on = on != 0;
+    c->learn_limit = limit;
+    return MPM_OK;
+}
+
+int mpm_set_residual_tracking(mpm_ctx* c, int on) {
+    if (!c) return MPM_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    if (on && !c->resid_dx) {
+        int rc;
+        if ((rc = dalloc(c, &c->resid_dx, 3 * (size_t)c->stride)) || (rc = dalloc(c, &c->resid_max, 64)) ||
+            (rc = dalloc(c, &c->resid_peak, 64)) || (rc = dalloc(c, &c->resid_sum, 64)))
+            return rc;
+    }
+    if (c->resid_dx) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipMemset(c->resid_max, 0, 64 * 4));
+        HIPCHK(c, hipMemset(c->resid_peak, 0, 64 * 4));
+        HIPCHK(c, hipMemset(c->resid_sum, 0, 64 * 4));
+    }
+    c->resid_on = on != 0;
+    c->resid_substeps = c->substeps;
+    return MPM_OK;
+}
+int mpm_get_residuals(mpm_ctx* c, float* mean, float* peak, int cap, uint64_t* substeps) {
+    if (!c || !mean || !peak || cap < 1) return MPM_E_INVALID;
+    if (!c->resid_dx) return fail(c, MPM_E_STATE, "residual tracking was never enabled");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float sum[64]; uint32_t pk[64];
+    HIPCHK(c, hipMemcpy(sum, c->resid_sum, sizeof(sum), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(pk, c->resid_peak, sizeof(pk), hipMemcpyDeviceToHost));
+    const uint64_t ns = c->substeps - c->resid_substeps;
+    for (int i = 0; i < cap && i < 64; ++i) { mean[i] = ns ? sum[i] / (float)ns : 0.0f; std::memcpy(&peak[i], &pk[i], 4); }
+    if (substeps) *substeps = ns;
+    return MPM_OK;
+}
+
+int mpm_mlp_grad(const float* params, const float* norm, const float* feats, const float* targs, uint32_t n,
+                 float* grads, float* y, float* loss) {
+    if (!params || !norm || !feats || !targs || !grads || !y || !loss || n == 0 || n % 16 || n > (1u << 20)) return MPM_E_INVALID;
+    float *dp = nullptr, *dg = nullptr, *dn = nullptr, *df = nullptr, *dt = nullptr, *dy = nullptr, *dl = nullptr;
+    short* dw = nullptr;
+    hipError_t e = hipSuccess;
+    auto al = [&](void** p, size_t b) { if (e == hipSuccess) e = hipMalloc(p, b); };
+    al((void**)&dp, MLP_NPARAM * 4); al((void**)&dg, MLP_NPARAM * 4); al((void**)&dn, 64 * 4);
+    al((void**)&df, (size_t)n * 64); al((void**)&dt, (size_t)n * 64); al((void**)&dy, (size_t)n * 64); al((void**)&dl, 4);
+    al((void**)&dw, (16 * 64 + 64 * 64 + 64 * 16) * 2);
+    if (e == hipSuccess) e = hipMemcpy(dp, params, MLP_NPARAM * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, norm, 64 * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(df, feats, (size_t)n * 64, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dt, targs, (size_t)n * 64, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(dg, 0, MLP_NPARAM * 4);
+    if (e == hipSuccess) e = hipMemset(dl, 0, 4);
+    if (e == hipSuccess) {
+        k_mlp_refresh_bf16<<<(MLP_NPARAM + 255) / 256, 256>>>(dp, dw);
+        MlpBufs M; M.params = dp; M.wbf = dw; M.grads = dg; M.norm = dn;
+        k_mlp_grad_fixed<<<(n / 16 + 3) / 4, 256>>>(M, df, dt, n, dy, dl);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(grads, dg, MLP_NPARAM * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(y, dy, (size_t)n * 64, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(loss, dl, 4, hipMemcpyDeviceToHost);
+    hipFree(dp); hipFree(dg); hipFree(dn); hipFree(df); hipFree(dt); hipFree(dy); hipFree(dl); hipFree(dw);
     return e == hipSuccess ? MPM_OK : MPM_E_HIP;
 }
 

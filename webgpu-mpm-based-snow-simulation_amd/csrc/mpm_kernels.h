@@ -35,7 +35,7 @@ enum Plane : int {
 };
 
 enum Counter : int {
-    C_NVALID = 0, C_NRUNS, C_NCHUNKS, C_NBLOCKS, C_STRAG, C_DROPS, C_PROBEFAIL, C_SUBSTEPS, C_NCOLL, C_NCOUNTERS = 16
+    C_NVALID = 0, C_NRUNS, C_NCHUNKS, C_NBLOCKS, C_STRAG, C_DROPS, C_PROBEFAIL, C_SUBSTEPS, C_NCOLL, C_FALLBACK, C_NCOUNTERS = 16
 };
 
 struct SimParams {
@@ -56,6 +56,10 @@ struct SimParams {
     uint32_t fast_dt;          // fixed_div(., dt, rdt, S) verified equal to i32((x/dt) S) for all x
     uint32_t grav_nz[3];       // grav_div component is nonzero (zero ones contribute i32 0 exactly)
     uint32_t wall_mode, wall_cells;
+    int32_t k;                 // PBMPM iterations of this substep
+    uint32_t learn_flags;      // bit0 learned warm start, bit1 residual tracking
+    float inv_domain;          // 1 / largest grid extent (residual normalisation)
+    float learn_limit;         // warm start: largest accepted |standardised prediction|
     // collider (uniforms @224..@432)
     float cmin[3], cmax[3], T[16], Tinv[16], cvel[3];
     uint32_t coll_num_indices, coll_num_objects, n_bvh_nodes, has_collider;
@@ -77,6 +81,9 @@ struct Bufs {
     int4* grid0; int4* grid1; int4* grid2;
     uint32_t* counters;
     uint32_t* cqueue;          // sorted indices of particles whose path reaches the collider's box
+    float* resid_dx;           // optional: dx of the previous G2P per particle (3 planes, stride)
+    uint32_t* resid_max;       // optional: per G2P iteration, max over particles of |dx_i - dx_{i-1}|_inf (float bits)
+    float* resid_sum;          // optional: per G2P iteration, sum over substeps of that max
     const uint32_t* coll_data; const uint4* bvh;
 };
 
@@ -169,6 +176,17 @@ MPM_DEV uint32_t wave_or(uint32_t v) {
     for (int o = 1; o < 64; o <<= 1) v |= __shfl_xor(v, o, 64);
     return v;
 }
+MPM_DEV float wave_sum_f(float v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// Orders the LDS traffic of one wave: everything before is visible to every lane after.
+MPM_DEV void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 MPM_DEV uint32_t wave_incl_scan(uint32_t v) {
     const int l = lane_id();
 #pragma unroll
@@ -240,7 +258,7 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
     const uint32_t gsz = gridDim.x * blockDim.x;
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t s = gi; s < P.cap; s += gsz) B.table[s] = EMPTY;
-    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; }
+    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; }
     const uint32_t n = P.n;
     const uint32_t nloop = (n + gsz - 1) / gsz;
     for (uint32_t it = 0; it < nloop; ++it) {
@@ -897,8 +915,34 @@ enum { PB_FIRST = 0, PB_MID = 1, PB_LAST = 2 };
 //   MID:   G2P of iteration i-1 from grid[gin], solve + P2G of iteration i into grid[gout].
 //   LAST:  G2P of iteration k-1 from grid[gin], integrateParticles, write the particle state.
 // Every pass also clears grid[gclr] (the buffer the next pass writes) over the live blocks.
+// Residual bookkeeping of the G2P of iteration `it` (optional): |dx_it - dx_{it-1}|_inf / domain.
+MPM_DEV float track_residual(const SimParams& P, const Bufs& B, uint32_t q, v3 dx, int it) {
+    const uint32_t sd = B.stride;
+    float r = 0.0f;
+    if (it > 0) {
+        const v3 pr = ld3(B.resid_dx, sd, 0, q);
+        r = fmaxf(fmaxf(fabs_(dx.x - pr.x), fabs_(dx.y - pr.y)), fabs_(dx.z - pr.z)) * P.inv_domain;
+    }
+    st3(B.resid_dx, sd, 0, q, dx);
+    return r;
+}
+MPM_DEV void publish_residual(const Bufs& B, float r, int it) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
+    if (lane_id() == 0 && r > 0.0f) atomicMax(&B.resid_max[it], as_u(r));
+}
+// Per-substep roll-up of the residuals: sum and peak over substeps, then reset.
+__global__ void k_resid_roll(Bufs B, int k, uint32_t* peak) {
+    const int it = threadIdx.x;
+    if (it >= k) return;
+    const uint32_t v = B.resid_max[it];
+    B.resid_sum[it] += as_f(v);
+    peak[it] = peak[it] > v ? peak[it] : v;
+    B.resid_max[it] = 0u;
+}
+
 template <int MODE>
-__global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr) {
+__global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
     __shared__ Tile T;
     const uint32_t nchunks = B.counters[C_NCHUNKS];
     if (gclr >= 0) {
@@ -920,6 +964,7 @@ __global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, 
             const uint32_t i = r * blockDim.x + threadIdx.x;
             const uint32_t q = T.start + i;
             bool cand = false;
+            float resid = 0.0f;
             if (i >= T.count) {
             } else if (MODE == PB_FIRST) {
                 const uint32_t old = B.perm[q];
@@ -953,6 +998,7 @@ __global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, 
                 v3 nv; m3 Bm;
                 const uint32_t dep = g2p_pbmpm(T, s, off, nv, Bm);
                 const v3 dx = nv * P.dt;
+                if (B.resid_dx) resid = track_residual(P, B, q, dx, it);
                 m3 D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                 solve_constraints(P, Fe, relax, D);
                 p2g_pbmpm(P, T, s, off, dep, mass, dx, D);
@@ -969,6 +1015,7 @@ __global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, 
                     v3 nv; m3 Bm;
                     g2p_pbmpm(T, s, off, nv, Bm);
                     p.dx = nv * P.dt;
+                    if (B.resid_dx) resid = track_residual(P, B, q, p.dx, it);
                     p.D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                     p.vel = nv;
                 } else {
@@ -980,6 +1027,7 @@ __global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, 
                 st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
             }
             if (MODE == PB_LAST) queue_collider(B, cand, q);
+            if (MODE != PB_FIRST && B.resid_dx && it > 0) publish_residual(B, resid, it);
         }
         __syncthreads();
         if (MODE != PB_LAST) tile_flush(T, go);

@@ -282,3 +282,68 @@ class MpmSimulation:
 
     def uniform_bytes(self):
         return self.uniformsManager.read(0, UNIFORMS_BYTES)
+
+    # ---- PBMPM iteration count and the learned warm start (no reference counterpart) ----
+    def set_iterations(self, k, force_divisor=None):
+        """nSolveConstraintIterations (GpuMpmPipelineManager.ts:445) and the per-iteration force
+        divisor (pbmpm.wgsl:137), which the reference keeps equal to it."""
+        fd = float(k if force_divisor is None else force_divisor)
+        check(self.lib, self.ctx, self.lib.mpm_set_iterations(self.ctx, int(k), fd))
+
+    def learn_collect(self, n_substeps, k_ref, every=1, max_pairs=None):
+        """Advances n_substeps PBMPM substeps with k_ref iterations, logging (features, target)."""
+        n = ctypes.c_uint64()
+        mp = self.nParticles * max(1, (n_substeps + every - 1) // every) if max_pairs is None else max_pairs
+        check(self.lib, self.ctx, self.lib.mpm_learn_collect(self.ctx, int(n_substeps), int(k_ref), int(every), int(mp), ctypes.byref(n)))
+        return n.value
+
+    def learn_train(self, steps, batch=4096, lr=1e-3, seed=1):
+        loss = ctypes.c_float()
+        check(self.lib, self.ctx, self.lib.mpm_learn_train(self.ctx, int(steps), int(batch), float(lr), int(seed), ctypes.byref(loss)))
+        return loss.value
+
+    def learn_params(self):
+        p = np.zeros(_lib.MLP_NPARAM, np.float32)
+        nm = np.zeros(64, np.float32)
+        check(self.lib, self.ctx, self.lib.mpm_learn_get_params(self.ctx, _ptr(p), _ptr(nm)))
+        return p, nm
+
+    def set_learn_params(self, params, norm):
+        p = np.ascontiguousarray(params, np.float32)
+        nm = np.ascontiguousarray(norm, np.float32)
+        if p.size != _lib.MLP_NPARAM or nm.size != 64:
+            raise ValueError("expected 6288 parameters and 64 standardisation values")
+        check(self.lib, self.ctx, self.lib.mpm_learn_set_params(self.ctx, _ptr(p), _ptr(nm)))
+
+    def learn_enable(self, on=True, limit=8.0):
+        check(self.lib, self.ctx, self.lib.mpm_learn_enable(self.ctx, 1 if on else 0, float(limit)))
+
+    def set_residual_tracking(self, on=True):
+        check(self.lib, self.ctx, self.lib.mpm_set_residual_tracking(self.ctx, 1 if on else 0))
+
+    def residuals(self):
+        """(mean, peak) per PBMPM iteration of max_particles |dx_i - dx_{i-1}|_inf / domain extent,
+        over the substeps since tracking was enabled (entry 0 is unused), and that substep count."""
+        mean = np.zeros(64, np.float32)
+        peak = np.zeros(64, np.float32)
+        ns = ctypes.c_uint64()
+        check(self.lib, self.ctx, self.lib.mpm_get_residuals(self.ctx, _ptr(mean), _ptr(peak), 64, ctypes.byref(ns)))
+        return mean, peak, ns.value
+
+
+def mlp_grad(params, norm, feats, targs):
+    """MFMA forward + backward of the warm-start network over the given samples (count a
+    multiple of 16): (gradient [6288], standardised outputs [n][16], sum of squared errors)."""
+    lib = _lib.load_library()
+    p = np.ascontiguousarray(params, np.float32)
+    nm = np.ascontiguousarray(norm, np.float32)
+    f = np.ascontiguousarray(feats, np.float32).reshape(-1, 16)
+    t = np.ascontiguousarray(targs, np.float32).reshape(-1, 16)
+    n = f.shape[0]
+    g = np.zeros(_lib.MLP_NPARAM, np.float32)
+    y = np.zeros((n, 16), np.float32)
+    loss = np.zeros(1, np.float32)
+    rc = lib.mpm_mlp_grad(_ptr(p), _ptr(nm), _ptr(f), _ptr(t), n, _ptr(g), _ptr(y), _ptr(loss))
+    if rc != 0:
+        raise RuntimeError(f"mpm_mlp_grad failed ({rc})")
+    return g, y, float(loss[0])
