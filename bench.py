@@ -77,6 +77,76 @@ def cpu_baseline(mpm, sc, sim, substeps=2):
                       f"oracle FP32 restatement on 1 thread of {cpu_model()} ({os.cpu_count()} logical CPUs)"}
 
 
+def learned_vs_plain(mpm, n, k_ref=16, substeps=600):
+    """The learned warm start against plain PBMPM on the bench scene.
+
+    Training pairs come from another sampling of the scene (seed 7): every 10th of `substeps`
+    substeps run with k_ref iterations, every particle logged. The network is trained on the GPU,
+    then both methods run the bench scene (seed 1) for `substeps` substeps with k_ref iterations
+    and per-iteration residual tracking, r_i = mean over substeps of
+    max_particles |dx_i - dx_{i-1} - g dt^2 / k|_2 / domain (gravity's per-iteration share
+    removed). Reported: the iterations each method needs to
+    reach the residual plain PBMPM has after k_ref / 2 iterations, and 1e-6; the final position
+    error after `substeps` substeps against the k_ref-iteration run, for plain and learned at a
+    few iteration counts (the scene breaks up chaotically, so this drifts for any k != k_ref)."""
+    def make(seed):
+        sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=n, seed=seed)
+        sim = mpm.MpmSimulation(n, **sc["sim"])
+        sim.set_collider(*sc["collider"])
+        sim.upload(sc["particles"])
+        return sim
+
+    t0 = time.perf_counter()
+    tr = make(7)
+    pairs = tr.learn_collect(substeps, k_ref=k_ref, every=10)
+    loss_a = tr.learn_train(4000, batch=65536, lr=1e-3, seed=1)
+    loss_b = tr.learn_train(2000, batch=65536, lr=2e-4, seed=2)
+    net = tr.learn_params()
+    tr.close()
+    t_train = time.perf_counter() - t0
+
+    def run(k, learned, track=False):
+        sim = make(1)
+        sim.set_iterations(k)
+        if learned:
+            sim.set_learn_params(*net)
+            sim.learn_enable(True)
+        if track:
+            sim.set_residual_tracking(True)
+        sim.step(1, substeps)
+        pos = sim.download()["pos"].astype(np.float64)
+        res = sim.residuals()[0][:k].astype(np.float64) if track else None
+        st = sim.stats()
+        sim.close()
+        return pos, res, st
+
+    ref, r_plain, _ = run(k_ref, False, True)
+    _, r_learn, st = run(k_ref, True, True)
+
+    def iters_to(r, thr):
+        hit = [i for i in range(1, len(r)) if r[i] <= thr]
+        return hit[0] if hit else None
+
+    thr = float(r_plain[k_ref // 2])
+    it_p, it_l = iters_to(r_plain, thr), iters_to(r_learn, thr)
+    drift = {}
+    for k in (2, 4, 8):
+        for learned in (False, True):
+            pos, _, _ = run(k, learned)
+            e = np.linalg.norm(pos - ref, axis=1)
+            drift[f"{'learned' if learned else 'plain'}_k{k}"] = {"rms": float(np.sqrt((e * e).mean())), "max": float(e.max())}
+    return {
+        "k_ref": k_ref, "substeps": substeps, "training_pairs": pairs,
+        "train_loss": [loss_a, loss_b], "train_and_collect_s": round(t_train, 2),
+        "residual_plain": [float(x) for x in r_plain[1:]], "residual_learned": [float(x) for x in r_learn[1:]],
+        "threshold": thr, "iters_plain": it_p, "iters_learned": it_l,
+        "iteration_ratio": (it_p / it_l) if (it_p and it_l) else None,
+        "iters_to_1e-6": {"plain": iters_to(r_plain, 1e-6), "learned": iters_to(r_learn, 1e-6)},
+        "fallbacks_last_substep": st["mlp_fallbacks"],
+        "position_error_after_600_vs_k_ref": drift,
+    }
+
+
 def copy_calibration(mpm, nbytes):
     """One plain device copy kernel of nbytes (k_selftest kind 10): what FETCH_SIZE / WRITE_SIZE
     report for it under rocprofv3 --pmc calibrates the counters against a copy of known size."""
@@ -108,6 +178,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--kernel-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
@@ -179,6 +250,9 @@ def main():
     }
     if args.calibrate_copy_mb:
         copy_calibration(mpm, args.calibrate_copy_mb << 20)
+    if not args.no_learned:
+        print("bench: learned warm start vs plain PBMPM", file=sys.stderr, flush=True)
+        line["learned_vs_plain"] = learned_vs_plain(mpm, n)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline (oracle, 2 substeps)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(mpm, sc, sim)

@@ -183,8 +183,10 @@ int  mpm_learn_enable(mpm_ctx* ctx, int on, float limit);
 int  mpm_mlp_grad(const float* params, const float* norm, const float* feats, const float* targs, uint32_t n,
                   float* grads, float* y, float* loss);
 
-/* Per-iteration PBMPM residual, r_i = max_particles |dx_i - dx_{i-1}|_inf / largest grid extent,
- * dx_i from the G2P of iteration i. mean / peak over the substeps since tracking was enabled. */
+/* Per-iteration PBMPM residual, r_i = max_particles |dx_i - dx_{i-1} - g dt^2 / force_divisor|_2
+ * / largest grid extent, dx_i from the G2P of iteration i. Every iteration deposits g dt /
+ * force_divisor of gravity (pbmpm.wgsl:137), so a free particle's dx grows by that times dt per
+ * iteration; the residual subtracts it and measures the constraint solve alone. mean / peak over the substeps since tracking was enabled. */
 int  mpm_set_residual_tracking(mpm_ctx* ctx, int on);
 int  mpm_get_residuals(mpm_ctx* ctx, float* mean, float* peak, int cap, uint64_t* substeps);
 

@@ -82,7 +82,7 @@ struct Bufs {
     uint32_t* counters;
     uint32_t* cqueue;          // sorted indices of particles whose path reaches the collider's box
     float* resid_dx;           // optional: dx of the previous G2P per particle (3 planes, stride)
-    uint32_t* resid_max;       // optional: per G2P iteration, max over particles of |dx_i - dx_{i-1}|_inf (float bits)
+    uint32_t* resid_max;       // optional: per G2P iteration, max over particles of |dx_i - dx_{i-1}|_2 (float bits)
     float* resid_sum;          // optional: per G2P iteration, sum over substeps of that max
     const uint32_t* coll_data; const uint4* bvh;
 };
@@ -915,13 +915,16 @@ enum { PB_FIRST = 0, PB_MID = 1, PB_LAST = 2 };
 //   MID:   G2P of iteration i-1 from grid[gin], solve + P2G of iteration i into grid[gout].
 //   LAST:  G2P of iteration k-1 from grid[gin], integrateParticles, write the particle state.
 // Every pass also clears grid[gclr] (the buffer the next pass writes) over the live blocks.
-// Residual bookkeeping of the G2P of iteration `it` (optional): |dx_it - dx_{it-1}|_inf / domain.
+// Residual bookkeeping of the G2P of iteration `it` (optional):
+// |dx_it - dx_{it-1} - (g / force_div) dt^2|_2 / domain.
 MPM_DEV float track_residual(const SimParams& P, const Bufs& B, uint32_t q, v3 dx, int it) {
     const uint32_t sd = B.stride;
     float r = 0.0f;
     if (it > 0) {
         const v3 pr = ld3(B.resid_dx, sd, 0, q);
-        r = fmaxf(fmaxf(fabs_(dx.x - pr.x), fabs_(dx.y - pr.y)), fabs_(dx.z - pr.z)) * P.inv_domain;
+        // each iteration adds grav_div dt to every node velocity: dx grows by grav_div dt^2
+        const v3 e = dx - pr - mk(P.grav_div[0], P.grav_div[1], P.grav_div[2]) * (P.dt * P.dt);
+        r = __builtin_sqrtf(e.x * e.x + e.y * e.y + e.z * e.z) * P.inv_domain;
     }
     st3(B.resid_dx, sd, 0, q, dx);
     return r;

@@ -322,7 +322,7 @@ class MpmSimulation:
         check(self.lib, self.ctx, self.lib.mpm_set_residual_tracking(self.ctx, 1 if on else 0))
 
     def residuals(self):
-        """(mean, peak) per PBMPM iteration of max_particles |dx_i - dx_{i-1}|_inf / domain extent,
+        """(mean, peak) per PBMPM iteration of max_particles |dx_i - dx_{i-1} - g dt^2 / k|_2 / domain extent,
         over the substeps since tracking was enabled (entry 0 is unused), and that substep count."""
         mean = np.zeros(64, np.float32)
         peak = np.zeros(64, np.float32)

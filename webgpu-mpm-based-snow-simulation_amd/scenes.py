@@ -84,7 +84,11 @@ def scene(name, n_particles=None, seed=None):
         vel[: n // 2, 0] = 2.0
         vel[n // 2:, 0] = -2.0
         p = make_particles(np.concatenate([a, b]), vel=vel)
-        sim = dict(grid_res=(128, 128, 128), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1.0 / 600,
+        dt = 1.0 / 600
+        # PBMPM carries motion in the displacement (P2G deposits m (dx + D off) / dt,
+        # pbmpm.wgsl:41-76), so the initial velocity enters as dx = v dt
+        p["dx"][:, :3] = vel * np.float32(dt)
+        sim = dict(grid_res=(128, 128, 128), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=dt,
                    gravity=(0.0, -9.8, 0.0), pbmpm_iterations=4, pbmpm_force_divisor=4.0)
         return dict(particles=p, sim=sim, collider=box_mesh((0.5, 0.5, 0.5), 0.2), method=1, substeps=600)
     if name == "explicit_slab_terrain":
