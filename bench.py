@@ -147,6 +147,33 @@ def learned_vs_plain(mpm, n, k_ref=16, substeps=600):
     }
 
 
+def ensemble_bench(mpm, steps=30, warmup=5):
+    """The 64-scene batched workload (scenes.ensemble_scenes): autotuned, then timed. Throughput
+    counts every scene's particles once per substep."""
+    scenes = mpm.scenes.ensemble_scenes()
+    n = sum(sc["sim"]["n_particles"] for sc in scenes)
+    with mpm.MpmEnsemble([sc["sim"] for sc in scenes]) as ens:
+        for i, sc in enumerate(scenes):
+            ens.upload(i, sc["particles"])
+        best, cands = ens.autotune(trial_substeps=3)
+        ens.step(warmup)
+        ens.sync()
+        t0 = time.perf_counter()
+        ens.step(steps)
+        ens.sync()
+        dt = time.perf_counter() - t0
+        ens.set_kernel_timing(True)
+        ens.step(5)
+        ens.sync()
+        kt = ens.kernel_times()
+        ens.set_kernel_timing(False)
+    return {"workload": "ensemble_64_scenes", "scenes": len(scenes), "particles": n,
+            "explicit_scenes": sum(1 for sc in scenes if sc["sim"]["method"] == 0),
+            "value": n * steps / dt, "unit": "particle-substeps/s", "ms_per_step": 1e3 * dt / steps,
+            "tuning": best, "autotune_candidates": cands,
+            "kernel_ms_per_substep": {k: round(v[0] / 5, 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0])}}
+
+
 def copy_calibration(mpm, nbytes):
     """One plain device copy kernel of nbytes (k_selftest kind 10): what FETCH_SIZE / WRITE_SIZE
     report for it under rocprofv3 --pmc calibrates the counters against a copy of known size."""
@@ -179,6 +206,7 @@ def main():
     ap.add_argument("--kernel-steps", type=int, default=60)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 64-scene workload")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
@@ -250,6 +278,9 @@ def main():
     }
     if args.calibrate_copy_mb:
         copy_calibration(mpm, args.calibrate_copy_mb << 20)
+    if not args.no_ensemble:
+        print("bench: batched 64-scene workload", file=sys.stderr, flush=True)
+        line["ensemble"] = ensemble_bench(mpm)
     if not args.no_learned:
         print("bench: learned warm start vs plain PBMPM", file=sys.stderr, flush=True)
         line["learned_vs_plain"] = learned_vs_plain(mpm, n)

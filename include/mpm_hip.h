@@ -24,7 +24,7 @@ extern "C" {
 #define MPM_UNIFORMS_BYTES 37376   /* GpuUniformsBufferManager.ts:17 */
 #define MPM_PARTICLE_BYTES 192     /* shaderPrelude/mpmOps.wgsl:1-12, GpuMpmBufferManager.ts:24 */
 #define MPM_BVH_NODE_BYTES 32      /* mpm/colliderPrelude.wgsl:6-11 */
-#define MPM_KERNEL_COUNT 17        /* kernels of one substep, see mpm_kernel_name() */
+#define MPM_KERNEL_COUNT 24        /* kernels of one substep, see mpm_kernel_name() */
 
 enum {
     MPM_OK = 0,
@@ -189,6 +189,34 @@ int  mpm_mlp_grad(const float* params, const float* norm, const float* feats, co
  * iteration; the residual subtracts it and measures the constraint solve alone. mean / peak over the substeps since tracking was enabled. */
 int  mpm_set_residual_tracking(mpm_ctx* ctx, int on);
 int  mpm_get_residuals(mpm_ctx* ctx, float* mean, float* peak, int cap, uint64_t* substeps);
+
+/* Batched scenes (no reference counterpart; the reference simulates one scene). n_scenes
+ * independent scenes, each with its own configuration (particle count, grid, dt, material,
+ * PBMPM iterations) and method, advance together: one call steps every scene by one of its own
+ * substeps. Binning runs once over all particles, and every transfer phase of every scene runs
+ * in ONE persistent kernel launch per substep (a phase-ordered task queue; optionally one
+ * launch per phase), captured in a hipGraph. Scenes have no collider and no pointer
+ * interaction; walls as configured. Results per scene are identical to the same scene stepped
+ * alone in its own context. The returned context works with mpm_sync, mpm_destroy,
+ * mpm_last_error, mpm_get_stats (totals), timing and kernel timing; particles go through
+ * mpm_ensemble_upload / _download (scene-local order). Limits: n_scenes <= 1024 and
+ * n_scenes * (largest phase count: PBMPM k + 1, explicit 2) <= 4096. */
+typedef struct mpm_tuning {
+    uint32_t chunk_particles;      /* particles per work chunk (LDS tile), 256..65536 */
+    uint32_t workgroups_per_cu;    /* resident workgroups of the transfer kernel per CU, 1..8 */
+    int32_t  persistent;           /* 1: one launch per substep; 0: one launch per phase */
+    float    ms_per_substep;       /* autotuner: measured time (0 otherwise) */
+} mpm_tuning;
+int  mpm_ensemble_create(const mpm_config* scene_cfgs, const int* methods, int n_scenes, mpm_ctx** out);
+int  mpm_ensemble_upload(mpm_ctx* ctx, int scene, const void* aos192, size_t n);
+int  mpm_ensemble_download(mpm_ctx* ctx, int scene, void* aos192, size_t n);
+int  mpm_ensemble_step(mpm_ctx* ctx, int n_substeps);
+int  mpm_ensemble_set_tuning(mpm_ctx* ctx, const mpm_tuning* t);
+int  mpm_ensemble_get_tuning(mpm_ctx* ctx, mpm_tuning* t);
+/* Autotuner: times each candidate (chunk size x workgroups per CU x persistent / per phase) for
+ * trial_substeps substeps on the live scenes, restores the state after every trial, and keeps
+ * the fastest. Candidates and their times go to out[0..cap), their count to *n_out. */
+int  mpm_ensemble_autotune(mpm_ctx* ctx, int trial_substeps, mpm_tuning* out, int cap, int* n_out, mpm_tuning* best);
 
 int  mpm_get_stats(mpm_ctx* ctx, mpm_stats* out);
 int  mpm_set_timing(mpm_ctx* ctx, int enabled);

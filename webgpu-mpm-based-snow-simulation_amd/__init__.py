@@ -5,5 +5,5 @@ from ._lib import (load_library, default_config, MpmError, MpmConfig, EXPORTED, 
                    METHOD_EXPLICIT, METHOD_PBMPM, UNIFORMS_BYTES, PARTICLE_BYTES)
 from .particles import PARTICLE_DTYPE, make_particles, state_columns, matrix
 from .managers import (GpuUniformsBufferManager, GpuMpmBufferManager, GpuColliderBufferManager,
-                       GpuMpmPipelineManager, MpmSimulation, build_bvh, mlp_grad)
+                       GpuMpmPipelineManager, MpmSimulation, MpmEnsemble, build_bvh, mlp_grad)
 from . import scenes

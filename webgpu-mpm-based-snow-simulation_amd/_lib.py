@@ -13,7 +13,7 @@ LIB_PATH = PKG_DIR / "libmpm_hip.so"
 UNIFORMS_BYTES = 37376
 PARTICLE_BYTES = 192
 BVH_NODE_BYTES = 32
-KERNEL_COUNT = 17
+KERNEL_COUNT = 24
 MLP_NPARAM = 6288
 
 MPM_OK = 0
@@ -80,6 +80,15 @@ class MpmTimings(ctypes.Structure):
     ]
 
 
+class MpmTuning(ctypes.Structure):
+    _fields_ = [
+        ("chunk_particles", ctypes.c_uint32),
+        ("workgroups_per_cu", ctypes.c_uint32),
+        ("persistent", ctypes.c_int32),
+        ("ms_per_substep", ctypes.c_float),
+    ]
+
+
 # (name, restype, argtypes) of every entry point declared in include/mpm_hip.h
 _P = ctypes.c_void_p
 _SZ = ctypes.c_size_t
@@ -110,6 +119,13 @@ _SIGS = [
     ("mpm_mlp_grad", ctypes.c_int, [_P, _P, _P, _P, ctypes.c_uint32, _P, _P, _P]),
     ("mpm_set_residual_tracking", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_get_residuals", ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    ("mpm_ensemble_create", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(_P)]),
+    ("mpm_ensemble_upload", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t]),
+    ("mpm_ensemble_download", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t]),
+    ("mpm_ensemble_step", ctypes.c_int, [_P, ctypes.c_int]),
+    ("mpm_ensemble_set_tuning", ctypes.c_int, [_P, ctypes.POINTER(MpmTuning)]),
+    ("mpm_ensemble_get_tuning", ctypes.c_int, [_P, ctypes.POINTER(MpmTuning)]),
+    ("mpm_ensemble_autotune", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(MpmTuning)]),
     ("mpm_get_stats", ctypes.c_int, [_P, ctypes.POINTER(MpmStats)]),
     ("mpm_set_timing", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_get_timings", ctypes.c_int, [_P, ctypes.POINTER(MpmTimings)]),

@@ -17,6 +17,7 @@
 #include <map>
 #include "mpm_kernels.h"
 #include "mpm_mlp.h"
+#include "mpm_ensemble.h"
 #include "../../include/mpm_hip.h"
 
 using namespace mpm;
@@ -83,6 +84,22 @@ struct mpm_ctx {
     size_t kt_n = 0;
     double kt_ms[MPM_KERNEL_COUNT] = {};
     uint64_t kt_cnt[MPM_KERNEL_COUNT] = {};
+    // batched scenes (ens > 0: this context advances ens independent scenes together)
+    uint32_t ens = 0, ens_R = 0;
+    std::vector<mpm_config> ens_cfg;
+    std::vector<int> ens_method;
+    std::vector<uint32_t> ens_part_base, ens_key_base, ens_blk_base, ens_nphase;
+    std::vector<SimParams> ens_sp;
+    std::vector<bool> ens_loaded;
+    SimParams ens_P0;
+    SimParams* d_sp = nullptr;
+    uint32_t *d_part_base = nullptr, *d_key_base = nullptr, *d_nphase = nullptr, *d_method = nullptr, *d_nblk = nullptr;
+    uint32_t *d_cbeg = nullptr, *d_seg = nullptr, *d_done = nullptr, *d_ctl = nullptr;
+    int ens_persistent = 1;
+    uint32_t ens_wg_per_cu = 2;
+    bool ens_graph_ok[2] = {false, false};
+    float* snap_planes = nullptr;
+    uint32_t* snap_pid = nullptr;
 };
 
 namespace {
@@ -91,9 +108,11 @@ const char* k_version = "libmpm_hip 0.1 (gfx950)";
 const char* k_kernel_names[MPM_KERNEL_COUNT] = {
     "k_keys_count", "k_scan_tiles", "k_scan_top", "k_scan_apply", "k_scatter", "k_insert", "k_compact", "k_nbr",
     "k_pbmpm<FIRST>", "k_pbmpm<MID>", "k_pbmpm<LAST>", "k_explicit_p2g", "k_grid_update", "k_explicit_g2p", "k_straggle",
-    "k_collide", "k_mlp_warmstart"};
+    "k_collide", "k_mlp_warmstart", "k_ens_keys_count", "k_ens_insert", "k_ens_compact", "k_ens_nbr", "k_ens_tasks",
+    "k_ens_transfer", "k_ens_straggle"};
 enum { KID_KEYS = 0, KID_SCAN_TILES, KID_SCAN_TOP, KID_SCAN_APPLY, KID_SCATTER, KID_INSERT, KID_COMPACT, KID_NBR,
-       KID_PB_FIRST, KID_PB_MID, KID_PB_LAST, KID_EX_P2G, KID_EX_GRID, KID_EX_G2P, KID_STRAGGLE, KID_COLLIDE, KID_MLP };
+       KID_PB_FIRST, KID_PB_MID, KID_PB_LAST, KID_EX_P2G, KID_EX_GRID, KID_EX_G2P, KID_STRAGGLE, KID_COLLIDE, KID_MLP,
+       KID_ENS_KEYS, KID_ENS_INSERT, KID_ENS_COMPACT, KID_ENS_NBR, KID_ENS_TASKS, KID_ENS_TRANSFER, KID_ENS_STRAGGLE };
 
 // Records an event after a kernel launch when per-kernel timing is on (the first event of a
 // step is the start mark, id -1).
@@ -187,11 +206,9 @@ int verify_divisor(mpm_ctx* c, float d, float S, uint32_t* ok) {
     return MPM_OK;
 }
 
-// Parse the uniform blob (GpuUniformsBufferManager.ts:54-182 offsets) + creation config.
-int make_params(mpm_ctx* c, SimParams& P) {
+// Parse a uniform blob (GpuUniformsBufferManager.ts:54-182 offsets) + creation config.
+int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, float force_div, bool scene_only, SimParams& P) {
     std::memset(&P, 0, sizeof(P));
-    const uint8_t* u = c->uni;
-    const mpm_config& g = c->cfg;
     P.dt = rdu<float>(u, 0);
     P.S = rdu<float>(u, 4);
     for (int a = 0; a < 3; ++a) {
@@ -208,7 +225,7 @@ int make_params(mpm_ctx* c, SimParams& P) {
         P.nb[a] = (P.res[a] + 3) / 4;
     }
     uint64_t K = (uint64_t)P.nb[0] * P.nb[1] * P.nb[2];
-    if (K > c->K) return fail(c, MPM_E_RANGE, "uniform grid resolution exceeds the resolution the context was created for");
+    if (!scene_only && K > c->K) return fail(c, MPM_E_RANGE, "uniform grid resolution exceeds the resolution the context was created for");
     P.K = (uint32_t)K;
     for (int i = 0; i < 16; ++i) { P.T[i] = rdu<float>(u, 256 + 4 * i); P.Tinv[i] = rdu<float>(u, 432 + 4 * i); }
     P.coll_num_indices = rdu<uint32_t>(u, 348);
@@ -233,9 +250,9 @@ int make_params(mpm_ctx* c, SimParams& P) {
     P.plast_hi = (float)(1.0 + g.theta_s);
     P.inv_density = (float)(1.0 / g.density);
     P.grad_scale = g.grad_scale;
-    P.force_div = c->force_div_cur;
-    P.k = c->k_cur;
-    P.learn_flags = (c->learn_on && c->mlp_ready ? 1u : 0u) | (c->resid_on ? 2u : 0u);
+    P.force_div = force_div;
+    P.k = k_iter;
+    P.learn_flags = scene_only ? 0u : ((c->learn_on && c->mlp_ready ? 1u : 0u) | (c->resid_on ? 2u : 0u));
     P.learn_limit = c->learn_limit;
     {
         float ext = 0.0f;
@@ -249,8 +266,8 @@ int make_params(mpm_ctx* c, SimParams& P) {
     if ((vr = verify_divisor(c, P.dt, P.S, &P.fast_dt))) return vr;
     P.wall_mode = g.wall_mode;
     P.wall_cells = g.wall_cells;
-    P.n_bvh_nodes = (uint32_t)c->n_bvh_nodes;
-    P.has_collider = (c->n_bvh_nodes > 0 && c->n_coll_words > 0) ? 1u : 0u;
+    P.n_bvh_nodes = scene_only ? 0u : (uint32_t)c->n_bvh_nodes;
+    P.has_collider = (!scene_only && c->n_bvh_nodes > 0 && c->n_coll_words > 0) ? 1u : 0u;
     if (P.has_collider && P.coll_num_indices != c->coll_num_indices)
         return fail(c, MPM_E_RANGE, "uniform colliderNumIndices differs from the uploaded collider");
     P.margin = g.collider_margin;
@@ -258,8 +275,12 @@ int make_params(mpm_ctx* c, SimParams& P) {
     P.fr_prox = g.friction_proximity;
     P.vel_factor = g.collider_velocity_factor;
     if (!(P.dt > 0.0f)) return fail(c, MPM_E_RANGE, "simulationTimestep must be > 0");
-    if (c->k_cur < 1 || c->k_cur > 64) return fail(c, MPM_E_RANGE, "pbmpm iterations must be in 1..64");
+    if (k_iter < 1 || k_iter > 64) return fail(c, MPM_E_RANGE, "pbmpm iterations must be in 1..64");
     return MPM_OK;
+}
+int make_params(mpm_ctx* c, SimParams& P) {
+    if (c->ens) return fail(c, MPM_E_STATE, "this entry point is for single-scene contexts; use mpm_ensemble_*");
+    return params_from(c, c->uni, c->cfg, c->k_cur, c->force_div_cur, false, P);
 }
 
 uint32_t hash_host(uint32_t n) {
@@ -395,6 +416,120 @@ uint32_t bvh_rec(const std::vector<BTri>& tris, std::vector<uint32_t>& idx, uint
     return ni;
 }
 
+// Device buffers of a context, sized by n, stride, K, cap, max_blocks, max_runs, max_chunks, ntiles.
+int alloc_buffers(mpm_ctx* c) {
+    const uint64_t K = c->K;
+    int rc = MPM_OK;
+    const size_t ps = (size_t)NPLANES * c->stride;
+    if ((rc = dalloc(c, &c->planes[0], ps)) || (rc = dalloc(c, &c->planes[1], ps)) ||
+        (rc = dalloc(c, &c->pid[0], c->stride)) || (rc = dalloc(c, &c->pid[1], c->stride)) ||
+        (rc = dalloc(c, &c->relax, c->stride)) || (rc = dalloc(c, &c->key, c->stride)) ||
+        (rc = dalloc(c, &c->mask, c->stride)) || (rc = dalloc(c, &c->perm, c->stride)) ||
+        (rc = dalloc(c, &c->kcount, K)) || (rc = dalloc(c, &c->kmask, K)) || (rc = dalloc(c, &c->kcursor, K)) ||
+        (rc = dalloc(c, &c->tile_sums, 3 * (size_t)c->ntiles)) || (rc = dalloc(c, &c->tile_ofs, 3 * (size_t)c->ntiles)) ||
+        (rc = dalloc(c, &c->run_key, c->max_runs)) || (rc = dalloc(c, &c->run_start, c->max_runs)) ||
+        (rc = dalloc(c, &c->run_count, c->max_runs)) || (rc = dalloc(c, &c->run_mask, c->max_runs)) ||
+        (rc = dalloc(c, &c->chunks, c->max_chunks)) ||
+        (rc = dalloc(c, &c->table, c->cap)) || (rc = dalloc(c, &c->slot_val, c->cap)) ||
+        (rc = dalloc(c, &c->nbr, 27 * (size_t)c->max_runs)) ||
+        (rc = dalloc(c, &c->block_key, c->max_blocks)) ||
+        (rc = dalloc(c, &c->grid[0], 64 * (size_t)c->max_blocks)) || (rc = dalloc(c, &c->grid[1], 64 * (size_t)c->max_blocks)) ||
+        (rc = dalloc(c, &c->grid[2], 64 * (size_t)c->max_blocks)) ||
+        (rc = dalloc(c, &c->counters, C_NCOUNTERS)) || (rc = dalloc(c, &c->cqueue, c->stride)) || (rc = dalloc(c, &c->d_scratch, 4))) {
+        return rc;
+    }
+    // zero-initialised key-space accumulators and counters (kernels keep them zero afterwards)
+    k_fill_u32<<<grid_for(K, 4096), 256, 0, c->stream>>>(c->kcount, 0u, (uint32_t)K);
+    k_fill_u32<<<grid_for(K, 4096), 256, 0, c->stream>>>(c->kmask, 0u, (uint32_t)K);
+    k_iota<<<grid_for(c->stride, 4096), 256, 0, c->stream>>>(c->pid[0], c->stride);
+    k_fill_u32<<<1, 256, 0, c->stream>>>(c->counters, 0u, C_NCOUNTERS);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { (void)hipGetLastError(); return MPM_E_HIP; }
+    for (int i = 0; i < 4; ++i) hipEventCreate(&c->ev[i]);
+    return MPM_OK;
+}
+
+
+// ------------------------------------------------------------------ batched scenes
+Ens ens_bufs(mpm_ctx* c) {
+    Ens E;
+    E.sp = c->d_sp; E.part_base = c->d_part_base; E.key_base = c->d_key_base; E.nphase = c->d_nphase; E.method = c->d_method;
+    E.nblk = c->d_nblk; E.cbeg = c->d_cbeg; E.seg = c->d_seg; E.done = c->d_done; E.ctl = c->d_ctl;
+    E.S = c->ens; E.R = c->ens_R;
+    return E;
+}
+
+// One substep of every scene: shared binning, task table, the transfer phases (one persistent
+// launch, or one launch per phase), unbinned particles.
+void enqueue_ens_substep(mpm_ctx* c, bool full) {
+    hipStream_t s = c->stream;
+    const SimParams& P0 = c->ens_P0;
+    Bufs B = make_bufs(c);
+    Ens E = ens_bufs(c);
+    if (full) hipEventRecord(c->ev[0], s);
+    const uint32_t gk = grid_for(std::max<uint64_t>(c->n, c->cap), 8192);
+    kt(c, -1);
+    k_ens_keys_count<<<gk, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_KEYS);
+    k_scan_tiles<<<c->ntiles, 256, 0, s>>>(P0, B); kt(c, KID_SCAN_TILES);
+    k_scan_top<<<1, 256, 0, s>>>(P0, B, c->ntiles); kt(c, KID_SCAN_TOP);
+    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P0, B); kt(c, KID_SCAN_APPLY);
+    k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P0, B); kt(c, KID_SCATTER);
+    const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 8192);
+    k_ens_insert<<<gi, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_INSERT);
+    k_ens_compact<<<grid_for(c->cap, 1u << 30), 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_COMPACT);
+    k_ens_nbr<<<gi, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_NBR);
+    k_ens_tasks<<<1, 256, 0, s>>>(B, E); kt(c, KID_ENS_TASKS);
+    if (full) hipEventRecord(c->ev[1], s);
+    const uint32_t G = (uint32_t)c->n_cu * c->ens_wg_per_cu;
+    const int nl = c->ens_persistent ? 1 : (int)c->ens_R;
+    for (int r = 0; r < nl; ++r) {
+        const int ph = c->ens_persistent ? -1 : r;
+        if (c->ens_wg_per_cu >= 3) k_ens_transfer<3><<<G, 256, 0, s>>>(B, E, ph);
+        else k_ens_transfer<2><<<G, 256, 0, s>>>(B, E, ph);
+        kt(c, KID_ENS_TRANSFER);
+    }
+    if (full) hipEventRecord(c->ev[2], s);
+    k_ens_straggle<<<256, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_STRAGGLE);
+    if (full) hipEventRecord(c->ev[3], s);
+}
+
+int ens_graph(mpm_ctx* c) {
+    const int par = c->cur;
+    if (c->gexec[0][par] && c->ens_graph_ok[par]) return MPM_OK;
+    if (c->gexec[0][par]) { hipGraphExecDestroy(c->gexec[0][par]); c->gexec[0][par] = nullptr; }
+    if (c->graph[0][par]) { hipGraphDestroy(c->graph[0][par]); c->graph[0][par] = nullptr; }
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_ens_substep(c, false);
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+    hipGraphExec_t e = nullptr;
+    hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+    if (r != hipSuccess) { hipGraphDestroy(g); return fail(c, MPM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(r)); }
+    c->graph[0][par] = g;
+    c->gexec[0][par] = e;
+    c->ens_graph_ok[par] = true;
+    return MPM_OK;
+}
+
+// Per-scene parameters with the shared chunk size; uploads them.
+int ens_set_chunk(mpm_ctx* c, uint32_t chunk) {
+    c->chunk = chunk;
+    c->ens_P0.chunk = chunk;
+    for (auto& P : c->ens_sp) P.chunk = chunk;
+    HIPCHK(c, hipMemcpy(c->d_sp, c->ens_sp.data(), c->ens_sp.size() * sizeof(SimParams), hipMemcpyHostToDevice));
+    c->ens_graph_ok[0] = c->ens_graph_ok[1] = false;
+    return MPM_OK;
+}
+
+int ens_check_flags(mpm_ctx* c) {
+    uint32_t f = 0;
+    HIPCHK(c, hipMemcpy(&f, c->d_ctl + 2, 4, hipMemcpyDeviceToHost));
+    if (f & ENS_ERR_WAIT) {
+        const uint32_t z = 0;
+        HIPCHK(c, hipMemcpy(c->d_ctl + 2, &z, 4, hipMemcpyHostToDevice));
+        return fail(c, MPM_E_HIP, "batched step: a phase dependency wait hit its spin limit (results of that substep are invalid)");
+    }
+    return MPM_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -476,31 +611,8 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     c->max_runs = (uint32_t)std::min<uint64_t>(c->n, K);
     c->max_chunks = (c->n + c->chunk - 1) / c->chunk + c->max_runs;
     c->ntiles = (uint32_t)((K + 1023) / 1024);
-    int rc = MPM_OK;
-    const size_t ps = (size_t)NPLANES * c->stride;
-    if ((rc = dalloc(c, &c->planes[0], ps)) || (rc = dalloc(c, &c->planes[1], ps)) ||
-        (rc = dalloc(c, &c->pid[0], c->stride)) || (rc = dalloc(c, &c->pid[1], c->stride)) ||
-        (rc = dalloc(c, &c->relax, c->stride)) || (rc = dalloc(c, &c->key, c->stride)) ||
-        (rc = dalloc(c, &c->mask, c->stride)) || (rc = dalloc(c, &c->perm, c->stride)) ||
-        (rc = dalloc(c, &c->kcount, K)) || (rc = dalloc(c, &c->kmask, K)) || (rc = dalloc(c, &c->kcursor, K)) ||
-        (rc = dalloc(c, &c->tile_sums, 3 * (size_t)c->ntiles)) || (rc = dalloc(c, &c->tile_ofs, 3 * (size_t)c->ntiles)) ||
-        (rc = dalloc(c, &c->run_key, c->max_runs)) || (rc = dalloc(c, &c->run_start, c->max_runs)) ||
-        (rc = dalloc(c, &c->run_count, c->max_runs)) || (rc = dalloc(c, &c->run_mask, c->max_runs)) ||
-        (rc = dalloc(c, &c->chunks, c->max_chunks)) ||
-        (rc = dalloc(c, &c->table, c->cap)) || (rc = dalloc(c, &c->slot_val, c->cap)) ||
-        (rc = dalloc(c, &c->nbr, 27 * (size_t)c->max_runs)) ||
-        (rc = dalloc(c, &c->block_key, c->max_blocks)) ||
-        (rc = dalloc(c, &c->grid[0], 64 * (size_t)c->max_blocks)) || (rc = dalloc(c, &c->grid[1], 64 * (size_t)c->max_blocks)) ||
-        (rc = dalloc(c, &c->grid[2], 64 * (size_t)c->max_blocks)) ||
-        (rc = dalloc(c, &c->counters, C_NCOUNTERS)) || (rc = dalloc(c, &c->cqueue, c->stride)) || (rc = dalloc(c, &c->d_scratch, 4))) {
-        release(c); delete c; return rc;
-    }
-    // zero-initialised key-space accumulators and counters (kernels keep them zero afterwards)
-    k_fill_u32<<<grid_for(K, 4096), 256, 0, c->stream>>>(c->kcount, 0u, (uint32_t)K);
-    k_fill_u32<<<grid_for(K, 4096), 256, 0, c->stream>>>(c->kmask, 0u, (uint32_t)K);
-    k_fill_u32<<<1, 256, 0, c->stream>>>(c->counters, 0u, C_NCOUNTERS);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) { (void)hipGetLastError(); release(c); delete c; return MPM_E_HIP; }
-    for (int i = 0; i < 4; ++i) hipEventCreate(&c->ev[i]);
+    int rc = alloc_buffers(c);
+    if (rc) { release(c); delete c; return rc; }
     // uniform fields the runner writes at construction (GpuSnowPipelineRunner.svelte.ts:136-144,171-177)
     std::memset(c->uni, 0, sizeof(c->uni));
     wru<float>(c->uni, 0, cfg->dt);
@@ -615,6 +727,7 @@ int mpm_build_bvh(const float* pos, size_t n_vertices, const uint32_t* indices, 
 
 int mpm_upload_particles(mpm_ctx* c, const void* aos, size_t n) {
     if (!c || !aos) return MPM_E_INVALID;
+    if (c->ens) return fail(c, MPM_E_INVALID, "batched context: use mpm_ensemble_upload");
     if (n != c->n) return fail(c, MPM_E_INVALID, "particle count differs from the context's n_particles");
     HIPCHK(c, hipSetDevice(c->device));
     float* tmp = nullptr;
@@ -632,6 +745,7 @@ int mpm_upload_particles(mpm_ctx* c, const void* aos, size_t n) {
 
 int mpm_download_particles(mpm_ctx* c, void* aos, size_t n) {
     if (!c || !aos) return MPM_E_INVALID;
+    if (c->ens) return fail(c, MPM_E_INVALID, "batched context: use mpm_ensemble_download");
     if (n != c->n) return fail(c, MPM_E_INVALID, "particle count differs from the context's n_particles");
     if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
     HIPCHK(c, hipSetDevice(c->device));
@@ -686,7 +800,7 @@ int mpm_sync(mpm_ctx* c) {
     if (!c) return MPM_E_INVALID;
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    return MPM_OK;
+    return c->ens ? ens_check_flags(c) : MPM_OK;
 }
 
 int mpm_debug_stage(mpm_ctx* c, int method, int stage) {
@@ -1047,6 +1161,265 @@ int mpm_mlp_grad(const float* params, const float* norm, const float* feats, con
     if (e == hipSuccess) e = hipMemcpy(loss, dl, 4, hipMemcpyDeviceToHost);
     hipFree(dp); hipFree(dg); hipFree(dn); hipFree(df); hipFree(dt); hipFree(dy); hipFree(dl); hipFree(dw);
     return e == hipSuccess ? MPM_OK : MPM_E_HIP;
+}
+
+
+// ------------------------------------------------------------------ batched scenes (C ABI)
+int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes, mpm_ctx** out) {
+    if (!cfgs || !methods || !out || n_scenes < 1 || n_scenes > (int)ENS_MAX_SCENES) return MPM_E_INVALID;
+    *out = nullptr;
+    uint64_t n = 0, K = 0;
+    uint32_t R = 0;
+    for (int s = 0; s < n_scenes; ++s) {
+        const mpm_config& g = cfgs[s];
+        if (g.n_particles == 0 || (methods[s] != MPM_METHOD_EXPLICIT && methods[s] != MPM_METHOD_PBMPM)) return MPM_E_INVALID;
+        for (int a = 0; a < 3; ++a) if (g.grid_res[a] == 0 || g.grid_res[a] > 65536 || !(g.grid_max[a] > g.grid_min[a])) return MPM_E_INVALID;
+        if (g.pbmpm_iterations < 1 || g.pbmpm_iterations > 64) return MPM_E_INVALID;
+        n += g.n_particles;
+        K += (uint64_t)((g.grid_res[0] + 3u) / 4u) * ((g.grid_res[1] + 3u) / 4u) * ((g.grid_res[2] + 3u) / 4u);
+        R = std::max<uint32_t>(R, methods[s] == MPM_METHOD_PBMPM ? (uint32_t)g.pbmpm_iterations + 1u : 2u);
+    }
+    if (n >= 0xFFFFFF00ull || K >= 0xFFFFFF00ull || (uint64_t)R * n_scenes > ENS_MAX_SEG) return MPM_E_RANGE;
+    mpm_ctx* c = new mpm_ctx();
+    c->cfg = cfgs[0];
+    c->device = cfgs[0].device;
+    if (hipSetDevice(c->device) != hipSuccess) { (void)hipGetLastError(); delete c; return MPM_E_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { (void)hipGetLastError(); delete c; return MPM_E_HIP; }
+    c->ens = (uint32_t)n_scenes;
+    c->ens_R = R;
+    c->ens_cfg.assign(cfgs, cfgs + n_scenes);
+    c->ens_method.assign(methods, methods + n_scenes);
+    c->n = (uint32_t)n;
+    c->stride = (c->n + 63u) & ~63u;
+    c->K = (uint32_t)K;
+    c->chunk = cfgs[0].chunk_particles ? cfgs[0].chunk_particles : 512u;
+    c->max_blocks = (uint32_t)K;
+    c->cap = next_prime((uint32_t)std::min<uint64_t>(2ull * std::min<uint64_t>(K, 8ull * n) + 1, 0xFFFFFF00ull));
+    c->max_runs = (uint32_t)std::min<uint64_t>(n, K);
+    c->max_chunks = (c->n + 255u) / 256u + c->max_runs;      // room for chunks down to 256 particles
+    c->ntiles = (uint32_t)((K + 1023) / 1024);
+    int rc = alloc_buffers(c);
+    if (!rc) {
+        const size_t S = (size_t)n_scenes;
+        if ((rc = dalloc(c, &c->d_sp, S)) || (rc = dalloc(c, &c->d_part_base, S + 1)) || (rc = dalloc(c, &c->d_key_base, S + 1)) ||
+            (rc = dalloc(c, &c->d_nphase, S)) || (rc = dalloc(c, &c->d_method, S)) || (rc = dalloc(c, &c->d_nblk, S)) ||
+            (rc = dalloc(c, &c->d_cbeg, S + 1)) || (rc = dalloc(c, &c->d_seg, (size_t)R * S + 1)) ||
+            (rc = dalloc(c, &c->d_done, (size_t)R * S)) || (rc = dalloc(c, &c->d_ctl, ENS_CTL_PHASE + 128))) {}
+    }
+    if (!rc) {
+        // per-scene parameters from each configuration (no collider, no pointer interaction)
+        uint32_t pb = 0, kb = 0;
+        c->ens_sp.resize(n_scenes);
+        for (int s = 0; s < n_scenes && !rc; ++s) {
+            const mpm_config& g = cfgs[s];
+            uint8_t uni[MPM_UNIFORMS_BYTES];
+            std::memset(uni, 0, sizeof(uni));
+            wru<float>(uni, 0, g.dt);
+            wru<float>(uni, 4, g.fixed_point_scale);
+            for (int a = 0; a < 3; ++a) {
+                wru<float>(uni, 16 + 4 * a, g.grid_min[a]);
+                wru<float>(uni, 32 + 4 * a, g.grid_max[a]);
+                wru<uint32_t>(uni, 208 + 4 * a, g.grid_res[a]);
+                wru<float>(uni, 352 + 4 * a, (float)(((double)g.grid_max[a] - (double)g.grid_min[a]) / (double)g.grid_res[a]));
+            }
+            SimParams& P = c->ens_sp[s];
+            rc = params_from(c, uni, g, g.pbmpm_iterations, g.pbmpm_force_divisor, true, P);
+            if (rc) break;
+            P.n = g.n_particles;
+            P.chunk = c->chunk;
+            P.cap = c->cap;
+            P.max_probes = c->cfg.max_probes ? c->cfg.max_probes : 100u;
+            P.kbase = kb;
+            P.scene = (uint32_t)s;
+            P.blk_base = kb;            // a scene's dense block range is its key range
+            P.blk_cap = P.K;
+            P.max_blocks = P.K;
+            c->ens_part_base.push_back(pb);
+            c->ens_key_base.push_back(kb);
+            c->ens_nphase.push_back(methods[s] == MPM_METHOD_PBMPM ? (uint32_t)g.pbmpm_iterations + 1u : 2u);
+            pb += g.n_particles;
+            kb += P.K;
+        }
+        c->ens_part_base.push_back(pb);
+        c->ens_key_base.push_back(kb);
+    }
+    if (!rc) {
+        std::memset(&c->ens_P0, 0, sizeof(SimParams));
+        c->ens_P0.n = c->n; c->ens_P0.K = c->K; c->ens_P0.chunk = c->chunk; c->ens_P0.cap = c->cap;
+        c->ens_P0.max_blocks = c->max_blocks; c->ens_P0.max_probes = c->cfg.max_probes ? c->cfg.max_probes : 100u;
+        std::vector<uint32_t> meth(n_scenes);
+        for (int s = 0; s < n_scenes; ++s) meth[s] = (uint32_t)methods[s];
+        hipError_t e = hipMemcpy(c->d_sp, c->ens_sp.data(), n_scenes * sizeof(SimParams), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c->d_part_base, c->ens_part_base.data(), (n_scenes + 1) * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c->d_key_base, c->ens_key_base.data(), (n_scenes + 1) * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c->d_nphase, c->ens_nphase.data(), n_scenes * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(c->d_method, meth.data(), n_scenes * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(c->d_ctl, 0, (ENS_CTL_PHASE + 128) * 4);
+        if (e != hipSuccess) rc = MPM_E_HIP;
+    }
+    if (rc) { release(c); delete c; return rc; }
+    c->ens_loaded.assign(n_scenes, false);
+    c->ens_wg_per_cu = cfgs[0].grid_workgroups ? std::max<uint32_t>(1u, cfgs[0].grid_workgroups / (uint32_t)c->n_cu) : 2u;
+    *out = c;
+    return MPM_OK;
+}
+
+int mpm_ensemble_upload(mpm_ctx* c, int scene, const void* aos, size_t n) {
+    if (!c || !aos) return MPM_E_INVALID;
+    if (!c->ens || scene < 0 || scene >= (int)c->ens) return fail(c, MPM_E_INVALID, "not a batched context, or scene out of range");
+    const uint32_t base = c->ens_part_base[scene], cnt = c->ens_part_base[scene + 1] - base;
+    if (n != cnt) return fail(c, MPM_E_INVALID, "particle count differs from the scene's n_particles");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float* tmp = nullptr;
+    HIPCHK(c, hipMalloc(&tmp, n * MPM_PARTICLE_BYTES));
+    hipError_t e = hipMemcpyAsync(tmp, aos, n * MPM_PARTICLE_BYTES, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        k_upload_range<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(tmp, c->planes[c->cur], c->pid[c->cur], c->n, c->stride, base, cnt);
+        e = hipStreamSynchronize(c->stream);
+    }
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(c, MPM_E_HIP, std::string("scene upload: ") + hipGetErrorString(e));
+    c->ens_loaded[scene] = true;
+    c->has_particles = std::all_of(c->ens_loaded.begin(), c->ens_loaded.end(), [](bool b) { return b; });
+    return MPM_OK;
+}
+
+int mpm_ensemble_download(mpm_ctx* c, int scene, void* aos, size_t n) {
+    if (!c || !aos) return MPM_E_INVALID;
+    if (!c->ens || scene < 0 || scene >= (int)c->ens) return fail(c, MPM_E_INVALID, "not a batched context, or scene out of range");
+    const uint32_t base = c->ens_part_base[scene], cnt = c->ens_part_base[scene + 1] - base;
+    if (n != cnt) return fail(c, MPM_E_INVALID, "particle count differs from the scene's n_particles");
+    HIPCHK(c, hipSetDevice(c->device));
+    float* tmp = nullptr;
+    HIPCHK(c, hipMalloc(&tmp, n * MPM_PARTICLE_BYTES));
+    k_download_range<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(c->planes[c->cur], c->pid[c->cur], tmp, c->n, c->stride, base, cnt);
+    hipError_t e = hipMemcpyAsync(aos, tmp, n * MPM_PARTICLE_BYTES, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(c, MPM_E_HIP, std::string("scene download: ") + hipGetErrorString(e));
+    return ens_check_flags(c);
+}
+
+int mpm_ensemble_step(mpm_ctx* c, int n_substeps) {
+    if (!c) return MPM_E_INVALID;
+    if (!c->ens) return fail(c, MPM_E_INVALID, "not a batched context");
+    if (n_substeps < 0) return fail(c, MPM_E_INVALID, "negative substep count");
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "every scene needs its particles uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    int rc;
+    for (int s = 0; s < n_substeps; ++s) {
+        const bool timed = c->timing && s == n_substeps - 1;
+        if (c->cfg.use_graph && !timed && !c->ktiming) {
+            if ((rc = ens_graph(c))) return rc;
+            HIPCHK(c, hipGraphLaunch(c->gexec[0][c->cur], c->stream));
+        } else {
+            enqueue_ens_substep(c, timed);
+            HIPCHK(c, hipGetLastError());
+        }
+        c->cur ^= 1;
+        c->substeps++;
+        if (c->ktiming && c->kt_n > 512) kt_collect(c);
+        if (timed) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            float a = 0, b = 0, d = 0;
+            hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+            hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+            hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
+            c->tm.binning_ns = a * 1e6; c->tm.transfer_ns = b * 1e6; c->tm.integrate_ns = d * 1e6;
+            c->tm.compute_simulation_step_ns = (a + b + d) * 1e6;
+            c->tm.substeps = c->substeps;
+        }
+    }
+    kt_collect(c);
+    return MPM_OK;
+}
+
+int mpm_ensemble_set_tuning(mpm_ctx* c, const mpm_tuning* t) {
+    if (!c || !t) return MPM_E_INVALID;
+    if (!c->ens) return fail(c, MPM_E_INVALID, "not a batched context");
+    if (t->chunk_particles < 256 || t->chunk_particles > 65536 || t->workgroups_per_cu < 1 || t->workgroups_per_cu > 8)
+        return fail(c, MPM_E_RANGE, "chunk_particles must be in 256..65536 and workgroups_per_cu in 1..8");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc;
+    if ((rc = ens_set_chunk(c, t->chunk_particles))) return rc;
+    c->ens_wg_per_cu = t->workgroups_per_cu;
+    c->ens_persistent = t->persistent ? 1 : 0;
+    c->ens_graph_ok[0] = c->ens_graph_ok[1] = false;
+    return MPM_OK;
+}
+int mpm_ensemble_get_tuning(mpm_ctx* c, mpm_tuning* t) {
+    if (!c || !t) return MPM_E_INVALID;
+    if (!c->ens) return fail(c, MPM_E_INVALID, "not a batched context");
+    t->chunk_particles = c->chunk;
+    t->workgroups_per_cu = c->ens_wg_per_cu;
+    t->persistent = c->ens_persistent;
+    t->ms_per_substep = 0.0f;
+    return MPM_OK;
+}
+
+// Times every candidate on the live scenes (state restored after each trial and at the end) and
+// keeps the fastest. out receives up to cap candidates with their times; the chosen one is set.
+int mpm_ensemble_autotune(mpm_ctx* c, int trial_substeps, mpm_tuning* out, int cap, int* n_out, mpm_tuning* best) {
+    if (!c || trial_substeps < 1 || (cap > 0 && !out)) return MPM_E_INVALID;
+    if (!c->ens) return fail(c, MPM_E_INVALID, "not a batched context");
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "every scene needs its particles uploaded");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc;
+    const size_t ps = (size_t)NPLANES * c->stride;
+    if (!c->snap_planes) {
+        if ((rc = dalloc(c, &c->snap_planes, ps)) || (rc = dalloc(c, &c->snap_pid, c->stride))) return rc;
+    }
+    HIPCHK(c, hipMemcpy(c->snap_planes, c->planes[c->cur], ps * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(c, hipMemcpy(c->snap_pid, c->pid[c->cur], (size_t)c->stride * 4, hipMemcpyDeviceToDevice));
+    const int cur0 = c->cur;
+    const uint64_t sub0 = c->substeps;
+    const uint32_t chunks[] = {256, 512, 1024};
+    const uint32_t wgs[] = {1, 2, 3};
+    std::vector<mpm_tuning> cand;
+    for (int pers = 1; pers >= 0; --pers)
+        for (uint32_t ch : chunks)
+            for (uint32_t w : wgs) { mpm_tuning t{}; t.chunk_particles = ch; t.workgroups_per_cu = w; t.persistent = pers; cand.push_back(t); }
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    int bi = -1;
+    for (size_t i = 0; i < cand.size(); ++i) {
+        if ((rc = mpm_ensemble_set_tuning(c, &cand[i]))) break;
+        HIPCHK(c, hipMemcpy(c->planes[cur0], c->snap_planes, ps * 4, hipMemcpyDeviceToDevice));
+        HIPCHK(c, hipMemcpy(c->pid[cur0], c->snap_pid, (size_t)c->stride * 4, hipMemcpyDeviceToDevice));
+        c->cur = cur0;
+        if ((rc = mpm_ensemble_step(c, 2))) break;                   // builds both graph parities
+        HIPCHK(c, hipMemcpy(c->planes[cur0], c->snap_planes, ps * 4, hipMemcpyDeviceToDevice));
+        HIPCHK(c, hipMemcpy(c->pid[cur0], c->snap_pid, (size_t)c->stride * 4, hipMemcpyDeviceToDevice));
+        c->cur = cur0;
+        HIPCHK(c, hipEventRecord(e0, c->stream));
+        if ((rc = mpm_ensemble_step(c, trial_substeps))) break;
+        HIPCHK(c, hipEventRecord(e1, c->stream));
+        HIPCHK(c, hipEventSynchronize(e1));
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        cand[i].ms_per_substep = ms / (float)trial_substeps;
+        if ((rc = ens_check_flags(c))) break;
+        if (bi < 0 || cand[i].ms_per_substep < cand[bi].ms_per_substep) bi = (int)i;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->planes[cur0], c->snap_planes, ps * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(c, hipMemcpy(c->pid[cur0], c->snap_pid, (size_t)c->stride * 4, hipMemcpyDeviceToDevice));
+    c->cur = cur0;
+    c->substeps = sub0;
+    if (rc) return rc;
+    if ((rc = mpm_ensemble_set_tuning(c, &cand[bi]))) return rc;
+    for (int i = 0; i < cap && i < (int)cand.size(); ++i) out[i] = cand[i];
+    if (n_out) *n_out = (int)cand.size();
+    if (best) *best = cand[bi];
+    return MPM_OK;
 }
 
 int mpm_set_timing(mpm_ctx* c, int enabled) {

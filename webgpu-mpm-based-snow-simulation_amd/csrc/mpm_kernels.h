@@ -44,6 +44,9 @@ struct SimParams {
     int32_t res[3];
     int32_t nb[3];
     uint32_t K;                // number of blocks in the block grid (key space)
+    uint32_t kbase;            // first global block key of this scene (batched scenes; 0 otherwise)
+    uint32_t scene;            // scene index, salts the block hash (0 for a single scene)
+    uint32_t blk_base, blk_cap; // batched scenes: dense block range [blk_base, blk_base + blk_cap)
     uint32_t n;                // particles
     uint32_t chunk;            // particles per work chunk
     uint32_t cap;              // hash table slots
@@ -163,6 +166,7 @@ MPM_DEV uint32_t touched_mask(const SimParams& P, const int c[3]) {
 }
 
 MPM_DEV void block_coords(const SimParams& P, uint32_t key, int b[3]) {
+    key -= P.kbase;
     b[0] = (int)(key % (uint32_t)P.nb[0]);
     b[1] = (int)((key / (uint32_t)P.nb[0]) % (uint32_t)P.nb[1]);
     b[2] = (int)(key / ((uint32_t)P.nb[0] * (uint32_t)P.nb[1]));
@@ -271,7 +275,7 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
             mask = touched_mask(P, c);
             int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
             if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
-                key = (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+                key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
                 valid = true;
             }
             B.key[i] = key; B.mask[i] = mask;
@@ -381,9 +385,12 @@ __global__ __launch_bounds__(256) void k_scatter(SimParams P, Bufs B) {
 
 // Open-addressing insert of one block key (mapAffectedBlocks.wgsl:7-62 allocates the same set;
 // the table stores the key itself so a CAS on one word claims or finds the slot).
+MPM_DEV uint32_t block_hash(const SimParams& P, int bx, int by, int bz) {
+    return hash3((uint32_t)bx, (uint32_t)by, (uint32_t)bz + (P.scene << 12)) % P.cap;
+}
 MPM_DEV uint32_t table_insert(const SimParams& P, const Bufs& B, int bx, int by, int bz) {
-    const uint32_t key = (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
-    uint32_t s = hash3((uint32_t)bx, (uint32_t)by, (uint32_t)bz) % P.cap;
+    const uint32_t key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+    uint32_t s = block_hash(P, bx, by, bz);
     for (uint32_t i = 0; i < P.max_probes; ++i) {
         uint32_t prev = atomicCAS(&B.table[s], EMPTY, key);
         if (prev == EMPTY || prev == key) return s;
@@ -414,16 +421,23 @@ __global__ __launch_bounds__(256) void k_insert(SimParams P, Bufs B) {
     }
 }
 // Compaction of occupied slots to dense block indices (n_allocated_blocks / mapped_block_indexes).
+// One global atomic per workgroup reserves its blocks; waves take consecutive sub-ranges.
 __global__ __launch_bounds__(256) void k_compact(SimParams P, Bufs B) {
+    __shared__ uint32_t wcount[4], wbase[4];
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t w = threadIdx.x >> 6;
     uint32_t key = s < P.cap ? B.table[s] : EMPTY;
     bool occ = key != EMPTY;
     uint64_t om = __ballot(occ);
-    if (!om) return;
-    int leader = __ffsll((unsigned long long)om) - 1;
-    uint32_t b = 0;
-    if (lane_id() == leader) b = atomicAdd(&B.counters[C_NBLOCKS], (uint32_t)__popcll(om));
-    b = __shfl(b, leader, 64) + (uint32_t)__popcll(om & lanemask_lt());
+    if (lane_id() == 0) wcount[w] = (uint32_t)__popcll(om);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t t = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        uint32_t b = t ? atomicAdd(&B.counters[C_NBLOCKS], t) : 0u;
+        for (int i = 0; i < 4; ++i) { wbase[i] = b; b += wcount[i]; }
+    }
+    __syncthreads();
+    const uint32_t b = wbase[w] + (uint32_t)__popcll(om & lanemask_lt());
     if (occ) {
         if (b < P.max_blocks) { B.slot_val[s] = b; B.block_key[b] = key; }
         else { B.slot_val[s] = EMPTY; atomicAdd(&B.counters[C_DROPS], 1u); }
@@ -437,8 +451,8 @@ MPM_DEV uint32_t n_blocks_of(const SimParams& P, const Bufs& B) {
 // table's own hash): dense block index, or EMPTY if the block is not allocated.
 MPM_DEV uint32_t table_lookup(const SimParams& P, const Bufs& B, int bx, int by, int bz) {
     if (bx < 0 || by < 0 || bz < 0 || bx >= P.nb[0] || by >= P.nb[1] || bz >= P.nb[2]) return EMPTY;
-    const uint32_t key = (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
-    uint32_t s = hash3((uint32_t)bx, (uint32_t)by, (uint32_t)bz) % P.cap;
+    const uint32_t key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+    uint32_t s = block_hash(P, bx, by, bz);
     for (uint32_t i = 0; i < P.max_probes; ++i) {
         const uint32_t k = B.table[s];
         if (k == EMPTY) return EMPTY;
@@ -780,6 +794,33 @@ MPM_DEV void tile_load_vel(const SimParams& P, Tile& T, const int4* __restrict__
         T.vel[t] = v;
     }
 }
+// Explicit grid update of the tile's nodes on the fly: the node velocity G2P reads after
+// gridUpdate.cs.wgsl:35-53 has run, computed from the P2G sums with the same operations
+// (v = p / m + g dt, sticky walls, p' = i32(v m S), then v' = (p' / S) / m as tile_load_vel).
+MPM_DEV void tile_load_vel_updated(const SimParams& P, Tile& T, const int4* __restrict__ g) {
+    for (int t = threadIdx.x; t < 216; t += blockDim.x) {
+        const int idx = T.node[t];
+        float4 v = make_float4(0.f, 0.f, 0.f, idx >= 0 ? 1.0f : 0.0f);
+        if (idx >= 0) {
+            const int4 q = g[idx];
+            const float m = (float)q.x / P.S;
+            if (m > 0.0f) {
+                const int tx = t % 6, ty = (t / 6) % 6, tz = t / 36;
+                const bool sticky = (P.wall_mode & 1u) && sticky_cell(P, T.base[0] + tx, T.base[1] + ty, T.base[2] + tz);
+                v3 u = mk((float)q.y / P.S, (float)q.z / P.S, (float)q.w / P.S) / m;
+                u = u + mk(P.grav[0], P.grav[1], P.grav[2]) * P.dt;
+                if (sticky) u = mk(0, 0, 0);
+                const v3 nm = (u * m) * P.S;
+                v.x = ((float)f2i(nm.x) / P.S) / m;
+                v.y = ((float)f2i(nm.y) / P.S) / m;
+                v.z = ((float)f2i(nm.z) / P.S) / m;
+                v.w = 2.0f;
+                if (sticky) { v.x = 0.f; v.y = 0.f; v.z = 0.f; }
+            }
+        }
+        T.vel[t] = v;
+    }
+}
 MPM_DEV void tile_zero_acc(Tile& T) {
     for (int t = threadIdx.x; t < 217; t += blockDim.x) { T.acc[0][t] = 0; T.acc[1][t] = 0; T.acc[2][t] = 0; T.acc[3][t] = 0; }
 }
@@ -944,20 +985,14 @@ __global__ void k_resid_roll(Bufs B, int k, uint32_t* peak) {
     B.resid_max[it] = 0u;
 }
 
+// One work chunk of a PBMPM transfer pass (the body of k_pbmpm; the batched-scene kernel calls
+// it with the chunk's scene parameters).
 template <int MODE>
-__global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
-    __shared__ Tile T;
-    const uint32_t nchunks = B.counters[C_NCHUNKS];
-    if (gclr >= 0) {
-        int4* gc = grid_sel(B, gclr);
-        const uint32_t ncell = n_blocks_of(P, B) * 64u;
-        for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) gc[c] = make_int4(0, 0, 0, 0);
-    }
-    const int4* gi = MODE != PB_FIRST ? grid_sel(B, gin) : nullptr;
-    int4* go = MODE != PB_LAST ? grid_sel(B, gout) : nullptr;
+MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, const int4* __restrict__ gi,
+                         int4* __restrict__ go, int it) {
     const uint32_t sd = B.stride;
-    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        tile_setup(P, B, T, B.chunks[ch]);
+    {
+        tile_setup(P, B, T, chunk);
         if (MODE != PB_FIRST) tile_load_vel(P, T, gi);
         if (MODE != PB_LAST) tile_zero_acc(T);
         __syncthreads();
@@ -1038,16 +1073,26 @@ __global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, 
     }
 }
 
-// ------------------------------------------------------------------ explicit MPM
-// particleToGrid.cs.wgsl:27-79 (hardening stressTensorOps.wgsl:39-56, PK1 stress :9-22)
-template <int DUMMY = 0>
-__global__ __launch_bounds__(256) void k_explicit_p2g(SimParams P, Bufs B) {
+template <int MODE>
+__global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
     __shared__ Tile T;
     const uint32_t nchunks = B.counters[C_NCHUNKS];
+    if (gclr >= 0) {
+        int4* gc = grid_sel(B, gclr);
+        const uint32_t ncell = n_blocks_of(P, B) * 64u;
+        for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) gc[c] = make_int4(0, 0, 0, 0);
+    }
+    const int4* gi = MODE != PB_FIRST ? grid_sel(B, gin) : nullptr;
+    int4* go = MODE != PB_LAST ? grid_sel(B, gout) : nullptr;
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) pbmpm_chunk<MODE>(P, B, T, B.chunks[ch], gi, go, it);
+}
+
+// ------------------------------------------------------------------ explicit MPM
+// particleToGrid.cs.wgsl:27-79 (hardening stressTensorOps.wgsl:39-56, PK1 stress :9-22)
+MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, int4* __restrict__ go) {
     const uint32_t sd = B.stride;
-    int4* go = B.grid0;
-    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        tile_setup(P, B, T, B.chunks[ch]);
+    {
+        tile_setup(P, B, T, chunk);
         tile_zero_acc(T);
         __syncthreads();
         const bool ok = T.center_ok != 0;
@@ -1077,7 +1122,12 @@ __global__ __launch_bounds__(256) void k_explicit_p2g(SimParams P, Bufs B) {
             Stencil s; stencil(P, T, pos, s);
             v3 d[3]; bspline_d(s.f, d);
             const v3 cdv = mk(P.cd[0], P.cd[1], P.cd[2]);
-            for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
+#pragma unroll
+            for (int oz = -1; oz <= 1; ++oz)
+#pragma unroll
+            for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll
+            for (int ox = -1; ox <= 1; ++ox) {
                 const int t = tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz);
                 if (T.node[t] < 0) continue;
                 const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
@@ -1097,6 +1147,12 @@ __global__ __launch_bounds__(256) void k_explicit_p2g(SimParams P, Bufs B) {
         tile_flush(T, go);
         __syncthreads();
     }
+}
+template <int DUMMY = 0>
+__global__ __launch_bounds__(256) void k_explicit_p2g(SimParams P, Bufs B) {
+    __shared__ Tile T;
+    const uint32_t nchunks = B.counters[C_NCHUNKS];
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) explicit_p2g_chunk(P, B, T, B.chunks[ch], B.grid0);
 }
 // gridUpdate.cs.wgsl:35-53 over the live blocks
 __global__ __launch_bounds__(256) void k_grid_update(SimParams P, Bufs B) {
@@ -1118,14 +1174,15 @@ __global__ __launch_bounds__(256) void k_grid_update(SimParams P, Bufs B) {
     }
 }
 // gridToParticle.cs.wgsl:28-80 + integrateParticles.wgsl:12-72
-template <int DUMMY = 0>
-__global__ __launch_bounds__(256) void k_explicit_g2p(SimParams P, Bufs B) {
-    __shared__ Tile T;
-    const uint32_t nchunks = B.counters[C_NCHUNKS];
+// One work chunk of the explicit G2P + integrate. FUSED: the tile applies the grid update
+// (gridUpdate.cs.wgsl:35-53) to its nodes itself, reading the P2G sums instead of the updated grid.
+template <bool FUSED>
+MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, const int4* __restrict__ g) {
     const uint32_t sd = B.stride;
-    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-        tile_setup(P, B, T, B.chunks[ch]);
-        tile_load_vel(P, T, B.grid0);
+    {
+        tile_setup(P, B, T, chunk);
+        if (FUSED) tile_load_vel_updated(P, T, g);
+        else tile_load_vel(P, T, g);
         __syncthreads();
         const bool ok = T.center_ok != 0;
         const uint32_t rounds = (T.count + blockDim.x - 1) / blockDim.x;
@@ -1144,7 +1201,12 @@ __global__ __launch_bounds__(256) void k_explicit_g2p(SimParams P, Bufs B) {
                 v3 d[3]; bspline_d(s.f, d);
                 v3 nv = mk(0, 0, 0);
                 m3 G = zero3();
-                for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
+#pragma unroll
+                for (int oz = -1; oz <= 1; ++oz)
+#pragma unroll
+                for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll
+                for (int ox = -1; ox <= 1; ++ox) {
                     const float4 qv = T.vel[tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz)];
                     if (qv.w != 2.0f) continue;
                     const v3 v = mk(qv.x, qv.y, qv.z);
@@ -1173,13 +1235,18 @@ __global__ __launch_bounds__(256) void k_explicit_g2p(SimParams P, Bufs B) {
     }
 }
 
+template <int DUMMY = 0>
+__global__ __launch_bounds__(256) void k_explicit_g2p(SimParams P, Bufs B) {
+    __shared__ Tile T;
+    const uint32_t nchunks = B.counters[C_NCHUNKS];
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) explicit_g2p_chunk<false>(P, B, T, B.chunks[ch], B.grid0);
+}
+
 // Unbinned particles: copied to the new order and integrated with their stored dx / D
 // (no grid transfer reaches them). Runs over the sorted tail [n_valid, N).
-__global__ __launch_bounds__(256) void k_straggle(SimParams P, Bufs B) {
-    const uint32_t nvalid = B.counters[C_NVALID];
+MPM_DEV void straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_t old) {
     const uint32_t sd = B.stride;
-    for (uint32_t q = nvalid + blockIdx.x * blockDim.x + threadIdx.x; q < P.n; q += gridDim.x * blockDim.x) {
-        const uint32_t old = B.perm[q];
+    {
         B.pid_nxt[q] = B.pid_cur[old];
         Part p;
         p.pos = ld3(B.cur, sd, PX, old); p.vel = ld3(B.cur, sd, VX, old); p.mass = ld(B.cur, sd, PM, old);
@@ -1191,6 +1258,11 @@ __global__ __launch_bounds__(256) void k_straggle(SimParams P, Bufs B) {
         stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
         st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
     }
+}
+__global__ __launch_bounds__(256) void k_straggle(SimParams P, Bufs B) {
+    const uint32_t nvalid = B.counters[C_NVALID];
+    for (uint32_t q = nvalid + blockIdx.x * blockDim.x + threadIdx.x; q < P.n; q += gridDim.x * blockDim.x)
+        straggle_one(P, B, q, B.perm[q]);
 }
 // Copy of the state without integration (debug-stage prefixes leave particles untouched).
 __global__ void k_copy_state(Bufs B, uint32_t n) {

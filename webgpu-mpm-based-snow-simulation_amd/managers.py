@@ -331,6 +331,107 @@ class MpmSimulation:
         return mean, peak, ns.value
 
 
+class MpmEnsemble:
+    """Many independent scenes stepped together by one libmpm_hip context (no reference
+    counterpart: the reference runs one scene). Each scene is a dict of MpmSimulation's keyword
+    arguments plus n_particles and method (METHOD_EXPLICIT / METHOD_PBMPM). Scenes have no collider
+    and no pointer interaction. step(n) advances every scene by n of its own substeps; each scene's
+    particles come out exactly as if that scene were stepped alone."""
+
+    def __init__(self, scenes, **common):
+        self.lib = _lib.load_library()
+        S = len(scenes)
+        cfgs = (_lib.MpmConfig * S)()
+        methods = (ctypes.c_int * S)()
+        self.sizes = []
+        for i, sc in enumerate(scenes):
+            kw = dict(sc)
+            methods[i] = int(kw.pop("method"))
+            n = int(kw.pop("n_particles"))
+            kw.setdefault("grid_min", (-5, -5, -5))
+            kw.setdefault("grid_max", (5, 5, 5))
+            kw.update(common)
+            c = _lib.default_config(n_particles=n, grid_res=list(kw.pop("grid_res")), grid_min=list(kw.pop("grid_min")),
+                                    grid_max=list(kw.pop("grid_max")), **kw)
+            cfgs[i] = c
+            self.sizes.append(n)
+        h = ctypes.c_void_p()
+        rc = self.lib.mpm_ensemble_create(ctypes.cast(cfgs, ctypes.c_void_p), ctypes.cast(methods, ctypes.c_void_p), S, ctypes.byref(h))
+        if rc != 0:
+            raise _lib.MpmError(rc, "mpm_ensemble_create")
+        self.ctx = h
+        self.n_scenes = S
+
+    def close(self):
+        if self.ctx:
+            self.lib.mpm_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene, particles):
+        p = np.ascontiguousarray(particles)
+        assert p.dtype == PARTICLE_DTYPE and p.shape[0] == self.sizes[scene]
+        check(self.lib, self.ctx, self.lib.mpm_ensemble_upload(self.ctx, int(scene), _ptr(p), p.shape[0]))
+
+    def download(self, scene):
+        out = np.zeros(self.sizes[scene], PARTICLE_DTYPE)
+        check(self.lib, self.ctx, self.lib.mpm_ensemble_download(self.ctx, int(scene), _ptr(out), out.shape[0]))
+        return out
+
+    def step(self, n_substeps=1):
+        check(self.lib, self.ctx, self.lib.mpm_ensemble_step(self.ctx, int(n_substeps)))
+
+    def sync(self):
+        check(self.lib, self.ctx, self.lib.mpm_sync(self.ctx))
+
+    def set_tuning(self, chunk_particles=512, workgroups_per_cu=2, persistent=True):
+        t = _lib.MpmTuning(chunk_particles, workgroups_per_cu, 1 if persistent else 0, 0.0)
+        check(self.lib, self.ctx, self.lib.mpm_ensemble_set_tuning(self.ctx, ctypes.byref(t)))
+
+    def tuning(self):
+        t = _lib.MpmTuning()
+        check(self.lib, self.ctx, self.lib.mpm_ensemble_get_tuning(self.ctx, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def autotune(self, trial_substeps=3):
+        """Times every candidate tuning on the live scenes (state restored) and keeps the
+        fastest: (best, [all candidates with ms_per_substep])."""
+        cap = 64
+        arr = (_lib.MpmTuning * cap)()
+        n = ctypes.c_int()
+        best = _lib.MpmTuning()
+        check(self.lib, self.ctx, self.lib.mpm_ensemble_autotune(self.ctx, int(trial_substeps), ctypes.cast(arr, ctypes.c_void_p),
+                                                                 cap, ctypes.byref(n), ctypes.byref(best)))
+        as_dict = lambda t: {k: getattr(t, k) for k, _ in t._fields_}
+        return as_dict(best), [as_dict(arr[i]) for i in range(min(n.value, cap))]
+
+    def stats(self):
+        s = _lib.MpmStats()
+        check(self.lib, self.ctx, self.lib.mpm_get_stats(self.ctx, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in s._fields_}
+
+    def set_kernel_timing(self, on=True):
+        check(self.lib, self.ctx, self.lib.mpm_set_kernel_timing(self.ctx, 1 if on else 0))
+
+    def kernel_times(self):
+        ms = np.zeros(_lib.KERNEL_COUNT, np.float64)
+        cnt = np.zeros(_lib.KERNEL_COUNT, np.uint64)
+        check(self.lib, self.ctx, self.lib.mpm_get_kernel_times(self.ctx, _ptr(ms), _ptr(cnt)))
+        return {self.lib.mpm_kernel_name(i).decode(): (float(ms[i]), int(cnt[i]))
+                for i in range(_lib.KERNEL_COUNT) if cnt[i]}
+
+
 def mlp_grad(params, norm, feats, targs):
     """MFMA forward + backward of the warm-start network over the given samples (count a
     multiple of 16): (gradient [6288], standardised outputs [n][16], sum of squared errors)."""

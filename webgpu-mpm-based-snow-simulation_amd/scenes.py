@@ -7,6 +7,9 @@
                   12-triangle box collider of side 0.2 at the domain centre.
   explicit_slab_terrain  slab 0.6 x 0.1 x 0.6 at y = 0.7 over a 64 x 64-vertex heightfield
                   (7,938 triangles), 256^3 grid, dt = 5e-5.
+  ensemble_scenes  64 independent scenes for the batched step: N log-uniform in [8,192, 524,288],
+                  grid by count (32^3 / 64^3 / 128^3), half explicit, half PBMPM with k in 2..6,
+                  sphere or cube emitters with velocity U[-3,3]^3.
 Gravity is -9.8 along +y (the scenes are y-up). Masses use one unit per particle so the
 2^16 fixed-point grid sums keep their resolution (the reference's random masses in [0,1]
 would truncate to zero at these particle spacings).
@@ -102,3 +105,36 @@ def scene(name, n_particles=None, seed=None):
                    gravity=(0.0, -9.8, 0.0))
         return dict(particles=p, sim=sim, collider=heightfield_mesh(64, 64, 1.0, 0.05, 0.25, rng), method=0, substeps=2000)
     raise KeyError(name)
+
+
+def ensemble_scenes(n_scenes=64, seed=3, n_min=8192, n_max=524288):
+    """Scenes of the batched workload (seed of scene s: 1000 seed + s). Each entry:
+    dict(particles, sim = MpmEnsemble scene kwargs including method and n_particles)."""
+    out = []
+    for s in range(n_scenes):
+        rng = np.random.default_rng(1000 * seed + s)
+        n = int(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        res = 32 if n < 32768 else (64 if n < 131072 else 128)
+        explicit = s % 2 == 0
+        # emitter of about 12 particles per cell
+        vol = n * (1.0 / res) ** 3 / 12.0
+        centre = rng.uniform(0.35, 0.65, 3)
+        if rng.random() < 0.5:
+            r = (3.0 * vol / (4.0 * np.pi)) ** (1.0 / 3.0)
+            pos = sphere_points(centre, r, n, rng)
+        else:
+            side = vol ** (1.0 / 3.0)
+            pos = (centre - 0.5 * side + side * rng.random((n, 3))).astype(np.float32)
+        v = rng.uniform(-3.0, 3.0, 3).astype(np.float32)
+        p = make_particles(pos, vel=v)
+        sim = dict(n_particles=n, grid_res=(res, res, res), grid_min=(0, 0, 0), grid_max=(1, 1, 1),
+                   gravity=(0.0, -9.8, 0.0))
+        if explicit:
+            sim.update(method=0, dt=1e-4, wall_mode=1, wall_cells=3, hardening_explicit=10.0)
+        else:
+            k = int(rng.integers(2, 7))
+            dt = 1.0 / 600
+            p["dx"][:, :3] = v * np.float32(dt)
+            sim.update(method=1, dt=dt, pbmpm_iterations=k, pbmpm_force_divisor=float(k))
+        out.append(dict(particles=p, sim=sim))
+    return out
