@@ -130,6 +130,13 @@ int  mpm_build_bvh(const float* positions, size_t n_vertices, const uint32_t* in
 int  mpm_upload_particles(mpm_ctx* ctx, const void* aos192, size_t n);
 int  mpm_download_particles(mpm_ctx* ctx, void* aos192, size_t n);
 
+/* scatterParticlesInMeshVolume (GpuSnowPipelineRunner.svelte.ts:368-381) with the scatterParticles
+ * pass (particleScatter.cs.wgsl:40-84): fills every particle record of the context by rejection
+ * sampling inside a closed triangle mesh. mesh_xyzw holds consecutive triangle vertices, 16 bytes
+ * each (the GpuMeshBufferManager.ts:29-43 buffer); the sampling box is the uniforms'
+ * meshMinCoords / meshMaxCoords (@176 / @192, GpuUniformsBufferManager.ts:86-92). */
+int  mpm_scatter_particles(mpm_ctx* ctx, const float* mesh_xyzw, size_t n_vertices);
+
 /* addExplicitMpmDispatches / addPbmpmDispatches x n_substeps, then queue.submit
  * (GpuSnowPipelineRunner.svelte.ts:392-445, GpuMpmPipelineManager.ts:327-521). Enqueues on the
  * context's stream and returns; mpm_sync waits. */

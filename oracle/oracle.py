@@ -88,6 +88,10 @@ def lib():
             "orc_bspline": (None, [_P, _P, _P]),
             "orc_closest_point_triangle": (None, [_P, _P, _P, _P, _P]),
             "orc_intersect_ray_triangle": (ctypes.c_float, [_P, _P, _P, _P, _P]),
+            "orc_noise3": (ctypes.c_float, [ctypes.c_float] * 3),
+            "orc_fbm": (ctypes.c_float, [ctypes.c_float] * 3),
+            "orc_inside_mesh": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
+            "orc_scatter": (None, [_P, ctypes.c_uint32, _P, _P, ctypes.c_uint32, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -191,4 +195,16 @@ def f32_vec(fn, *arrays):
     out = np.empty(arrays[0].size, np.float32)
     for i in range(out.size):
         out[i] = fn(*(float(a[i]) for a in arrays))
+    return out
+
+
+def scatter(mesh_xyzw, mesh_min, mesh_max, n):
+    """scatterParticles (particleScatter.cs.wgsl:40-84) for particle indices [0, n): 192-byte
+    records as a float32 array of shape (n, 48). mesh_xyzw: consecutive triangle vertices,
+    16 bytes each (the WGSL array<vec3f> stride)."""
+    v = np.ascontiguousarray(mesh_xyzw, np.float32).reshape(-1, 4)
+    mn = np.ascontiguousarray(mesh_min, np.float32)
+    mx = np.ascontiguousarray(mesh_max, np.float32)
+    out = np.zeros((n, 48), np.float32)
+    lib().orc_scatter(_ptr(v), v.shape[0], _ptr(mn), _ptr(mx), n, _ptr(out))
     return out

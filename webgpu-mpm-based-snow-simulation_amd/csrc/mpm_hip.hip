@@ -18,6 +18,7 @@
 #include "mpm_kernels.h"
 #include "mpm_mlp.h"
 #include "mpm_ensemble.h"
+#include "mpm_scatter.h"
 #include "../../include/mpm_hip.h"
 
 using namespace mpm;
@@ -739,6 +740,28 @@ int mpm_upload_particles(mpm_ctx* c, const void* aos, size_t n) {
     }
     hipFree(tmp);
     if (e != hipSuccess) return fail(c, MPM_E_HIP, std::string("particle upload: ") + hipGetErrorString(e));
+    c->has_particles = true;
+    return MPM_OK;
+}
+
+int mpm_scatter_particles(mpm_ctx* c, const float* mesh_xyzw, size_t n_vertices) {
+    if (!c || !mesh_xyzw) return MPM_E_INVALID;
+    if (c->ens) return fail(c, MPM_E_INVALID, "batched context: scatter into a single-scene context");
+    if (n_vertices < 3 || n_vertices % 3 || n_vertices > (1ull << 30)) return fail(c, MPM_E_INVALID, "mesh vertex count must be a positive multiple of 3");
+    HIPCHK(c, hipSetDevice(c->device));
+    float mn[3], mx[3];
+    for (int a = 0; a < 3; ++a) { mn[a] = rdu<float>(c->uni, 176 + 4 * a); mx[a] = rdu<float>(c->uni, 192 + 4 * a); }
+    float4* dm = nullptr;
+    HIPCHK(c, hipMalloc(&dm, n_vertices * 16));
+    hipError_t e = hipMemcpyAsync(dm, mesh_xyzw, n_vertices * 16, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) {
+        k_scatter_init<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(dm, (uint32_t)(n_vertices / 3), mn[0], mn[1], mn[2], mx[0], mx[1], mx[2],
+                                                                        c->planes[c->cur], c->pid[c->cur], c->n, c->stride);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    hipFree(dm);
+    if (e != hipSuccess) return fail(c, MPM_E_HIP, std::string("particle scatter: ") + hipGetErrorString(e));
     c->has_particles = true;
     return MPM_OK;
 }

@@ -54,6 +54,8 @@ class GpuUniformsBufferManager:
     def writeMeshMinCoords(self, mn): self._f(176, mn)
     def writeMeshMaxCoords(self, mx): self._f(192, mx)
     def writeGridResolution(self, res): self._u(208, res)
+    def writeMeshMinCoords(self, mn): self._f(176, mn)
+    def writeMeshMaxCoords(self, mx): self._f(192, mx)
     def writeColliderMinCoords(self, mn): self._f(224, mn)
     def writeColliderMaxCoords(self, mx): self._f(240, mx)
     def writeColliderTransformMat(self, m): self._f(256, m)
@@ -126,6 +128,25 @@ class GpuColliderBufferManager:
         lib, ctx = sim.lib, sim.ctx
         check(lib, ctx, lib.mpm_set_collider(ctx, _ptr(self.colliderData), self.colliderData.size, self.numIndices,
                                              _ptr(self.bvhNodes), self.numBvhNodes))
+
+
+class GpuMeshBufferManager:
+    """src/lib/gpu/particleInitialize/GpuMeshBufferManager.ts:7-45: the scatter mesh as
+    consecutive triangle vertices, 16 bytes each, and its bounding box."""
+
+    def __init__(self, vertices):
+        v = np.asarray(vertices, np.float64).reshape(-1, 3)
+        self.numVertices = v.shape[0]
+        self.minCoords = v.min(axis=0).tolist() if len(v) else [np.inf] * 3
+        self.maxCoords = v.max(axis=0).tolist() if len(v) else [-np.inf] * 3
+        self.meshVertices = np.zeros((v.shape[0], 4), np.float32)
+        self.meshVertices[:, :3] = v
+
+
+def triangle_soup(vertices, indices):
+    """Indexed mesh -> consecutive triangle vertices (the scatter mesh layout)."""
+    v = np.asarray(vertices, np.float64).reshape(-1, 3)
+    return v[np.asarray(indices, np.int64).ravel()]
 
 
 class GpuMpmBufferManager:
@@ -211,6 +232,18 @@ class MpmSimulation:
 
     def updateColliderVel(self, v):
         self.uniformsManager.writeColliderVel(v)
+
+    # ---- particle init (GpuSnowPipelineRunner.svelte.ts:158-160, 368-381)
+    def set_mesh(self, vertices):
+        """The scatter mesh: consecutive triangle vertices (see triangle_soup)."""
+        self.meshManager = GpuMeshBufferManager(vertices)
+        self.uniformsManager.writeMeshMinCoords(self.meshManager.minCoords)
+        self.uniformsManager.writeMeshMaxCoords(self.meshManager.maxCoords)
+        return self.meshManager
+
+    def scatterParticlesInMeshVolume(self):
+        mv = np.ascontiguousarray(self.meshManager.meshVertices)
+        check(self.lib, self.ctx, self.lib.mpm_scatter_particles(self.ctx, _ptr(mv), mv.shape[0]))
 
     # ---- particles
     def upload(self, particles):
