@@ -137,6 +137,18 @@ int  mpm_download_particles(mpm_ctx* ctx, void* aos192, size_t n);
  * meshMinCoords / meshMaxCoords (@176 / @192, GpuUniformsBufferManager.ts:86-92). */
 int  mpm_scatter_particles(mpm_ctx* ctx, const float* mesh_xyzw, size_t n_vertices);
 
+/* Render-side particle splats: trilinear u32 fixed-point mass sums on a dense grid, cleared first.
+ *  MPM_SPLAT_DENSITY  calculateDensity (marchingCubes/mcDensity.cs.wgsl:14-63) on a res[0..2] grid
+ *                     (the reference sizes it floor(grid resolution / 2), GpuMarchingCubesBufferManager.ts:44-48)
+ *  MPM_SPLAT_MASS     calculateGridMass (volumetric/calculateGridMass.wgsl:6-51) on the uniforms'
+ *                     gridResolution (res ignored)
+ * The grid stays on the device (mpm_splat_grid, valid until the next larger splat or destroy) and is
+ * copied to host_out (cap cells) when host_out is non-null. Enqueued on the context's stream. */
+#define MPM_SPLAT_DENSITY 0
+#define MPM_SPLAT_MASS 1
+int  mpm_splat(mpm_ctx* ctx, int kind, const uint32_t* res, uint32_t* host_out, size_t cap);
+void* mpm_splat_grid(mpm_ctx* ctx);
+
 /* addExplicitMpmDispatches / addPbmpmDispatches x n_substeps, then queue.submit
  * (GpuSnowPipelineRunner.svelte.ts:392-445, GpuMpmPipelineManager.ts:327-521). Enqueues on the
  * context's stream and returns; mpm_sync waits. */

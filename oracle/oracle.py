@@ -92,6 +92,7 @@ def lib():
             "orc_fbm": (ctypes.c_float, [ctypes.c_float] * 3),
             "orc_inside_mesh": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
             "orc_scatter": (None, [_P, ctypes.c_uint32, _P, _P, ctypes.c_uint32, _P]),
+            "orc_splat": (None, [ctypes.c_int, _P, ctypes.c_uint32, _P, _P, ctypes.c_float, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -208,3 +209,15 @@ def scatter(mesh_xyzw, mesh_min, mesh_max, n):
     out = np.zeros((n, 48), np.float32)
     lib().orc_scatter(_ptr(v), v.shape[0], _ptr(mn), _ptr(mx), n, _ptr(out))
     return out
+
+
+def splat(kind, particles, gmin, gmax, S, res):
+    """calculateDensity (kind 0) / calculateGridMass (kind 1) of 192-byte records: u32 grid of
+    shape (rz, ry, rx)."""
+    a = np.ascontiguousarray(particles).view(np.float32).reshape(-1, 48)
+    g0 = np.ascontiguousarray(gmin, np.float32)
+    g1 = np.ascontiguousarray(gmax, np.float32)
+    r = np.ascontiguousarray(res, np.uint32)
+    out = np.zeros(int(r[0]) * int(r[1]) * int(r[2]), np.uint32)
+    lib().orc_splat(int(kind), _ptr(a), a.shape[0], _ptr(g0), _ptr(g1), float(S), _ptr(r), _ptr(out))
+    return out.reshape(int(r[2]), int(r[1]), int(r[0]))

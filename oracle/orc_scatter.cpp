@@ -1,4 +1,5 @@
-// Oracle of the particle scatter init — TEST INFRASTRUCTURE ONLY (tests/ and bench.py load it).
+// Oracle of the particle scatter init and the render-side splats — TEST INFRASTRUCTURE ONLY
+// (tests/ and bench.py load it).
 //
 // CPU restatement, FP32 with no contraction, of
 //   src/lib/gpu/particleInitialize/particleScatter.cs.wgsl:7-84   randFloat, randVec3,
@@ -8,6 +9,8 @@
 // Operation order follows the WGSL text left to right; mix(a, b, t) = a (1 - t) + b t (WGSL
 // definition), dot = (x x + y y) + z z. f32(u32) rounds to nearest even and f32(0xFFFFFFFF) = 2^32,
 // so every u32 / f32(0xFFFFFFFF) is an exact scaling by 2^-32.
+//   src/lib/gpu/marchingCubes/mcDensity.cs.wgsl:14-63             calculateDensity
+//   src/lib/gpu/volumetric/calculateGridMass.wgsl:6-51             calculateGridMass
 // Pinned (tests/test_scatter_oracle.py) by: noise3 at lattice points equals the hash value,
 // fbm's octave sum at lattice points, the mass closed form (hash1(i) 2^-32)^2, the first
 // candidate of every particle from the integer hash stream, the inside test of an axis-aligned
@@ -122,6 +125,41 @@ void orc_scatter(const float* mesh_xyzw, uint32_t n_vertices, const float* mesh_
         r[7] = m1 * m1;
         r[8] = 1.0f; r[13] = 1.0f; r[18] = 1.0f;       // F_e = I (columns at 8, 12, 16)
         r[20] = 1.0f; r[25] = 1.0f; r[30] = 1.0f;      // F_p = I (columns at 20, 24, 28)
+    }
+}
+
+// WGSL u32(f32): truncation toward zero clamped to [0, 2^32 - 1].
+static uint32_t u32_sat(float x) {
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xFFFFFFFFu;
+    return (uint32_t)x;
+}
+
+// kind 0: calculateDensity on a res grid; kind 1: calculateGridMass. aos: 48 floats per particle.
+void orc_splat(int kind, const float* aos, uint32_t n, const float* gmin, const float* gmax, float S,
+               const uint32_t* res, uint32_t* out) {
+    const uint64_t cells = (uint64_t)res[0] * res[1] * res[2];
+    std::memset(out, 0, cells * 4);
+    const F3 g0 = {gmin[0], gmin[1], gmin[2]}, g1 = {gmax[0], gmax[1], gmax[2]};
+    const F3 cell = cdiv(g1 - g0, F3{(float)res[0], (float)res[1], (float)res[2]});
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* r = aos + (size_t)i * 48;
+        const F3 pos = {r[0], r[1], r[2]};
+        const float mass = r[7];
+        if (!(pos.x >= g0.x) || !(pos.y >= g0.y) || !(pos.z >= g0.z)) continue;
+        if (pos.x >= g1.x || pos.y >= g1.y || pos.z >= g1.z) continue;
+        const F3 pc = cdiv(pos - g0, cell) - F3{0.5f, 0.5f, 0.5f};
+        int st[3];
+        if (kind == 0) { st[0] = (int)std::floor(pc.x); st[1] = (int)std::floor(pc.y); st[2] = (int)std::floor(pc.z); }
+        else { st[0] = (int)u32_sat(pc.x); st[1] = (int)u32_sat(pc.y); st[2] = (int)u32_sat(pc.z); }
+        const F3 f = pc - F3{(float)st[0], (float)st[1], (float)st[2]};
+        const float w[2][3] = {{1.0f - f.x, 1.0f - f.y, 1.0f - f.z}, {f.x, f.y, f.z}};
+        for (int z = 0; z < 2; ++z) for (int y = 0; y < 2; ++y) for (int x = 0; x < 2; ++x) {
+            const int cx = st[0] + x, cy = st[1] + y, cz = st[2] + z;
+            if (cx < 0 || cy < 0 || cz < 0 || cx >= (int)res[0] || cy >= (int)res[1] || cz >= (int)res[2]) continue;
+            const float weight = (w[x][0] * w[y][1]) * w[z][2];
+            out[(uint64_t)cx + res[0] * ((uint64_t)cy + res[1] * (uint64_t)cz)] += u32_sat((mass * weight) * S);
+        }
     }
 }
 

@@ -111,3 +111,38 @@ def test_scatter_rejection_keeps_candidates_inside_an_octahedron():
     # the octahedron fills 1/6 of its box, so most first candidates are rejected
     moved = sum(1 for i in range(200) if np.abs(_first_candidate(i, mn, mx).astype(np.float64) - 0.5).sum() >= 0.3)
     assert moved > 140
+
+
+# ------------------------------------------------------------------ render-side splats
+def _recs(pos, mass):
+    a = np.zeros((len(pos), 48), np.float32)
+    a[:, :3] = pos
+    a[:, 7] = mass
+    return a
+
+
+def test_splat_density_cell_centre_and_conservation():
+    gmin, gmax, res, S = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (16, 8, 4), 65536.0
+    # a particle at the centre of cell (5, 3, 1): all of its mass on that node
+    p = np.array([[(5 + 0.5) / 16, (3 + 0.5) / 8, (1 + 0.5) / 4]], np.float32)
+    g = orc.splat(0, _recs(p, [0.75]), gmin, gmax, S, res)
+    assert g[1, 3, 5] == 49152 and g.sum() == 49152
+    # interior particles: the node sum loses less than one unit per node to truncation
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(0.2, 0.8, (5000, 3)).astype(np.float32)
+    m = rng.uniform(0.1, 1.0, 5000).astype(np.float32)
+    g = orc.splat(0, _recs(pos, m), gmin, gmax, S, res)
+    tot = float((m.astype(np.float64) * S).sum())
+    assert tot - 8 * 5000 <= float(g.sum()) <= tot
+
+
+def test_splat_mass_lower_boundary_quirk():
+    """calculateGridMass truncates the cell-centred coordinate with u32(): at pos = gridMin the
+    coordinate is -0.5, the start cell 0 and the weights (1.5, -0.5) per axis; products with one or
+    three negative factors clamp to 0."""
+    gmin, gmax, res, S = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (8, 8, 8), 65536.0
+    g = orc.splat(1, _recs(np.zeros((1, 3), np.float32), [1.0]), gmin, gmax, S, res)
+    assert g[0, 0, 0] == int(np.float32(3.375) * S)
+    for (z, y, x) in ((0, 1, 1), (1, 0, 1), (1, 1, 0)):
+        assert g[z, y, x] == int(np.float32(0.375) * S)
+    assert g.sum() == int(np.float32(3.375) * S) + 3 * int(np.float32(0.375) * S)

@@ -120,6 +120,8 @@ _SIGS = [
     ("mpm_set_residual_tracking", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_get_residuals", ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     ("mpm_scatter_particles", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    ("mpm_splat", ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),
+    ("mpm_splat_grid", _P, [_P]),
     ("mpm_ensemble_create", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(_P)]),
     ("mpm_ensemble_upload", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t]),
     ("mpm_ensemble_download", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t]),

@@ -19,6 +19,7 @@
 #include "mpm_mlp.h"
 #include "mpm_ensemble.h"
 #include "mpm_scatter.h"
+#include "mpm_splat.h"
 #include "../../include/mpm_hip.h"
 
 using namespace mpm;
@@ -101,6 +102,9 @@ struct mpm_ctx {
     bool ens_graph_ok[2] = {false, false};
     float* snap_planes = nullptr;
     uint32_t* snap_pid = nullptr;
+    // render-side splat grid
+    uint32_t* splat = nullptr;
+    size_t splat_cells = 0;
 };
 
 namespace {
@@ -765,6 +769,42 @@ int mpm_scatter_particles(mpm_ctx* c, const float* mesh_xyzw, size_t n_vertices)
     c->has_particles = true;
     return MPM_OK;
 }
+
+int mpm_splat(mpm_ctx* c, int kind, const uint32_t* res_in, uint32_t* host_out, size_t cap) {
+    if (!c) return MPM_E_INVALID;
+    if (c->ens) return fail(c, MPM_E_INVALID, "batched context: splat a single-scene context");
+    if (kind != MPM_SPLAT_DENSITY && kind != MPM_SPLAT_MASS) return fail(c, MPM_E_INVALID, "unknown splat kind");
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
+    uint32_t res[3];
+    for (int a = 0; a < 3; ++a) res[a] = kind == MPM_SPLAT_MASS ? rdu<uint32_t>(c->uni, 208 + 4 * a) : (res_in ? res_in[a] : 0u);
+    const uint64_t cells = (uint64_t)res[0] * res[1] * res[2];
+    if (cells == 0 || cells > (1ull << 32)) return fail(c, MPM_E_RANGE, "splat grid resolution must be positive and hold at most 2^32 cells");
+    if (host_out && cap < cells) return fail(c, MPM_E_RANGE, "splat output capacity too small");
+    HIPCHK(c, hipSetDevice(c->device));
+    if (cells > c->splat_cells) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->splat) { hipFree(c->splat); c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->splat)); c->splat = nullptr; }
+        int rc;
+        if ((rc = dalloc(c, &c->splat, (size_t)cells))) return rc;
+        c->splat_cells = cells;
+    }
+    const float S = rdu<float>(c->uni, 4);
+    float g0[3], g1[3];
+    for (int a = 0; a < 3; ++a) { g0[a] = rdu<float>(c->uni, 16 + 4 * a); g1[a] = rdu<float>(c->uni, 32 + 4 * a); }
+    HIPCHK(c, hipMemsetAsync(c->splat, 0, cells * 4, c->stream));
+    const uint32_t G = grid_for(c->n, 1u << 30);
+    if (kind == MPM_SPLAT_DENSITY)
+        k_splat<SPLAT_DENSITY><<<G, 256, 0, c->stream>>>(c->planes[c->cur], c->n, c->stride, g0[0], g0[1], g0[2], g1[0], g1[1], g1[2], S, res[0], res[1], res[2], c->splat);
+    else
+        k_splat<SPLAT_MASS><<<G, 256, 0, c->stream>>>(c->planes[c->cur], c->n, c->stride, g0[0], g0[1], g0[2], g1[0], g1[1], g1[2], S, res[0], res[1], res[2], c->splat);
+    HIPCHK(c, hipGetLastError());
+    if (host_out) {
+        HIPCHK(c, hipMemcpyAsync(host_out, c->splat, cells * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return MPM_OK;
+}
+void* mpm_splat_grid(mpm_ctx* c) { return c ? (void*)c->splat : nullptr; }
 
 int mpm_download_particles(mpm_ctx* c, void* aos, size_t n) {
     if (!c || !aos) return MPM_E_INVALID;

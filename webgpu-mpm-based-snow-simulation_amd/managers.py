@@ -245,6 +245,21 @@ class MpmSimulation:
         mv = np.ascontiguousarray(self.meshManager.meshVertices)
         check(self.lib, self.ctx, self.lib.mpm_scatter_particles(self.ctx, _ptr(mv), mv.shape[0]))
 
+    # ---- render-side splats (mcDensity.cs.wgsl, calculateGridMass.wgsl)
+    def splat_density(self, res):
+        """Marching-cubes density grid, shape (rz, ry, rx) of u32 fixed-point mass."""
+        r = np.asarray(res, np.uint32)
+        out = np.zeros(int(r[0]) * int(r[1]) * int(r[2]), np.uint32)
+        check(self.lib, self.ctx, self.lib.mpm_splat(self.ctx, 0, _ptr(r), _ptr(out), out.size))
+        return out.reshape(int(r[2]), int(r[1]), int(r[0]))
+
+    def splat_mass(self):
+        """Volumetric mass grid on the simulation grid resolution, shape (rz, ry, rx)."""
+        r = self.uniformsManager.read(208, 12).view(np.uint32)
+        out = np.zeros(int(r[0]) * int(r[1]) * int(r[2]), np.uint32)
+        check(self.lib, self.ctx, self.lib.mpm_splat(self.ctx, 1, None, _ptr(out), out.size))
+        return out.reshape(int(r[2]), int(r[1]), int(r[0]))
+
     # ---- particles
     def upload(self, particles):
         p = np.ascontiguousarray(particles)
