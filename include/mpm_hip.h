@@ -137,6 +137,29 @@ int  mpm_download_particles(mpm_ctx* ctx, void* aos192, size_t n);
  * meshMinCoords / meshMaxCoords (@176 / @192, GpuUniformsBufferManager.ts:86-92). */
 int  mpm_scatter_particles(mpm_ctx* ctx, const float* mesh_xyzw, size_t n_vertices);
 
+/* loadGltfScene (src/lib/components/simulationViewer/loadScene.ts:31-251) in C++ for a glTF binary
+ * (.glb): world-space positions with the y/z swap, normals, uvs, per-vertex texture indices,
+ * offset indices, one collider object per mesh and the scatter mesh (the vertex of every index).
+ * On entry the n_* fields are capacities (a NULL array is not written); on return they are the
+ * counts. MPM_E_RANGE when a non-NULL array is too small (counts still set); MPM_E_INVALID with a
+ * message in err for unsupported or malformed files. Context-free. */
+typedef struct mpm_scene_arrays {
+    float*    positions;         /* 3 per vertex */
+    float*    uvs;               /* 2 per vertex */
+    uint32_t* material_indices;  /* 1 per vertex */
+    size_t    n_vertices;
+    float*    normals;           /* 3 per normal; meshes without normals add none (as the reference) */
+    size_t    n_normals;
+    uint32_t* indices;
+    size_t    n_indices;
+    uint32_t* objects;           /* 8 words per mesh: min xyz (f32 bits), first index, max xyz, index count
+                                    (the uniforms' ColliderObject layout, uniforms.wgsl:1-6) */
+    size_t    n_objects;
+    float*    soup_xyzw;         /* 4 per index: the scatter mesh (GpuMeshBufferManager.ts layout) */
+    size_t    n_soup;
+} mpm_scene_arrays;
+int  mpm_load_glb(const void* glb, size_t len, mpm_scene_arrays* out, char* err, size_t err_cap);
+
 /* Render-side particle splats: trilinear u32 fixed-point mass sums on a dense grid, cleared first.
  *  MPM_SPLAT_DENSITY  calculateDensity (marchingCubes/mcDensity.cs.wgsl:14-63) on a res[0..2] grid
  *                     (the reference sizes it floor(grid resolution / 2), GpuMarchingCubesBufferManager.ts:44-48)

@@ -4,6 +4,6 @@ src/lib/gpu/mpm/, src/lib/gpu/uniforms/, src/lib/gpu/collider/)."""
 from ._lib import (load_library, default_config, MpmError, MpmConfig, EXPORTED, LIB_PATH,
                    METHOD_EXPLICIT, METHOD_PBMPM, UNIFORMS_BYTES, PARTICLE_BYTES)
 from .particles import PARTICLE_DTYPE, make_particles, state_columns, matrix
-from .managers import (GpuUniformsBufferManager, GpuMpmBufferManager, GpuColliderBufferManager, GpuMeshBufferManager, triangle_soup,
+from .managers import (GpuUniformsBufferManager, GpuMpmBufferManager, GpuColliderBufferManager, GpuMeshBufferManager, triangle_soup, load_glb,
                        GpuMpmPipelineManager, MpmSimulation, MpmEnsemble, build_bvh, mlp_grad)
 from . import scenes

@@ -80,6 +80,17 @@ class MpmTimings(ctypes.Structure):
     ]
 
 
+class MpmSceneArrays(ctypes.Structure):
+    _fields_ = [
+        ("positions", ctypes.c_void_p), ("uvs", ctypes.c_void_p), ("material_indices", ctypes.c_void_p),
+        ("n_vertices", ctypes.c_size_t),
+        ("normals", ctypes.c_void_p), ("n_normals", ctypes.c_size_t),
+        ("indices", ctypes.c_void_p), ("n_indices", ctypes.c_size_t),
+        ("objects", ctypes.c_void_p), ("n_objects", ctypes.c_size_t),
+        ("soup_xyzw", ctypes.c_void_p), ("n_soup", ctypes.c_size_t),
+    ]
+
+
 class MpmTuning(ctypes.Structure):
     _fields_ = [
         ("chunk_particles", ctypes.c_uint32),
@@ -120,6 +131,7 @@ _SIGS = [
     ("mpm_set_residual_tracking", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_get_residuals", ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     ("mpm_scatter_particles", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+    ("mpm_load_glb", ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(MpmSceneArrays), ctypes.c_char_p, ctypes.c_size_t]),
     ("mpm_splat", ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),
     ("mpm_splat_grid", _P, [_P]),
     ("mpm_ensemble_create", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(_P)]),

@@ -20,6 +20,7 @@
 #include "mpm_ensemble.h"
 #include "mpm_scatter.h"
 #include "mpm_splat.h"
+#include "mpm_gltf.h"
 #include "../../include/mpm_hip.h"
 
 using namespace mpm;
@@ -805,6 +806,33 @@ int mpm_splat(mpm_ctx* c, int kind, const uint32_t* res_in, uint32_t* host_out, 
     return MPM_OK;
 }
 void* mpm_splat_grid(mpm_ctx* c) { return c ? (void*)c->splat : nullptr; }
+
+int mpm_load_glb(const void* glb, size_t len, mpm_scene_arrays* a, char* err, size_t err_cap) {
+    auto seterr = [&](const std::string& m) { if (err && err_cap) { std::strncpy(err, m.c_str(), err_cap - 1); err[err_cap - 1] = 0; } };
+    if (!glb || !a) { seterr("null argument"); return MPM_E_INVALID; }
+    mpm_gltf::Scene sc;
+    mpm_gltf::Loader ld;
+    if (!ld.load((const uint8_t*)glb, len, sc)) { seterr(ld.err); return MPM_E_INVALID; }
+    const size_t nv = sc.positions.size() / 3, nn = sc.normals.size() / 3, ni = sc.indices.size(), no = sc.objects.size(), ns = sc.soup.size() / 4;
+    const bool room = (!a->positions || a->n_vertices >= nv) && (!a->uvs || a->n_vertices >= nv) &&
+                      (!a->material_indices || a->n_vertices >= nv) && (!a->normals || a->n_normals >= nn) &&
+                      (!a->indices || a->n_indices >= ni) && (!a->objects || a->n_objects >= no) && (!a->soup_xyzw || a->n_soup >= ns);
+    a->n_vertices = nv; a->n_normals = nn; a->n_indices = ni; a->n_objects = no; a->n_soup = ns;
+    if (!room) { seterr("output capacity too small"); return MPM_E_RANGE; }
+    if (a->positions) std::memcpy(a->positions, sc.positions.data(), nv * 12);
+    if (a->normals) std::memcpy(a->normals, sc.normals.data(), nn * 12);
+    if (a->uvs) std::memcpy(a->uvs, sc.uvs.data(), nv * 8);
+    if (a->material_indices) std::memcpy(a->material_indices, sc.material_indices.data(), nv * 4);
+    if (a->indices) std::memcpy(a->indices, sc.indices.data(), ni * 4);
+    if (a->soup_xyzw) std::memcpy(a->soup_xyzw, sc.soup.data(), ns * 16);
+    if (a->objects)
+        for (size_t o = 0; o < no; ++o) {
+            uint32_t* w = a->objects + 8 * o;
+            std::memcpy(w, sc.objects[o].mn, 12); w[3] = sc.objects[o].start;
+            std::memcpy(w + 4, sc.objects[o].mx, 12); w[7] = sc.objects[o].count;
+        }
+    return MPM_OK;
+}
 
 int mpm_download_particles(mpm_ctx* c, void* aos, size_t n) {
     if (!c || !aos) return MPM_E_INVALID;

@@ -143,6 +143,34 @@ class GpuMeshBufferManager:
         self.meshVertices[:, :3] = v
 
 
+def load_glb(data):
+    """loadGltfScene (loadScene.ts:31-251) for a .glb file's bytes, in the library's C++ reader:
+    dict(positions (V,3), normals (Nn,3), uvs (V,2), materialIndices (V,), indices (I,),
+    objects [dict(min, max, startIndex, countIndices)], vertices (I,3) = the scatter mesh)."""
+    lib = _lib.load_library()
+    buf = np.frombuffer(bytes(data), np.uint8)
+    a = _lib.MpmSceneArrays()
+    err = ctypes.create_string_buffer(512)
+    rc = lib.mpm_load_glb(_ptr(buf), buf.size, ctypes.byref(a), err, 512)
+    if rc != 0:
+        raise _lib.MpmError(rc, err.value.decode())
+    pos = np.zeros((a.n_vertices, 3), np.float32)
+    uv = np.zeros((a.n_vertices, 2), np.float32)
+    mat = np.zeros(a.n_vertices, np.uint32)
+    nor = np.zeros((a.n_normals, 3), np.float32)
+    idx = np.zeros(a.n_indices, np.uint32)
+    obj = np.zeros((a.n_objects, 8), np.uint32)
+    soup = np.zeros((a.n_soup, 4), np.float32)
+    b = _lib.MpmSceneArrays(_ptr(pos), _ptr(uv), _ptr(mat), a.n_vertices, _ptr(nor), a.n_normals, _ptr(idx), a.n_indices,
+                            _ptr(obj), a.n_objects, _ptr(soup), a.n_soup)
+    rc = lib.mpm_load_glb(_ptr(buf), buf.size, ctypes.byref(b), err, 512)
+    if rc != 0:
+        raise _lib.MpmError(rc, err.value.decode())
+    objects = [dict(min=obj[o, 0:3].view(np.float32).tolist(), startIndex=int(obj[o, 3]),
+                    max=obj[o, 4:7].view(np.float32).tolist(), countIndices=int(obj[o, 7])) for o in range(a.n_objects)]
+    return dict(positions=pos, normals=nor, uvs=uv, materialIndices=mat, indices=idx, objects=objects, vertices=soup[:, :3].copy())
+
+
 def triangle_soup(vertices, indices):
     """Indexed mesh -> consecutive triangle vertices (the scatter mesh layout)."""
     v = np.asarray(vertices, np.float64).reshape(-1, 3)
