@@ -130,6 +130,13 @@ int  mpm_build_bvh(const float* positions, size_t n_vertices, const uint32_t* in
 int  mpm_upload_particles(mpm_ctx* ctx, const void* aos192, size_t n);
 int  mpm_download_particles(mpm_ctx* ctx, void* aos192, size_t n);
 
+/* Per-particle Young's modulus for the explicit method (no reference counterpart: the reference
+ * uses one E, stressTensorOps.wgsl:1-6). E is given by particle (upload order); the Lame
+ * parameters are formed per particle exactly as the scene-wide ones (double precision, then FP32)
+ * and travel with the particle. E = NULL returns to the scene-wide constants. Call after the
+ * particles are uploaded or scattered. */
+int  mpm_set_particle_youngs(mpm_ctx* ctx, const float* E, size_t n);
+
 /* scatterParticlesInMeshVolume (GpuSnowPipelineRunner.svelte.ts:368-381) with the scatterParticles
  * pass (particleScatter.cs.wgsl:40-84): fills every particle record of the context by rejection
  * sampling inside a closed triangle mesh. mesh_xyzw holds consecutive triangle vertices, 16 bytes

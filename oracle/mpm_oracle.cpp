@@ -128,6 +128,8 @@ template <class R> struct Particle {
     M3<R> Fe, Fp;
     V3<R> dx;
     M3<R> D;
+    bool own_lame = false;  // per-particle Lame parameters (explicit stress) instead of the constants
+    R mu0 = 0, la0 = 0;
 };
 
 // Grid accumulator policies.
@@ -307,6 +309,7 @@ template <class R, class AP> struct Sim {
         V3<R> w[3], d[3]; weights(f, w); weight_derivs(f, d);
         R mu = (R)(float)(P.youngs_modulus / (2.0 * (1.0 + P.poissons_ratio)));
         R la = (R)(float)(P.youngs_modulus * P.poissons_ratio / ((1.0 + P.poissons_ratio) * (1.0 - 2.0 * P.poissons_ratio)));
+        if (pt.own_lame) { mu = pt.mu0; la = pt.la0; }
         // stressTensorOps.wgsl:39-56
         R Jp = det3(pt.Fp);
         R ef = r_exp((R)P.hardening_explicit * ((R)1 - Jp));
@@ -826,6 +829,22 @@ void orc_run_stage(void* hp, int method, int stage) {
 void orc_get_particles(void* hp, float* aos192) { ((Handle*)hp)->visit([&](auto* s) { store_aos(s->parts, aos192); return 0; }); }
 void orc_get_particles_f64(void* hp, double* out37) { ((Handle*)hp)->visit([&](auto* s) { store_flat(s->parts, out37); return 0; }); }
 void orc_set_particles(void* hp, const float* aos192, uint32_t n) { ((Handle*)hp)->visit([&](auto* s) { load_aos(s->parts, aos192, n); return 0; }); }
+// Per-particle Young's modulus (explicit stress): Lame parameters in double, rounded to FP32,
+// the same forms as the scene-wide constants. E == NULL clears them.
+void orc_set_particle_youngs(void* hp, const float* E, uint32_t n) {
+    ((Handle*)hp)->visit([&](auto* s) {
+        const double nu = s->P.poissons_ratio;
+        for (uint32_t i = 0; i < n && i < s->parts.size(); ++i) {
+            auto& p = s->parts[i];
+            p.own_lame = E != nullptr;
+            if (E) {
+                p.mu0 = (decltype(p.mu0))(float)((double)E[i] / (2.0 * (1.0 + nu)));
+                p.la0 = (decltype(p.la0))(float)((double)E[i] * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+            }
+        }
+        return 0;
+    });
+}
 // Cell-keyed dump of every allocated cell: xyz[3*i], vals[4*i] = (mass, mx, my, mz) as raw
 // accumulator values (fixed-point integers, or unquantized sums in mode 2).
 size_t orc_dump_grid(void* hp, int32_t* xyz, double* vals, size_t cap) {

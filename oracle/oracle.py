@@ -71,6 +71,7 @@ def lib():
             "orc_get_particles": (None, [_P, _P]),
             "orc_get_particles_f64": (None, [_P, _P]),
             "orc_set_particles": (None, [_P, _P, ctypes.c_uint32]),
+            "orc_set_particle_youngs": (None, [_P, _P, ctypes.c_uint32]),
             "orc_dump_grid": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_size_t]),
             "orc_active_blocks": (ctypes.c_size_t, [_P, _P, _P, ctypes.c_size_t]),
             "orc_get_stats": (None, [_P, ctypes.POINTER(OrcStats)]),
@@ -144,6 +145,13 @@ class OracleSim:
             self.close()
         except Exception:
             pass
+
+    def set_particle_youngs(self, E):
+        if E is None:
+            lib().orc_set_particle_youngs(self.h, None, 0)
+            return
+        e = np.ascontiguousarray(E, np.float32)
+        lib().orc_set_particle_youngs(self.h, _ptr(e), e.size)
 
     def set_uniforms(self, uniforms):
         self._uni = np.ascontiguousarray(uniforms, np.uint8)

@@ -130,6 +130,7 @@ _SIGS = [
     ("mpm_mlp_grad", ctypes.c_int, [_P, _P, _P, _P, ctypes.c_uint32, _P, _P, _P]),
     ("mpm_set_residual_tracking", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_get_residuals", ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    ("mpm_set_particle_youngs", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     ("mpm_scatter_particles", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
     ("mpm_load_glb", ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(MpmSceneArrays), ctypes.c_char_p, ctypes.c_size_t]),
     ("mpm_splat", ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),

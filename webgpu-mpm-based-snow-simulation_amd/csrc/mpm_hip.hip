@@ -79,6 +79,7 @@ struct mpm_ctx {
     bool resid_on = false;
     uint64_t resid_substeps = 0;
     std::map<uint64_t, bool> div_verified;   // (divisor, scale) bits -> fixed_div exact for all inputs
+    bool per_particle_lame = false;
     uint32_t* d_scratch = nullptr;           // one counter for k_verify_div
     // per-kernel event timing (eager launches, one event after every kernel)
     bool ktiming = false;
@@ -286,7 +287,9 @@ int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, f
 }
 int make_params(mpm_ctx* c, SimParams& P) {
     if (c->ens) return fail(c, MPM_E_STATE, "this entry point is for single-scene contexts; use mpm_ensemble_*");
-    return params_from(c, c->uni, c->cfg, c->k_cur, c->force_div_cur, false, P);
+    const int rc = params_from(c, c->uni, c->cfg, c->k_cur, c->force_div_cur, false, P);
+    P.per_particle_lame = c->per_particle_lame ? 1u : 0u;
+    return rc;
 }
 
 uint32_t hash_host(uint32_t n) {
@@ -831,6 +834,37 @@ int mpm_load_glb(const void* glb, size_t len, mpm_scene_arrays* a, char* err, si
             std::memcpy(w, sc.objects[o].mn, 12); w[3] = sc.objects[o].start;
             std::memcpy(w + 4, sc.objects[o].mx, 12); w[7] = sc.objects[o].count;
         }
+    return MPM_OK;
+}
+
+int mpm_set_particle_youngs(mpm_ctx* c, const float* E, size_t n) {
+    if (!c) return MPM_E_INVALID;
+    if (c->ens) return fail(c, MPM_E_INVALID, "batched context: per-particle moduli are for single-scene contexts");
+    if (!E) { c->per_particle_lame = false; return MPM_OK; }
+    if (n != c->n) return fail(c, MPM_E_INVALID, "one modulus per particle");
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "upload the particles first");
+    const double nu = c->cfg.poissons_ratio;
+    std::vector<float> mu(n), la(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (!(E[i] > 0.0f) || !std::isfinite(E[i])) return fail(c, MPM_E_RANGE, "moduli must be positive and finite");
+        // the same double-precision forms as the scene-wide constants
+        mu[i] = (float)((double)E[i] / (2.0 * (1.0 + nu)));
+        la[i] = (float)((double)E[i] * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float* tmp = nullptr;
+    HIPCHK(c, hipMalloc(&tmp, 2 * n * sizeof(float)));
+    hipError_t e = hipMemcpy(tmp, mu.data(), n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(tmp + n, la.data(), n * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        k_set_lame<<<grid_for(n, 1u << 30), 256, 0, c->stream>>>(tmp, tmp + n, c->planes[c->cur], c->pid[c->cur], c->n, c->stride);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    hipFree(tmp);
+    if (e != hipSuccess) return fail(c, MPM_E_HIP, std::string("per-particle moduli: ") + hipGetErrorString(e));
+    c->per_particle_lame = true;
     return MPM_OK;
 }
 

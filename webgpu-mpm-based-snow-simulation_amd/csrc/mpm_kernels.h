@@ -31,7 +31,9 @@ enum : uint32_t { EMPTY = 0xFFFFFFFFu };
 
 enum Plane : int {
     PX = 0, PY, PZ, PM, VX, VY, VZ, DXX, DXY, DXZ,
-    FE = 10, FP = 19, DD = 28, NPLANES = 37
+    FE = 10, FP = 19, DD = 28,
+    MU0 = 37, LA0 = 38,        // optional per-particle Lame parameters (explicit method)
+    NPLANES = 39
 };
 
 enum Counter : int {
@@ -54,6 +56,7 @@ struct SimParams {
     // material / method
     float mu0, la0, xi_e, xi_p, blend, one_minus_blend, vol_exp, plast_lo, plast_hi, inv_density, grad_scale;
     float force_div, grav[3];
+    uint32_t per_particle_lame; // explicit stress uses planes MU0 / LA0 instead of mu0 / la0
     float grav_div[3];         // grav / force_div (the per-node force of pbmpm.wgsl:137 without a pointer)
     float rdt;                 // RN(1/dt) for fixed_div
     uint32_t fast_dt;          // fixed_div(., dt, rdt, S) verified equal to i32((x/dt) S) for all x
@@ -250,6 +253,16 @@ __global__ void k_download(const float* __restrict__ s, const uint32_t* __restri
     q[32] = ld(s, stride, DXX, i); q[33] = ld(s, stride, DXY, i); q[34] = ld(s, stride, DXZ, i); q[35] = 0.0f;
 }
 
+// Per-particle Lame parameters given by original particle id, written to the rows holding them.
+__global__ void k_set_lame(const float* __restrict__ mu_by_id, const float* __restrict__ la_by_id, float* __restrict__ s,
+                           const uint32_t* __restrict__ pid, uint32_t n, uint32_t stride) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = pid[i];
+    if (id >= n) return;
+    st(s, stride, MU0, i, mu_by_id[id]);
+    st(s, stride, LA0, i, la_by_id[id]);
+}
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1015,6 +1028,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 B.pid_nxt[q] = B.pid_cur[old];
                 st3(B.nxt, sd, PX, q, pos); st(B.nxt, sd, PM, q, mass);
                 stm(B.nxt, sd, FE, q, Fe); stm(B.nxt, sd, FP, q, Fp);
+                if (P.per_particle_lame) { st(B.nxt, sd, MU0, q, ld(B.cur, sd, MU0, old)); st(B.nxt, sd, LA0, q, ld(B.cur, sd, LA0, old)); }
                 const float relax = pbmpm_relax(P, Fp);
                 B.relax[q] = relax;
                 if (ok) {
@@ -1107,11 +1121,15 @@ MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             B.pid_nxt[q] = B.pid_cur[old];
             st3(B.nxt, sd, PX, q, pos); st(B.nxt, sd, PM, q, mass);
             stm(B.nxt, sd, FE, q, Fe); stm(B.nxt, sd, FP, q, Fp);
+            float mu = P.mu0, la = P.la0;
+            if (P.per_particle_lame) {
+                mu = ld(B.cur, sd, MU0, old); la = ld(B.cur, sd, LA0, old);
+                st(B.nxt, sd, MU0, q, mu); st(B.nxt, sd, LA0, q, la);
+            }
             if (!ok) {
                 st3(B.nxt, sd, VX, q, vel); st3(B.nxt, sd, DXX, q, ld3(B.cur, sd, DXX, old)); stm(B.nxt, sd, DD, q, ldm(B.cur, sd, DD, old));
                 continue;
             }
-            float mu = P.mu0, la = P.la0;
             const float ef = fexp(P.xi_e * (1.0f - det3(Fp)));
             mu = mu * ef; la = la * ef;
             const float J = det3(Fe);
@@ -1253,6 +1271,7 @@ MPM_DEV void straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_
         p.Fe = ldm(B.cur, sd, FE, old); p.Fp = ldm(B.cur, sd, FP, old); p.D = ldm(B.cur, sd, DD, old);
         p.dx = ld3(B.cur, sd, DXX, old);
         st(B.nxt, sd, PM, q, p.mass);
+        if (P.per_particle_lame) { st(B.nxt, sd, MU0, q, ld(B.cur, sd, MU0, old)); st(B.nxt, sd, LA0, q, ld(B.cur, sd, LA0, old)); }
         integrate(P, B, p);
         st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
         stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);

@@ -261,6 +261,14 @@ class MpmSimulation:
     def updateColliderVel(self, v):
         self.uniformsManager.writeColliderVel(v)
 
+    def set_particle_youngs(self, E):
+        """Per-particle Young's modulus for the explicit method (None: back to the constant)."""
+        if E is None:
+            check(self.lib, self.ctx, self.lib.mpm_set_particle_youngs(self.ctx, None, 0))
+            return
+        e = np.ascontiguousarray(E, np.float32)
+        check(self.lib, self.ctx, self.lib.mpm_set_particle_youngs(self.ctx, _ptr(e), e.size))
+
     # ---- particle init (GpuSnowPipelineRunner.svelte.ts:158-160, 368-381)
     def set_mesh(self, vertices):
         """The scatter mesh: consecutive triangle vertices (see triangle_soup)."""

@@ -5,8 +5,9 @@
   pbmpm_two_spheres  1,048,576 particles in two r = 0.15 spheres at (0.3,0.5,0.5) and
                   (0.7,0.5,0.5) moving +/-2 m/s along x, 128^3 grid, k = 4, dt = 1/600,
                   12-triangle box collider of side 0.2 at the domain centre.
-  explicit_slab_terrain  slab 0.6 x 0.1 x 0.6 at y = 0.7 over a 64 x 64-vertex heightfield
-                  (7,938 triangles), 256^3 grid, dt = 5e-5.
+  explicit_slab_terrain  4,194,304 particles in a slab 0.6 x 0.1 x 0.6 at y = 0.7 over a
+                  64 x 64-vertex heightfield (7,938 triangles), 256^3 grid, dt = 5e-5, Young's modulus
+                  log-uniform in [5e4, 2e5] per particle.
   ensemble_scenes  64 independent scenes for the batched step: N log-uniform in [8,192, 524,288],
                   grid by count (32^3 / 64^3 / 128^3), half explicit, half PBMPM with k in 2..6,
                   sphere or cube emitters with velocity U[-3,3]^3.
@@ -101,9 +102,11 @@ def scene(name, n_particles=None, seed=None):
         lo = np.array([0.2, 0.65, 0.2])
         pos = (lo + rng.random((n, 3)) * np.array([0.6, 0.1, 0.6])).astype(np.float32)
         p = make_particles(pos)
+        youngs = np.exp(rng.uniform(np.log(5e4), np.log(2e5), n)).astype(np.float32)
         sim = dict(grid_res=(res, res, res), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=5e-5,
                    gravity=(0.0, -9.8, 0.0))
-        return dict(particles=p, sim=sim, collider=heightfield_mesh(64, 64, 1.0, 0.05, 0.25, rng), method=0, substeps=2000)
+        return dict(particles=p, sim=sim, collider=heightfield_mesh(64, 64, 1.0, 0.05, 0.25, rng), method=0, substeps=2000,
+                    youngs=youngs)
     raise KeyError(name)
 
 
