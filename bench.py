@@ -174,6 +174,32 @@ def ensemble_bench(mpm, steps=30, warmup=5):
             "kernel_ms_per_substep": {k: round(v[0] / 5, 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0])}}
 
 
+def terrain_bench(mpm, steps=100, warmup=10):
+    """The 4,194,304-particle explicit slab over the 7,938-triangle terrain (256^3 grid,
+    per-particle Young's modulus): particle-substeps/s and per-kernel device time."""
+    sc = mpm.scenes.scene("explicit_slab_terrain")
+    n = sc["particles"].shape[0]
+    with mpm.MpmSimulation(n, **sc["sim"]) as sim:
+        sim.set_collider(*sc["collider"])
+        sim.upload(sc["particles"])
+        sim.set_particle_youngs(sc["youngs"])
+        sim.step(0, warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(0, steps)
+        sim.sync()
+        dt = time.perf_counter() - t0
+        sim.set_kernel_timing(True)
+        sim.step(0, 10)
+        sim.sync()
+        kt = sim.kernel_times()
+        st = sim.stats()
+    return {"workload": "explicit_slab_terrain", "particles": n, "grid": "256^3 hashed", "collider_triangles": len(sc["collider"][1]) // 3,
+            "value": n * steps / dt, "unit": "particle-substeps/s", "ms_per_step": 1e3 * dt / steps,
+            "blocks_allocated": st["blocks_allocated"],
+            "kernel_ms_per_substep": {k: round(v[0] / 10, 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0])}}
+
+
 def copy_calibration(mpm, nbytes):
     """One plain device copy kernel of nbytes (k_selftest kind 10): what FETCH_SIZE / WRITE_SIZE
     report for it under rocprofv3 --pmc calibrates the counters against a copy of known size."""
@@ -207,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 64-scene workload")
+    ap.add_argument("--no-terrain", action="store_true", help="skip the 4M-particle explicit terrain workload")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
@@ -278,6 +305,9 @@ def main():
     }
     if args.calibrate_copy_mb:
         copy_calibration(mpm, args.calibrate_copy_mb << 20)
+    if not args.no_terrain:
+        print("bench: 4M explicit terrain workload", file=sys.stderr, flush=True)
+        line["explicit_terrain"] = terrain_bench(mpm)
     if not args.no_ensemble:
         print("bench: batched 64-scene workload", file=sys.stderr, flush=True)
         line["ensemble"] = ensemble_bench(mpm)

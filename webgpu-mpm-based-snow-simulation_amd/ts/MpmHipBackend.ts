@@ -19,6 +19,9 @@ const SYMBOLS = {
   mpm_set_timing: { parameters: ["pointer", "i32"], result: "i32" },
   mpm_get_timings: { parameters: ["pointer", "buffer"], result: "i32" },
   mpm_last_error: { parameters: ["pointer"], result: "pointer" },
+  mpm_scatter_particles: { parameters: ["pointer", "buffer", "usize"], result: "i32" },
+  mpm_splat: { parameters: ["pointer", "i32", "buffer", "buffer", "usize"], result: "i32" },
+  mpm_load_glb: { parameters: ["buffer", "usize", "buffer", "buffer", "usize"], result: "i32" },
 } as const;
 
 // mpm_config layout (include/mpm_hip.h), 184 bytes, natural alignment.
@@ -80,6 +83,8 @@ export class MpmHipBackend {
   writeGridMinCoords(v: [number, number, number]) { this.writeBuffer(16, new Float32Array(v)); }
   writeGridMaxCoords(v: [number, number, number]) { this.writeBuffer(32, new Float32Array(v)); }
   writeGridResolution(v: [number, number, number]) { this.writeBuffer(208, new Uint32Array(v)); }
+  writeMeshMinCoords(v: [number, number, number]) { this.writeBuffer(176, new Float32Array(v)); }
+  writeMeshMaxCoords(v: [number, number, number]) { this.writeBuffer(192, new Float32Array(v)); }
   writeColliderMinCoords(v: [number, number, number]) { this.writeBuffer(224, new Float32Array(v)); }
   writeColliderMaxCoords(v: [number, number, number]) { this.writeBuffer(240, new Float32Array(v)); }
   writeColliderTransformMat(m: Float32Array) { this.writeBuffer(256, m); }
@@ -98,6 +103,22 @@ export class MpmHipBackend {
   setCollider(colliderData: Uint32Array, numIndices: number, bvhNodes: Uint32Array) {
     this.check(this.lib.symbols.mpm_set_collider(this.ctx, new Uint8Array(colliderData.buffer), BigInt(colliderData.length),
       numIndices, new Uint8Array(bvhNodes.buffer), BigInt(bvhNodes.length / 8)), this.ctx);
+  }
+
+  // scatterParticlesInMeshVolume (GpuSnowPipelineRunner.svelte.ts:368-381): the mesh is the
+  // GpuMeshBufferManager vertex list (consecutive triangles, 4 floats per vertex); its AABB must
+  // already be in the uniforms (writeMeshMinCoords / writeMeshMaxCoords).
+  scatterParticlesInMeshVolume(meshXyzw: Float32Array) {
+    this.check(this.lib.symbols.mpm_scatter_particles(this.ctx, new Uint8Array(meshXyzw.buffer), BigInt(meshXyzw.length / 4)), this.ctx);
+  }
+
+  // marching-cubes density (kind 0, res = the density grid) / volumetric mass grid (kind 1)
+  splat(kind: 0 | 1, res: [number, number, number]): Uint32Array {
+    const cells = res[0] * res[1] * res[2];
+    const out = new Uint32Array(cells);
+    this.check(this.lib.symbols.mpm_splat(this.ctx, kind, new Uint8Array(new Uint32Array(res).buffer),
+      new Uint8Array(out.buffer), BigInt(cells)), this.ctx);
+    return out;
   }
 
   // particleDataBuffer (192-byte ParticleData records)
