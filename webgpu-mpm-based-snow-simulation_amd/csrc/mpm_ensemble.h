@@ -173,7 +173,8 @@ __global__ __launch_bounds__(256) void k_ens_nbr(SimParams P0, Bufs B, Ens E) {
     for (uint32_t sc = 0; sc < E.S; ++sc) {
         const uint32_t nb = min(E.nblk[sc], E.sp[sc].blk_cap);
         int4* g = B.grid0 + (size_t)E.sp[sc].blk_base * 64u;
-        for (uint32_t c = gi; c < nb * 64u; c += gsz) g[c] = make_int4(0, 0, 0, 0);
+        int* gm = B.gmass + (size_t)E.sp[sc].blk_base * 64u;
+        for (uint32_t c = gi; c < nb * 64u; c += gsz) { g[c] = make_int4(0, 0, 0, 0); gm[c] = 0; }
     }
 }
 

@@ -44,6 +44,8 @@ struct mpm_ctx {
     uint2* chunks = nullptr;
     uint32_t *table = nullptr, *slot_val = nullptr, *nbr = nullptr, *block_key = nullptr;
     int4* grid[3] = {nullptr, nullptr, nullptr};
+    int* gmass = nullptr;
+    bool last_pbmpm = false;
     uint32_t* counters = nullptr;
     uint32_t* cqueue = nullptr;
     uint32_t* coll_data = nullptr;
@@ -185,6 +187,7 @@ Bufs make_bufs(mpm_ctx* c) {
     B.chunks = c->chunks;
     B.table = c->table; B.slot_val = c->slot_val; B.nbr = c->nbr; B.block_key = c->block_key;
     B.grid0 = c->grid[0]; B.grid1 = c->grid[1]; B.grid2 = c->grid[2];
+    B.gmass = c->gmass;
     B.counters = c->counters;
     B.cqueue = c->cqueue;
     B.coll_data = c->coll_data; B.bvh = c->bvh;
@@ -349,6 +352,7 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
         if (stage == MPM_STAGE_WARM_START) return false;
         k_pbmpm<PB_FIRST><<<G, 256, 0, s>>>(P, B, -1, 0, 1, -1); kt(c, KID_PB_FIRST);
         c->last_grid = 0;
+        c->last_pbmpm = true;
         if (stage == 1) return false;
         for (int i = 1; i < k; ++i) {
             k_pbmpm<PB_MID><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3, i - 1); kt(c, KID_PB_MID);
@@ -361,6 +365,7 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
     } else {
         k_explicit_p2g<0><<<G, 256, 0, s>>>(P, B); kt(c, KID_EX_P2G);
         c->last_grid = 0;
+        c->last_pbmpm = false;
         if (stage == 1) return false;
         k_grid_update<<<G, 256, 0, s>>>(P, B); kt(c, KID_EX_GRID);
         if (stage == 2) return false;
@@ -443,7 +448,7 @@ int alloc_buffers(mpm_ctx* c) {
         (rc = dalloc(c, &c->nbr, 27 * (size_t)c->max_runs)) ||
         (rc = dalloc(c, &c->block_key, c->max_blocks)) ||
         (rc = dalloc(c, &c->grid[0], 64 * (size_t)c->max_blocks)) || (rc = dalloc(c, &c->grid[1], 64 * (size_t)c->max_blocks)) ||
-        (rc = dalloc(c, &c->grid[2], 64 * (size_t)c->max_blocks)) ||
+        (rc = dalloc(c, &c->grid[2], 64 * (size_t)c->max_blocks)) || (rc = dalloc(c, &c->gmass, 64 * (size_t)c->max_blocks)) ||
         (rc = dalloc(c, &c->counters, C_NCOUNTERS)) || (rc = dalloc(c, &c->cqueue, c->stride)) || (rc = dalloc(c, &c->d_scratch, 4))) {
         return rc;
     }
@@ -966,6 +971,11 @@ int mpm_dump_grid(mpm_ctx* c, int32_t* xyz, int32_t* vals, size_t cap, size_t* n
     if (nb) {
         HIPCHK(c, hipMemcpy(keys.data(), c->block_key, nb * 4, hipMemcpyDeviceToHost));
         HIPCHK(c, hipMemcpy(g.data(), c->grid[c->last_grid], (size_t)nb * 64 * sizeof(int4), hipMemcpyDeviceToHost));
+        if (c->last_pbmpm) {   // PBMPM keeps the node mass in its own array (deposited once per substep)
+            std::vector<int> gm((size_t)nb * 64);
+            HIPCHK(c, hipMemcpy(gm.data(), c->gmass, (size_t)nb * 64 * sizeof(int), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < gm.size(); ++i) g[i].x = gm[i];
+        }
     }
     SimParams P;
     if ((rc = make_params(c, P))) return rc;

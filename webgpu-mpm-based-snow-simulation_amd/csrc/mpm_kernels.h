@@ -85,6 +85,7 @@ struct Bufs {
     uint2* chunks;
     uint32_t* table; uint32_t* slot_val; uint32_t* nbr; uint32_t* block_key;
     int4* grid0; int4* grid1; int4* grid2;
+    int* gmass;                // PBMPM: node mass, deposited once per substep (identical in every iteration)
     uint32_t* counters;
     uint32_t* cqueue;          // sorted indices of particles whose path reaches the collider's box
     float* resid_dx;           // optional: dx of the previous G2P per particle (3 planes, stride)
@@ -487,7 +488,7 @@ __global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) {
         B.nbr[t] = table_lookup(P, B, b[0] + (int)(j % 3u) - 1, b[1] + (int)((j / 3u) % 3u) - 1, b[2] + (int)(j / 9u) - 1);
     }
     const uint32_t ncell = n_blocks_of(P, B) * 64u;
-    for (uint32_t c = gi; c < ncell; c += gsz) B.grid0[c] = make_int4(0, 0, 0, 0);
+    for (uint32_t c = gi; c < ncell; c += gsz) { B.grid0[c] = make_int4(0, 0, 0, 0); B.gmass[c] = 0; }
 }
 
 // ================================================================== particle update helpers
@@ -785,13 +786,15 @@ MPM_DEV void tile_setup(const SimParams& P, const Bufs& B, Tile& T, uint2 ch) {
         T.vel[t] = make_float4(0.f, 0.f, 0.f, idx >= 0 ? 1.0f : 0.0f);
     }
 }
-// node velocity as gridToParticle.cs.wgsl:43-49 / :102-108 computes it
-MPM_DEV void tile_load_vel(const SimParams& P, Tile& T, const int4* __restrict__ g) {
+// node velocity as gridToParticle.cs.wgsl:43-49 / :102-108 computes it; the node mass comes from
+// the separate PBMPM mass array when gm is given, else from the grid's mass channel
+MPM_DEV void tile_load_vel(const SimParams& P, Tile& T, const int4* __restrict__ g, const int* __restrict__ gm = nullptr) {
     for (int t = threadIdx.x; t < 216; t += blockDim.x) {
         const int idx = T.node[t];
         float4 v = make_float4(0.f, 0.f, 0.f, idx >= 0 ? 1.0f : 0.0f);
         if (idx >= 0) {
-            const int4 q = g[idx];
+            int4 q = g[idx];
+            if (gm) q.x = gm[idx];
             const float m = (float)q.x / P.S;
             if (m > 0.0f) {
                 v.x = ((float)q.y / P.S) / m;
@@ -844,6 +847,20 @@ MPM_DEV void tile_flush(Tile& T, int4* __restrict__ g) {
         int* q = reinterpret_cast<int*>(g + idx);
         const int a0 = T.acc[0][t], a1 = T.acc[1][t], a2 = T.acc[2][t], a3 = T.acc[3][t];
         if (a0) atomicAdd(q, a0);
+        if (a1) atomicAdd(q + 1, a1);
+        if (a2) atomicAdd(q + 2, a2);
+        if (a3) atomicAdd(q + 3, a3);
+    }
+}
+// PBMPM flush: momentum into the grid buffer; mass (first pass only) into the mass array.
+template <bool MASS>
+MPM_DEV void tile_flush_pb(Tile& T, int4* __restrict__ g, int* __restrict__ gm) {
+    for (int t = threadIdx.x; t < 216; t += blockDim.x) {
+        const int idx = T.node[t];
+        if (idx < 0) continue;
+        int* q = reinterpret_cast<int*>(g + idx);
+        const int a1 = T.acc[1][t], a2 = T.acc[2][t], a3 = T.acc[3][t];
+        if (MASS) { const int a0 = T.acc[0][t]; if (a0) atomicAdd(gm + idx, a0); }
         if (a1) atomicAdd(q + 1, a1);
         if (a2) atomicAdd(q + 2, a2);
         if (a3) atomicAdd(q + 3, a3);
@@ -918,6 +935,7 @@ MPM_DEV uint32_t deposit_mask(const Tile& T, const Stencil& s) {
 // so the node sums are the same bits as the reference's separate atomics. Each product keeps the
 // reference operand order; the per-axis factors D.c[a] * off[a][o] are shared by the nodes that
 // use them, which changes no rounding.
+template <bool MASS>
 MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, const float off[3][3], uint32_t ok,
                        float mass, v3 dx, const m3& D) {
     v3 ax[3], by[3], cz[3];
@@ -954,7 +972,7 @@ MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, const floa
             if (P.grav_nz[1]) py = (int32_t)((uint32_t)py + (uint32_t)f2i(((gdiv.y * wm) * P.dt) * P.S));
             if (P.grav_nz[2]) pz = (int32_t)((uint32_t)pz + (uint32_t)f2i(((gdiv.z * wm) * P.dt) * P.S));
         }
-        atomicAdd(&T.acc[0][t], f2i(wm * P.S));
+        if (MASS) atomicAdd(&T.acc[0][t], f2i(wm * P.S));
         atomicAdd(&T.acc[1][t], px);
         atomicAdd(&T.acc[2][t], py);
         atomicAdd(&T.acc[3][t], pz);
@@ -969,6 +987,10 @@ enum { PB_FIRST = 0, PB_MID = 1, PB_LAST = 2 };
 //   MID:   G2P of iteration i-1 from grid[gin], solve + P2G of iteration i into grid[gout].
 //   LAST:  G2P of iteration k-1 from grid[gin], integrateParticles, write the particle state.
 // Every pass also clears grid[gclr] (the buffer the next pass writes) over the live blocks.
+// Node mass does not change between the iterations of a substep (positions and masses change only
+// in integrate, and the deposit set is the allocated stencil every time), so FIRST deposits it
+// once into gmass and MID / LAST read it there: the same integer sums the reference rebuilds
+// every iteration.
 // Residual bookkeeping of the G2P of iteration `it` (optional):
 // |dx_it - dx_{it-1} - (g / force_div) dt^2|_2 / domain.
 MPM_DEV float track_residual(const SimParams& P, const Bufs& B, uint32_t q, v3 dx, int it) {
@@ -1006,7 +1028,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
     const uint32_t sd = B.stride;
     {
         tile_setup(P, B, T, chunk);
-        if (MODE != PB_FIRST) tile_load_vel(P, T, gi);
+        if (MODE != PB_FIRST) tile_load_vel(P, T, gi, B.gmass);
         if (MODE != PB_LAST) tile_zero_acc(T);
         __syncthreads();
         const bool ok = T.center_ok != 0;
@@ -1035,7 +1057,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                     Stencil s; stencil(P, T, pos, s);
                     float off[3][3]; stencil_offsets(s, off);
                     solve_constraints(P, Fe, relax, D);
-                    p2g_pbmpm(P, T, s, off, deposit_mask(T, s), mass, dx, D);
+                    p2g_pbmpm<true>(P, T, s, off, deposit_mask(T, s), mass, dx, D);
                 } else {   // run whose own block is not allocated: carried unchanged to LAST
                     stm(B.nxt, sd, DD, q, D); st3(B.nxt, sd, DXX, q, dx); st3(B.nxt, sd, VX, q, ld3(B.cur, sd, VX, old));
                 }
@@ -1053,7 +1075,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 if (B.resid_dx) resid = track_residual(P, B, q, dx, it);
                 m3 D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                 solve_constraints(P, Fe, relax, D);
-                p2g_pbmpm(P, T, s, off, dep, mass, dx, D);
+                p2g_pbmpm<false>(P, T, s, off, dep, mass, dx, D);
                 }
             } else {
                 Part p;
@@ -1082,13 +1104,14 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
             if (MODE != PB_FIRST && B.resid_dx && it > 0) publish_residual(B, resid, it);
         }
         __syncthreads();
-        if (MODE != PB_LAST) tile_flush(T, go);
+        if (MODE == PB_FIRST) tile_flush_pb<true>(T, go, B.gmass);
+        if (MODE == PB_MID) tile_flush_pb<false>(T, go, B.gmass);
         __syncthreads();
     }
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256, 3) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
+__global__ __launch_bounds__(256, MODE == PB_LAST ? 3 : 4) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
     __shared__ Tile T;
     const uint32_t nchunks = B.counters[C_NCHUNKS];
     if (gclr >= 0) {
