@@ -1054,9 +1054,9 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 const float relax = pbmpm_relax(P, Fp);
                 B.relax[q] = relax;
                 if (ok) {
+                    solve_constraints(P, Fe, relax, D);
                     Stencil s; stencil(P, T, pos, s);
                     float off[3][3]; stencil_offsets(s, off);
-                    solve_constraints(P, Fe, relax, D);
                     p2g_pbmpm<true>(P, T, s, off, deposit_mask(T, s), mass, dx, D);
                 } else {   // run whose own block is not allocated: carried unchanged to LAST
                     stm(B.nxt, sd, DD, q, D); st3(B.nxt, sd, DXX, q, dx); st3(B.nxt, sd, VX, q, ld3(B.cur, sd, VX, old));
@@ -1075,7 +1075,10 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 if (B.resid_dx) resid = track_residual(P, B, q, dx, it);
                 m3 D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                 solve_constraints(P, Fe, relax, D);
-                p2g_pbmpm<false>(P, T, s, off, dep, mass, dx, D);
+                // the stencil is recomputed (same values) rather than kept live across the solve
+                Stencil s2; stencil(P, T, pos, s2);
+                float off2[3][3]; stencil_offsets(s2, off2);
+                p2g_pbmpm<false>(P, T, s2, off2, dep, mass, dx, D);
                 }
             } else {
                 Part p;
