@@ -212,6 +212,17 @@ def copy_calibration(mpm, nbytes):
     assert rc == 0 and out[-1] == 1.0
 
 
+def measured_valu(kernel):
+    """VALU wave-instructions per launch from the committed rocprofv3 --pmc summary, if one exists."""
+    f = ROOT / "profiles" / "valu.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get(kernel, {}).get("valu_wave_instructions_per_launch")
+    except (ValueError, OSError):
+        return None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if one exists."""
     f = ROOT / "profiles" / "traffic.json"
@@ -299,6 +310,9 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
                      "kernel": dom, "avg_launch_us": avg_s * 1e6,
                      "algorithmic_bytes_per_launch": algo_bytes,
+                     # the kernel is VALU-bound: issue utilisation = wave-instructions x 4 cycles /
+                     # (1,024 SIMDs x 2.4 GHz x launch time), instructions from the committed counters
+                     "valu_issue_utilisation": (measured_valu(dom) * 4 / (1024 * 2.4e9 * avg_s)) if measured_valu(dom) else None,
                      "step_frac_reference_bytes": (STEP_ALGO_BYTES(k) * value / 1e9) / HBM_PEAK_GBS},
         "kernel_ms_per_substep": {nm: round(v, 5) for nm, v in sorted(per_substep.items(), key=lambda x: -x[1])},
         "eager_ms_per_substep": eager_ms,
