@@ -676,6 +676,28 @@ MPM_DEV bool integrate_body(const SimParams& P, Part& p) {
     }
     return collision_candidate(P, p.pos, p.dx);
 }
+// integrateParticles.wgsl:12-72 for a particle pass that writes the new state to row q of nxt.
+// The motion part (x += dx, walls, collision candidate) touches only x, dx, v and the deformation
+// part (F_e = (I + D) F_e, plasticity) only F_e, F_p, so the two are independent and are done in
+// the order that keeps the fewest values live: F_e update, stores of D, motion and its stores,
+// then plasticity. The results are the same bits as integrate_body.
+MPM_DEV bool integrate_store(const SimParams& P, const Bufs& B, Part& p, uint32_t q) {
+    const uint32_t sd = B.stride;
+    p.Fe = mmul(madd(ident(), p.D), p.Fe);
+    stm(B.nxt, sd, DD, q, p.D);
+    p.pos = p.pos + p.dx;
+    float* pp = &p.pos.x; float* dd = &p.dx.x; float* vv = &p.vel.x;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (pp[a] < P.gmin[a]) { dd[a] = dd[a] * -0.5f; pp[a] = P.gmin[a]; vv[a] = vv[a] * -0.5f; }
+        if (pp[a] >= P.gmax[a]) { dd[a] = dd[a] * -0.5f; pp[a] = P.gmax[a]; vv[a] = vv[a] * -0.5f; }
+    }
+    const bool cand = collision_candidate(P, p.pos, p.dx);
+    st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel); st3(B.nxt, sd, DXX, q, p.dx);
+    plasticity(P, p);
+    stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
+    return cand;
+}
 // integrateParticles.wgsl:12-72 in one piece.
 MPM_DEV void integrate(const SimParams& P, const Bufs& B, Part& p) {
     if (integrate_body(P, p)) resolve_collision_narrow(P, B, p);
@@ -1098,10 +1120,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 } else {
                     p.dx = ld3(B.nxt, sd, DXX, q); p.D = ldm(B.nxt, sd, DD, q); p.vel = ld3(B.nxt, sd, VX, q);
                 }
-                cand = integrate_body(P, p);
-                st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
-                stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
-                st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
+                cand = integrate_store(P, B, p, q);
             }
             if (MODE == PB_LAST) queue_collider(B, cand, q);
             if (MODE != PB_FIRST && B.resid_dx && it > 0) publish_residual(B, resid, it);
@@ -1114,7 +1133,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
 }
 
 template <int MODE>
-__global__ __launch_bounds__(256, MODE == PB_LAST ? 3 : 4) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
+__global__ __launch_bounds__(256, 4) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
     __shared__ Tile T;
     const uint32_t nchunks = B.counters[C_NCHUNKS];
     if (gclr >= 0) {
@@ -1268,10 +1287,7 @@ MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             } else {
                 p.dx = ld3(B.nxt, sd, DXX, q); p.D = ldm(B.nxt, sd, DD, q); p.vel = ld3(B.nxt, sd, VX, q);
             }
-            cand = integrate_body(P, p);
-            st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
-            stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
-            st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
+            cand = integrate_store(P, B, p, q);
             }
             queue_collider(B, cand, q);
         }
@@ -1280,7 +1296,7 @@ MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
 }
 
 template <int DUMMY = 0>
-__global__ __launch_bounds__(256) void k_explicit_g2p(SimParams P, Bufs B) {
+__global__ __launch_bounds__(256, 4) void k_explicit_g2p(SimParams P, Bufs B) {
     __shared__ Tile T;
     const uint32_t nchunks = B.counters[C_NCHUNKS];
     for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) explicit_g2p_chunk<false>(P, B, T, B.chunks[ch], B.grid0);
