@@ -234,7 +234,7 @@ MPM_DEV void ens_clear_slice(const SimParams& P, int4* g, uint32_t nblk, uint32_
 
 // The transfer phases of every scene. phase < 0: persistent, all phases with dependency waits;
 // phase >= 0: only that phase's tasks (the launch boundary orders the phases). OCC: resident
-// workgroups per CU the register allocation targets (2: no spills; 3: a few spilled registers).
+// workgroups per CU the register allocation targets (2, 3 or 4; the autotuner measures which pays).
 template <int OCC>
 __global__ __launch_bounds__(256, OCC) void k_ens_transfer(Bufs B, Ens E, int phase) {
     __shared__ Tile T;

@@ -67,7 +67,7 @@ struct mpm_ctx {
     // learned warm start: MLP parameters, Adam state, standardisation, logged dataset
     float *mlp_params = nullptr, *mlp_grads = nullptr, *mlp_m1 = nullptr, *mlp_m2 = nullptr, *mlp_norm = nullptr;
     short* mlp_wbf = nullptr;
-    double* mlp_colsum = nullptr;
+    float* mlp_colsum = nullptr;              // per-workgroup partial column moments (1024 x 64)
     float* mlp_loss = nullptr;
     float *ds_feats = nullptr, *ds_targs = nullptr;
     uint64_t ds_cap = 0, ds_n = 0;
@@ -497,7 +497,8 @@ void enqueue_ens_substep(mpm_ctx* c, bool full) {
     const int nl = c->ens_persistent ? 1 : (int)c->ens_R;
     for (int r = 0; r < nl; ++r) {
         const int ph = c->ens_persistent ? -1 : r;
-        if (c->ens_wg_per_cu >= 3) k_ens_transfer<3><<<G, 256, 0, s>>>(B, E, ph);
+        if (c->ens_wg_per_cu >= 4) k_ens_transfer<4><<<G, 256, 0, s>>>(B, E, ph);
+        else if (c->ens_wg_per_cu == 3) k_ens_transfer<3><<<G, 256, 0, s>>>(B, E, ph);
         else k_ens_transfer<2><<<G, 256, 0, s>>>(B, E, ph);
         kt(c, KID_ENS_TRANSFER);
     }
@@ -1095,7 +1096,7 @@ static int ensure_mlp(mpm_ctx* c) {
     if ((rc = dalloc(c, &c->mlp_params, MLP_NPARAM)) || (rc = dalloc(c, &c->mlp_grads, MLP_NPARAM)) ||
         (rc = dalloc(c, &c->mlp_m1, MLP_NPARAM)) || (rc = dalloc(c, &c->mlp_m2, MLP_NPARAM)) ||
         (rc = dalloc(c, &c->mlp_norm, 64)) || (rc = dalloc(c, &c->mlp_wbf, 16 * 64 + 64 * 64 + 64 * 16)) ||
-        (rc = dalloc(c, &c->mlp_colsum, 64)) || (rc = dalloc(c, &c->mlp_loss, 1)))
+        (rc = dalloc(c, &c->mlp_colsum, 1024 * 64)) || (rc = dalloc(c, &c->mlp_loss, 1)))
         return rc;
     HIPCHK(c, hipMemset(c->mlp_grads, 0, MLP_NPARAM * 4));
     HIPCHK(c, hipMemset(c->mlp_m1, 0, MLP_NPARAM * 4));
@@ -1156,11 +1157,12 @@ int mpm_learn_train(mpm_ctx* c, int steps, int batch, float lr, uint32_t seed, f
     if ((rc = ensure_mlp(c))) return rc;
     if (!c->mlp_ready) {
         // standardisation from the collected pairs, then initial weights
-        HIPCHK(c, hipMemsetAsync(c->mlp_colsum, 0, 64 * 8, c->stream));
         k_mlp_colstats<<<1024, 256, 0, c->stream>>>(c->ds_feats, c->ds_targs, c->ds_n, c->mlp_colsum);
-        double cs[64];
-        HIPCHK(c, hipMemcpyAsync(cs, c->mlp_colsum, sizeof(cs), hipMemcpyDeviceToHost, c->stream));
+        std::vector<float> partial(1024 * 64);
+        HIPCHK(c, hipMemcpyAsync(partial.data(), c->mlp_colsum, partial.size() * 4, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
+        double cs[64] = {};
+        for (size_t b = 0; b < 1024; ++b) for (int k = 0; k < 64; ++k) cs[k] += partial[b * 64 + k];
         float norm[64];
         for (int k = 0; k < 16; ++k) {
             const double n = (double)c->ds_n;
@@ -1514,7 +1516,7 @@ int mpm_ensemble_autotune(mpm_ctx* c, int trial_substeps, mpm_tuning* out, int c
     const int cur0 = c->cur;
     const uint64_t sub0 = c->substeps;
     const uint32_t chunks[] = {256, 512, 1024};
-    const uint32_t wgs[] = {1, 2, 3};
+    const uint32_t wgs[] = {1, 2, 3, 4};
     std::vector<mpm_tuning> cand;
     for (int pers = 1; pers >= 0; --pers)
         for (uint32_t ch : chunks)

@@ -435,16 +435,38 @@ __global__ __launch_bounds__(256) void k_mlp_log_targets(Bufs B, uint32_t n, con
     for (int c = 0; c < 4; ++c) o[c] = make_float4(t[4 * c], t[4 * c + 1], t[4 * c + 2], t[4 * c + 3]);
 }
 
-// Column sums (FP64) of the dataset for standardisation: sums[0..15] x, [16..31] x^2, [32..47] t, [48..63] t^2.
-__global__ __launch_bounds__(256) void k_mlp_colstats(const float* feats, const float* targs, uint64_t n, double* sums) {
-    double acc[4] = {0, 0, 0, 0};
+// Column moments of the dataset for standardisation, FP32 with compensated (Kahan) summation:
+// each workgroup writes its 64 partial sums (x, x^2, t, t^2 per column, 16 columns each) to
+// part[blockIdx.x * 64 ..]; the host adds the partials in double.
+__global__ __launch_bounds__(256) void k_mlp_colstats(const float* feats, const float* targs, uint64_t n, float* part) {
+    __shared__ float red[16][64];
+    float acc[4] = {0, 0, 0, 0}, cmp[4] = {0, 0, 0, 0};
     const int c = threadIdx.x & 15;
     const uint64_t rows_per = (gridDim.x * (uint64_t)blockDim.x) / 16;
     for (uint64_t r = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / 16; r < n; r += rows_per) {
-        const double x = feats[r * 16 + c], t = targs[r * 16 + c];
-        acc[0] += x; acc[1] += x * x; acc[2] += t; acc[3] += t * t;
+        const float x = feats[r * 16 + c], t = targs[r * 16 + c];
+        const float v[4] = {x, x * x, t, t * t};
+        for (int k = 0; k < 4; ++k) {
+            const float y = v[k] - cmp[k];
+            const float s = acc[k] + y;
+            cmp[k] = (s - acc[k]) - y;
+            acc[k] = s;
+        }
     }
-    for (int k = 0; k < 4; ++k) atomicAdd(&sums[16 * k + c], acc[k]);
+    // 16 threads share each column: reduce them in LDS (rows of 16 lanes)
+    const int row = threadIdx.x >> 4;
+    for (int k = 0; k < 4; ++k) red[row][16 * k + c] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        float s = 0.0f, comp = 0.0f;
+        for (int r = 0; r < 16; ++r) {
+            const float y = red[r][threadIdx.x] - comp;
+            const float t = s + y;
+            comp = (t - s) - y;
+            s = t;
+        }
+        part[blockIdx.x * 64u + threadIdx.x] = s;
+    }
 }
 
 // Warm start of a PBMPM substep: D += de-standardised MLP prediction, per particle, in `cur`.
