@@ -126,3 +126,78 @@ def test_moved_and_scaled_collider_bitexact(mpm, orc):
         o.step(1, 1)
         assert_particles_equal(mpm, sim.download(), o.particles(), f"moved collider substep {s}")
     sim.close()
+
+
+def _chain_bvh(n_tris, lo, hi):
+    """Right-leaning chain: internal node i has the leaf of triangle i on its left and internal
+    node i + 1 (or the last leaf) on its right; every box is the whole mesh's."""
+    m = n_tris - 1
+    nodes = np.zeros((m + n_tris, 8), np.uint32)
+    nodes.view(np.float32)[:, 0:3] = lo
+    nodes.view(np.float32)[:, 4:7] = hi
+    for i in range(m):
+        nodes[i, 3] = m + i
+        nodes[i, 7] = i + 1 if i + 1 < m else m + n_tris - 1
+    for j in range(n_tris):
+        nodes[m + j, 3] = j
+        nodes[m + j, 7] = 0x80000000 | 1
+    return nodes
+
+
+def test_bvh_deeper_than_the_traversal_stack_bitexact(mpm, orc):
+    """colliderPrelude.wgsl:215-291 pushes children only while fewer than 31 entries are pending:
+    in a chain with more than 31 right turns the deeper triangles are never visited. The device
+    walk (threaded records) must skip exactly those; a flat leaf with every triangle gives other
+    particles, so the limit is exercised."""
+    rng = np.random.default_rng(41)
+    v, idx = mpm.scenes.heightfield_mesh(12, 12, 1.0, 0.3, 0.45, rng)
+    n = 6000
+    pos = np.stack([rng.uniform(0.05, 0.95, n), rng.uniform(0.47, 0.55, n), rng.uniform(0.05, 0.95, n)], 1).astype(np.float32)
+    p = mpm.make_particles(pos, vel=(0.0, -2.0, 0.0))
+    sc = dict(particles=p, sim=dict(grid_res=(64, 64, 64), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1e-4,
+                                    gravity=(0.0, -9.8, 0.0)), collider=None)
+    sim, o = make_pair(mpm, orc, sc)
+    cm = sim.set_collider(v, idx)
+    n_tris = cm.numIndices // 3
+    chain = _chain_bvh(n_tris, np.min(v, 0), np.max(v, 0))
+    cm.bvhNodes, cm.numBvhNodes = chain, chain.shape[0]
+    cm.upload(sim)
+    o.set_uniforms(sim.uniform_bytes())
+    o.set_collider(cm.colliderData, chain)
+    flat = np.zeros((1, 8), np.uint32)
+    flat.view(np.float32)[0, 0:3] = np.min(v, 0)
+    flat.view(np.float32)[0, 4:7] = np.max(v, 0)
+    flat[0, 7] = n_tris | 0x80000000
+    of = orc.OracleSim(p, sim.uniform_bytes(), hash_map_size=200_003, max_blocks=70_000, gravity=(0.0, -9.8, 0.0))
+    of.set_collider(cm.colliderData, flat)
+    for s in range(3):
+        sim.step(0, 1)
+        o.step(0, 1)
+        of.step(0, 1)
+        assert_particles_equal(mpm, sim.download(), o.particles(), f"chain BVH substep {s}")
+    differ = (mpm.state_columns(o.particles()) != mpm.state_columns(of.particles())).any(1).sum()
+    assert differ > 50, differ
+    sim.close()
+
+
+def test_bvh_that_is_not_a_tree_is_rejected(mpm):
+    """A node reached twice (shared child or a cycle) would make the traversal revisit or loop:
+    mpm_set_collider refuses it and leaves the collider it had in place."""
+    v, idx = mpm.scenes.box_mesh((0.5, 0.5, 0.5), 0.2)
+    with mpm.MpmSimulation(1000, grid_res=(32, 32, 32), grid_min=(0, 0, 0), grid_max=(1, 1, 1)) as sim:
+        cm = sim.set_collider(v, idx)
+        lo, hi = np.min(v, 0), np.max(v, 0)
+        for shape in ("shared", "cycle"):
+            nodes = np.zeros((3, 8), np.uint32)
+            nodes.view(np.float32)[:, 0:3] = lo
+            nodes.view(np.float32)[:, 4:7] = hi
+            nodes[0, 3], nodes[0, 7] = 1, (1 if shape == "shared" else 2)
+            nodes[1, 3], nodes[1, 7] = 0, 0x80000000 | 12
+            if shape == "shared":
+                nodes[2, 3], nodes[2, 7] = 0, 0x80000000 | 12
+            else:
+                nodes[2, 3], nodes[2, 7] = 1, 0          # internal node whose right child is the root
+            rc = sim.lib.mpm_set_collider(sim.ctx, mpm.managers._ptr(cm.colliderData), cm.colliderData.size, cm.numIndices,
+                                          mpm.managers._ptr(nodes), nodes.shape[0])
+            assert rc == mpm._lib.MPM_E_INVALID, (shape, rc)
+            assert "tree" in sim.lib.mpm_last_error(sim.ctx).decode()

@@ -49,8 +49,9 @@ struct mpm_ctx {
     uint32_t* counters = nullptr;
     uint32_t* cqueue = nullptr;
     uint32_t* coll_data = nullptr;
-    uint4* bvh = nullptr;
-    size_t n_coll_words = 0, n_bvh_nodes = 0;
+    float4* crec = nullptr;           // threaded BVH records (2 float4 each)
+    float4* ctri = nullptr;           // world-space triangles (CT_WORDS float4 each)
+    size_t n_coll_words = 0, n_coll_rec = 0, n_coll_tri = 0;
     uint32_t coll_num_indices = 0;
     int last_grid = 0;
     bool has_particles = false;
@@ -190,7 +191,7 @@ Bufs make_bufs(mpm_ctx* c) {
     B.gmass = c->gmass;
     B.counters = c->counters;
     B.cqueue = c->cqueue;
-    B.coll_data = c->coll_data; B.bvh = c->bvh;
+    B.coll_data = c->coll_data; B.crec = c->crec; B.ctri = c->ctri;
     B.resid_dx = c->resid_on ? c->resid_dx : nullptr;
     B.resid_max = c->resid_max;
     B.resid_sum = c->resid_sum;
@@ -276,8 +277,9 @@ int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, f
     if ((vr = verify_divisor(c, P.dt, P.S, &P.fast_dt))) return vr;
     P.wall_mode = g.wall_mode;
     P.wall_cells = g.wall_cells;
-    P.n_bvh_nodes = scene_only ? 0u : (uint32_t)c->n_bvh_nodes;
-    P.has_collider = (!scene_only && c->n_bvh_nodes > 0 && c->n_coll_words > 0) ? 1u : 0u;
+    P.n_coll_rec = scene_only ? 0u : (uint32_t)c->n_coll_rec;
+    P.n_coll_tri = scene_only ? 0u : (uint32_t)c->n_coll_tri;
+    P.has_collider = (!scene_only && c->n_coll_rec > 0 && c->n_coll_words > 0) ? 1u : 0u;
     if (P.has_collider && P.coll_num_indices != c->coll_num_indices)
         return fail(c, MPM_E_RANGE, "uniform colliderNumIndices differs from the uploaded collider");
     P.margin = g.collider_margin;
@@ -372,7 +374,12 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
         if (full) hipEventRecord(c->ev[2], s);
         k_explicit_g2p<0><<<G, 256, 0, s>>>(P, B); kt(c, KID_EX_G2P);
     }
-    if (P.has_collider) { k_collide<<<gs, 256, 0, s>>>(P, B); kt(c, KID_COLLIDE); }
+    if (P.has_collider) {
+        const uint32_t gc = std::min<uint32_t>(grid_for(c->n, 1u << 30), (uint32_t)c->n_cu * 8u);
+        if (P.n_coll_rec <= COLL_LDS_REC && P.n_coll_tri <= COLL_LDS_TRI) k_collide<true><<<gc, 256, 0, s>>>(P, B);
+        else k_collide<false><<<gc, 256, 0, s>>>(P, B);
+        kt(c, KID_COLLIDE);
+    }
     k_straggle<<<gs, 256, 0, s>>>(P, B); kt(c, KID_STRAGGLE);
     if (full) hipEventRecord(c->ev[3], s);
     return true;
@@ -683,16 +690,48 @@ int mpm_set_collider(mpm_ctx* c, const uint32_t* data, size_t n_words, uint32_t 
     }
     const size_t n_vert_words = n_words - n_indices;
     for (size_t i = 0; i < n_indices; ++i) if ((uint64_t)data[i] * 3 + 2 >= n_vert_words) return fail(c, MPM_E_INVALID, "collider index beyond the vertex data");
+    if (n_nodes > REC_INDEX || n_tris > REC_INDEX) return fail(c, MPM_E_RANGE, "collider too large (2^30 nodes or triangles)");
+    // Threaded records in the visiting order of the reference's traversal (colliderPrelude.wgsl
+    // :174-356: pop a node; cull by box; leaf -> triangles; else, while fewer than 31 entries are
+    // pending, push left then right). The right subtree is visited first; a node popped with s
+    // pending entries has its right child popped with s + 1 and its left child with s.
+    std::vector<uint32_t> rec;   // 8 words per record
+    if (n_words && n_nodes) {
+        std::vector<uint8_t> seen(n_nodes, 0);
+        struct Task { uint32_t node, s, fix; };   // fix != ~0u: set the skip index of record `fix`
+        std::vector<Task> todo{{0u, 0u, ~0u}};
+        while (!todo.empty()) {
+            const Task t = todo.back();
+            todo.pop_back();
+            if (t.fix != ~0u) { rec[8 * (size_t)t.fix + 7] |= (uint32_t)(rec.size() / 8); continue; }
+            if (seen[t.node]) return fail(c, MPM_E_INVALID, "BVH nodes do not form a tree (a node is reached twice)");
+            seen[t.node] = 1;
+            const uint32_t* nw = w + 8 * (size_t)t.node;
+            const uint32_t l = nw[3], r = nw[7];
+            const uint32_t idx = (uint32_t)(rec.size() / 8);
+            uint32_t lo_w = 0, hi_w;
+            if (r & 0x80000000u) { lo_w = l; hi_w = REC_LEAF | (l + (r & 0x7FFFFFFFu)); }
+            else if (t.s < 31u) {
+                hi_w = REC_DESCEND;
+                todo.push_back({0u, 0u, idx});
+                todo.push_back({l, t.s, ~0u});
+                todo.push_back({r, t.s + 1u, ~0u});
+            } else hi_w = idx + 1u;
+            rec.insert(rec.end(), {nw[0], nw[1], nw[2], lo_w, nw[4], nw[5], nw[6], hi_w});
+        }
+    }
     if (c->coll_data) { hipFree(c->coll_data); c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->coll_data)); c->coll_data = nullptr; }
-    if (c->bvh) { hipFree(c->bvh); c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->bvh)); c->bvh = nullptr; }
-    c->n_coll_words = 0; c->n_bvh_nodes = 0;
+    if (c->crec) { hipFree(c->crec); c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->crec)); c->crec = nullptr; }
+    if (c->ctri) { hipFree(c->ctri); c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->ctri)); c->ctri = nullptr; }
+    c->n_coll_words = 0; c->n_coll_rec = 0; c->n_coll_tri = 0;
     int rc;
     if (n_words && n_nodes) {
         if ((rc = dalloc(c, &c->coll_data, n_words))) return rc;
-        if ((rc = dalloc(c, &c->bvh, 2 * n_nodes))) return rc;
+        if ((rc = dalloc(c, &c->crec, rec.size() / 4))) return rc;
+        if ((rc = dalloc(c, &c->ctri, (size_t)CT_WORDS * n_tris))) return rc;
         HIPCHK(c, hipMemcpy(c->coll_data, data, n_words * 4, hipMemcpyHostToDevice));
-        HIPCHK(c, hipMemcpy(c->bvh, nodes, n_nodes * 32, hipMemcpyHostToDevice));
-        c->n_coll_words = n_words; c->n_bvh_nodes = n_nodes;
+        HIPCHK(c, hipMemcpy(c->crec, rec.data(), rec.size() * 4, hipMemcpyHostToDevice));
+        c->n_coll_words = n_words; c->n_coll_rec = rec.size() / 8; c->n_coll_tri = n_tris;
     }
     c->coll_num_indices = n_indices;
     wru<uint32_t>(c->uni, 348, n_indices);

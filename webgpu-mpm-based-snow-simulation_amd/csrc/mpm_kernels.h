@@ -68,7 +68,8 @@ struct SimParams {
     float learn_limit;         // warm start: largest accepted |standardised prediction|
     // collider (uniforms @224..@432)
     float cmin[3], cmax[3], T[16], Tinv[16], cvel[3];
-    uint32_t coll_num_indices, coll_num_objects, n_bvh_nodes, has_collider;
+    uint32_t coll_num_indices, coll_num_objects, has_collider;
+    uint32_t n_coll_rec, n_coll_tri;   // threaded BVH records, triangles of the collider
     float margin, fr_ccd, fr_prox, vel_factor;
     // pointer interaction (uniforms @496..@528)
     float ip[3], istrength, iradius;
@@ -91,7 +92,9 @@ struct Bufs {
     float* resid_dx;           // optional: dx of the previous G2P per particle (3 planes, stride)
     uint32_t* resid_max;       // optional: per G2P iteration, max over particles of |dx_i - dx_{i-1}|_2 (float bits)
     float* resid_sum;          // optional: per G2P iteration, sum over substeps of that max
-    const uint32_t* coll_data; const uint4* bvh;
+    const uint32_t* coll_data;
+    const float4* crec;        // threaded BVH: 2 float4 per record (visiting order of the stack DFS)
+    float4* ctri;              // world-space triangles, CT_WORDS float4 each (written by k_nbr every substep)
 };
 
 MPM_DEV int4* grid_sel(const Bufs& B, int i) { return i == 0 ? B.grid0 : (i == 1 ? B.grid1 : B.grid2); }
@@ -477,7 +480,8 @@ MPM_DEV uint32_t table_lookup(const SimParams& P, const Bufs& B, int bx, int by,
 }
 // Neighbour-block table of every run (27 dense block indices, allocated by any particle) and the
 // clear of grid buffer 0 over the allocated blocks (clearMappedBlocks.wgsl:9-26 restricted to
-// live blocks).
+// live blocks); also the world-space collider triangles of this substep.
+MPM_DEV void collider_prep_triangle(const SimParams& P, const Bufs& B, uint32_t t);
 __global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) {
     const uint32_t nruns = B.counters[C_NRUNS];
     const uint32_t gsz = gridDim.x * blockDim.x;
@@ -489,6 +493,8 @@ __global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) {
     }
     const uint32_t ncell = n_blocks_of(P, B) * 64u;
     for (uint32_t c = gi; c < ncell; c += gsz) { B.grid0[c] = make_int4(0, 0, 0, 0); B.gmass[c] = 0; }
+    if (P.has_collider)
+        for (uint32_t t = gi; t < P.n_coll_tri; t += gsz) collider_prep_triangle(P, B, t);
 }
 
 // ================================================================== particle update helpers
@@ -515,9 +521,10 @@ MPM_DEV v3 xform(const float* T, v3 p) {
     v3 c0 = mk(T[0], T[1], T[2]), c1 = mk(T[4], T[5], T[6]), c2 = mk(T[8], T[9], T[10]), c3 = mk(T[12], T[13], T[14]);
     return ((c0 * p.x + c1 * p.y) + c2 * p.z) + c3 * 1.0f;
 }
-// colliderPrelude.wgsl:37-78 (Ericson, Real-Time Collision Detection 5.1.5)
-MPM_DEV v3 closest_point_triangle(v3 p, v3 a, v3 b, v3 c) {
-    v3 ab = b - a, ac = c - a, ap = p - a;
+// colliderPrelude.wgsl:37-78 (Ericson, Real-Time Collision Detection 5.1.5). ab = b - a and
+// ac = c - a come precomputed (the same FP32 differences the function forms first).
+MPM_DEV v3 closest_point_triangle(v3 p, v3 a, v3 b, v3 c, v3 ab, v3 ac) {
+    v3 ap = p - a;
     float d1 = dot(ab, ap), d2 = dot(ac, ap);
     if (d1 <= 0.0f && d2 <= 0.0f) return a;
     v3 bp = p - b;
@@ -539,10 +546,9 @@ MPM_DEV v3 closest_point_triangle(v3 p, v3 a, v3 b, v3 c) {
     float v = vb * denom, w = vc * denom;
     return (a + v * ab) + w * ac;
 }
-// colliderPrelude.wgsl:80-102 (Möller–Trumbore 1997)
-MPM_DEV float intersect_ray_triangle(v3 o, v3 dir, v3 a, v3 b, v3 c) {
+// colliderPrelude.wgsl:80-102 (Möller–Trumbore 1997); e1 = b - a, e2 = c - a precomputed.
+MPM_DEV float intersect_ray_triangle(v3 o, v3 dir, v3 a, v3 e1, v3 e2) {
     const float EPS = 1e-6f;
-    v3 e1 = b - a, e2 = c - a;
     v3 h = cross(dir, e2);
     float aa = dot(e1, h);
     if (aa > -EPS && aa < EPS) return -1.0f;
@@ -562,6 +568,26 @@ MPM_DEV v3 collider_vertex(const SimParams& P, const uint32_t* cd, uint32_t idx)
     return mk(as_f(cd[b]), as_f(cd[b + 1]), as_f(cd[b + 2]));
 }
 MPM_DEV float fsign(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+MPM_DEV v3 xyz(float4 a) { return mk(a.x, a.y, a.z); }
+MPM_DEV float4 with_w(v3 a, float w) { return make_float4(a.x, a.y, a.z, w); }
+
+// World-space collider triangle t: everything of one triangle visit that does not depend on the
+// particle, computed with the operations the narrow phase of colliderPrelude.wgsl:174-356 applies
+// per visit (vertex transform, edges, face normal n = cross(e1, e2) / |cross(e1, e2)|; the
+// closest-feature normal is taken only where |cross| > 1e-6, flagged in n.w).
+enum { CT_V0 = 0, CT_V1, CT_V2, CT_E1, CT_E2, CT_N, CT_WORDS };
+MPM_DEV void collider_prep_triangle(const SimParams& P, const Bufs& B, uint32_t t) {
+    const v3 v0 = xform(P.T, collider_vertex(P, B.coll_data, B.coll_data[t * 3u]));
+    const v3 v1 = xform(P.T, collider_vertex(P, B.coll_data, B.coll_data[t * 3u + 1u]));
+    const v3 v2 = xform(P.T, collider_vertex(P, B.coll_data, B.coll_data[t * 3u + 2u]));
+    const v3 e1 = v1 - v0, e2 = v2 - v0;
+    const v3 nn = cross(e1, e2);
+    const float len = length(nn);
+    float4* o = B.ctri + (size_t)CT_WORDS * t;
+    o[CT_V0] = with_w(v0, 0.0f); o[CT_V1] = with_w(v1, 0.0f); o[CT_V2] = with_w(v2, 0.0f);
+    o[CT_E1] = with_w(e1, 0.0f); o[CT_E2] = with_w(e2, 0.0f);
+    o[CT_N] = with_w(nn / len, len > 1e-6f ? 1.0f : 0.0f);
+}
 
 // colliderPrelude.wgsl:154-172: broad phase — the particle's path box against the collider's
 // transformed bounds grown by the margin. Particles that fail it leave resolveParticleCollision
@@ -578,9 +604,19 @@ MPM_DEV bool collision_candidate(const SimParams& P, v3 pos, v3 dx) {
     return P.coll_num_objects != 0u;
 }
 
-// colliderPrelude.wgsl:174-356 for a broad-phase candidate: BVH depth-first traversal (stack of
-// 32, left pushed first), nearest-point and swept-ray tests per leaf triangle, velocity response.
-MPM_DEV void resolve_collision_narrow(const SimParams& P, const Bufs& B, Part& pt) {
+// Threaded BVH record (built on the host from the node array): lo = (box min, first triangle of a
+// leaf), hi = (box max, word). word: REC_LEAF | end triangle for a leaf; else REC_DESCEND if the
+// traversal enters the children, and the record index after the subtree.
+constexpr uint32_t REC_LEAF = 0x80000000u, REC_DESCEND = 0x40000000u, REC_INDEX = 0x3FFFFFFFu;
+
+// colliderPrelude.wgsl:174-356 for a broad-phase candidate. The reference walks the BVH depth
+// first with a stack of 32 (left child pushed first, so the right subtree is visited first, and
+// no push once 31 entries are pending); the records list the nodes in exactly that visiting order
+// with the index to continue at when a box is missed, so the walk below visits the same nodes in
+// the same order without a stack. Per leaf triangle: nearest point and swept-ray test, strict
+// improvements only (the first of equal candidates wins, as in the reference); then the velocity
+// response. `rec` / `tri` may point to LDS copies.
+MPM_DEV void resolve_collision_narrow(const SimParams& P, const float4* __restrict__ rec, const float4* __restrict__ tri, Part& pt) {
     const float margin = P.margin;
     const v3 mg = mk(margin, margin, margin);
     const v3 cur = pt.pos;
@@ -594,44 +630,43 @@ MPM_DEV void resolve_collision_narrow(const SimParams& P, const Bufs& B, Part& p
     const bool do_ccd = length(ray) > 1e-4f;
     const v3 cur_l = xform(P.Tinv, cur), prev_l = xform(P.Tinv, prev);
     const v3 pmin = vmin(cur_l, prev_l), pmax = vmax(cur_l, prev_l);
-    uint32_t stack[32];
-    uint32_t sp = 0;
-    stack[sp++] = 0;
-    while (sp > 0) {
-        const uint32_t ni = stack[--sp];
-        if (ni >= P.n_bvh_nodes) continue;
-        const uint4 n0 = B.bvh[2 * ni], n1 = B.bvh[2 * ni + 1];
-        const v3 bmin = mk(as_f(n0.x), as_f(n0.y), as_f(n0.z)) - mg;
-        const v3 bmax = mk(as_f(n1.x), as_f(n1.y), as_f(n1.z)) + mg;
-        if (pmin.x > bmax.x || pmin.y > bmax.y || pmin.z > bmax.z || pmax.x < bmin.x || pmax.y < bmin.y || pmax.z < bmin.z) continue;
-        if (n1.w & 0x80000000u) {
-            const uint32_t start = n0.w, end = start + (n1.w & 0x7FFFFFFFu);
-            for (uint32_t i = start; i < end; ++i) {
-                const v3 v0 = xform(P.T, collider_vertex(P, B.coll_data, B.coll_data[i * 3u]));
-                const v3 v1 = xform(P.T, collider_vertex(P, B.coll_data, B.coll_data[i * 3u + 1u]));
-                const v3 v2 = xform(P.T, collider_vertex(P, B.coll_data, B.coll_data[i * 3u + 2u]));
-                const v3 cp = closest_point_triangle(cur, v0, v1, v2);
+    uint32_t i = 0;
+    while (i < P.n_coll_rec) {
+        const float4 lo = rec[2 * i], hi = rec[2 * i + 1];
+        const uint32_t w = as_u(hi.w);
+        const v3 bmin = xyz(lo) - mg;
+        const v3 bmax = xyz(hi) + mg;
+        const bool leaf = (w & REC_LEAF) != 0u;
+        if (pmin.x > bmax.x || pmin.y > bmax.y || pmin.z > bmax.z || pmax.x < bmin.x || pmax.y < bmin.y || pmax.z < bmin.z) {
+            i = leaf ? i + 1u : (w & REC_INDEX);
+            continue;
+        }
+        if (leaf) {
+            const uint32_t end = w & REC_INDEX;
+            for (uint32_t t = as_u(lo.w); t < end; ++t) {
+                const float4* tp = tri + (size_t)CT_WORDS * t;
+                const v3 v0 = xyz(tp[CT_V0]), e1 = xyz(tp[CT_E1]), e2 = xyz(tp[CT_E2]);
+                const v3 cp = closest_point_triangle(cur, v0, xyz(tp[CT_V1]), xyz(tp[CT_V2]), e1, e2);
                 const v3 df = cur - cp;
                 const float d2 = dot(df, df);
                 if (d2 < min_d2) {
                     min_d2 = d2; cl_p = cp;
-                    const v3 nn = cross(v1 - v0, v2 - v0);
-                    const float len = length(nn);
-                    if (len > 1e-6f) cl_n = nn / len;
+                    const float4 n = tp[CT_N];
+                    if (n.w != 0.0f) cl_n = xyz(n);
                 }
                 if (do_ccd) {
-                    const float t = intersect_ray_triangle(prev, ray, v0, v1, v2);
-                    if (t > 0.0f && t < min_t) {
-                        min_t = t; has_hit = true;
-                        hit_p = prev + ray * t;
-                        const v3 fn = normalize(cross(v1 - v0, v2 - v0));
+                    const float t_hit = intersect_ray_triangle(prev, ray, v0, e1, e2);
+                    if (t_hit > 0.0f && t_hit < min_t) {
+                        min_t = t_hit; has_hit = true;
+                        hit_p = prev + ray * t_hit;
+                        const v3 fn = xyz(tp[CT_N]);
                         hit_n = fn * fsign(dot(ray, fn));
                     }
                 }
             }
-        } else if (sp < 31u) {
-            stack[sp++] = n0.w;
-            stack[sp++] = n1.w;
+            i = i + 1u;
+        } else {
+            i = (w & REC_DESCEND) ? i + 1u : (w & REC_INDEX);
         }
     }
     const v3 cvel = mk(P.cvel[0], P.cvel[1], P.cvel[2]);
@@ -661,7 +696,6 @@ MPM_DEV void resolve_collision_narrow(const SimParams& P, const Bufs& B, Part& p
         pt.pos = cl_p + pd * margin;
     }
 }
-
 // integrateParticles.wgsl:12-66 — everything before resolveParticleCollision. Returns whether the
 // particle is a collider candidate (its narrow phase then runs in k_collide).
 MPM_DEV bool integrate_body(const SimParams& P, Part& p) {
@@ -700,7 +734,7 @@ MPM_DEV bool integrate_store(const SimParams& P, const Bufs& B, Part& p, uint32_
 }
 // integrateParticles.wgsl:12-72 in one piece.
 MPM_DEV void integrate(const SimParams& P, const Bufs& B, Part& p) {
-    if (integrate_body(P, p)) resolve_collision_narrow(P, B, p);
+    if (integrate_body(P, p)) resolve_collision_narrow(P, B.crec, B.ctri, p);
 }
 // Appends sorted index q to the collider queue (one atomic per wave).
 MPM_DEV void queue_collider(const Bufs& B, bool cand, uint32_t q) {
@@ -713,15 +747,30 @@ MPM_DEV void queue_collider(const Bufs& B, bool cand, uint32_t q) {
     if (cand) B.cqueue[b + (uint32_t)__popcll(m & lanemask_lt())] = q;
 }
 // resolveParticleCollision (colliderPrelude.wgsl:174-356) for the queued particles: reads and
-// writes x, v, dx of the new-order state.
+// writes x, v, dx of the new-order state. LDS = true: the records and triangles are first copied
+// into LDS (colliders of at most COLL_LDS_REC records and COLL_LDS_TRI triangles).
+constexpr uint32_t COLL_LDS_REC = 64, COLL_LDS_TRI = 128;
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_collide(SimParams P, Bufs B) {
     const uint32_t nq = B.counters[C_NCOLL];
     const uint32_t sd = B.stride;
+    if (blockIdx.x * blockDim.x >= nq) return;
+    const float4* rec = B.crec;
+    const float4* tri = B.ctri;
+    if (LDS) {
+        __shared__ float4 srec[2 * COLL_LDS_REC];
+        __shared__ float4 stri[CT_WORDS * COLL_LDS_TRI];
+        for (uint32_t i = threadIdx.x; i < 2u * P.n_coll_rec; i += blockDim.x) srec[i] = B.crec[i];
+        for (uint32_t i = threadIdx.x; i < (uint32_t)CT_WORDS * P.n_coll_tri; i += blockDim.x) stri[i] = B.ctri[i];
+        __syncthreads();
+        rec = srec;
+        tri = stri;
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
         const uint32_t q = B.cqueue[i];
         Part p;
         p.pos = ld3(B.nxt, sd, PX, q); p.dx = ld3(B.nxt, sd, DXX, q); p.vel = ld3(B.nxt, sd, VX, q);
-        resolve_collision_narrow(P, B, p);
+        resolve_collision_narrow(P, rec, tri, p);
         st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, DXX, q, p.dx); st3(B.nxt, sd, VX, q, p.vel);
     }
 }
