@@ -150,3 +150,34 @@ def test_divisor_fast_path_verified(mpm):
         print("divisor", d, "fast" if f.value else "ieee")
     assert fast[1.0 / 600] == 1 and fast[1e-4] == 1
     sim.close()
+
+
+@pytest.mark.parametrize("case", ["scale_not_pow2", "huge_dx_lanes"])
+def test_pbmpm_p2g_general_form_bitexact(mpm, orc, case):
+    """P2G folds the fixed-point scale into the mass only for S = 2^k and particles with
+    |m|, |dx|, |D| <= 2^36; everything else takes the general form, per lane. Both cases are
+    compared against the oracle: a scale of 10^5 (every particle general), and a few particles
+    with |dx| = 2^40 scattered through the waves of an otherwise folded pass."""
+    sc = _small_pbmpm(mpm, n=6000, seed=14)
+    if case == "huge_dx_lanes":
+        p = sc["particles"].copy()
+        p["dx"][::97, 0] = np.float32(2.0 ** 40)
+        sc = dict(sc, particles=p)
+    for stage in (1, 2):
+        sim, o = make_pair(mpm, orc, sc, pbmpm_iterations=3, pbmpm_force_divisor=3.0)
+        if case == "scale_not_pow2":
+            sim.uniformsManager.writeFixedPointScale(1e5)
+            o.set_uniforms(sim.uniform_bytes())
+        sim.debug_stage(1, stage)
+        o.run_stage(1, stage)
+        assert_grid_equal(sim, o, f"{case} stage {stage}")
+        sim.close()
+    sim, o = make_pair(mpm, orc, sc, pbmpm_iterations=3, pbmpm_force_divisor=3.0)
+    if case == "scale_not_pow2":
+        sim.uniformsManager.writeFixedPointScale(1e5)
+        o.set_uniforms(sim.uniform_bytes())
+    for s in range(3):
+        sim.step(1, 1)
+        o.step(1, 1)
+        assert_particles_equal(mpm, sim.download(), o.particles(), f"{case} substep {s + 1}")
+    sim.close()

@@ -82,6 +82,7 @@ struct mpm_ctx {
     bool resid_on = false;
     uint64_t resid_substeps = 0;
     std::map<uint64_t, bool> div_verified;   // (divisor, scale) bits -> fixed_div exact for all inputs
+    std::map<uint64_t, bool> quot_verified;  // divisor bits -> quot_i32 exact for all |x| <= 2^100
     bool per_particle_lame = false;
     uint32_t* d_scratch = nullptr;           // one counter for k_verify_div
     // per-kernel event timing (eager launches, one event after every kernel)
@@ -199,20 +200,23 @@ Bufs make_bufs(mpm_ctx* c) {
 }
 
 // Checks on the device, once per (divisor, fixed-point scale) pair, that fixed_div's FMA form
-// gives i32((x / d) * S) for every float x; the kernels use it only for pairs that pass.
-int verify_divisor(mpm_ctx* c, float d, float S, uint32_t* ok) {
+// gives i32((x / d) * S) for every float x (mode 0), or once per divisor that quot_i32 gives
+// i32(x / d) for every float with |x| <= 2^100 (mode 1); the kernels use a form only where it passed.
+int verify_divisor(mpm_ctx* c, float d, float S, int mode, uint32_t* ok) {
+    if (mode == 1) S = 1.0f;
     uint32_t db, sb; std::memcpy(&db, &d, 4); std::memcpy(&sb, &S, 4);
     const uint64_t key = ((uint64_t)db << 32) | sb;
-    auto it = c->div_verified.find(key);
-    if (it != c->div_verified.end()) { *ok = it->second ? 1u : 0u; return MPM_OK; }
+    std::map<uint64_t, bool>& seen = mode == 1 ? c->quot_verified : c->div_verified;
+    auto it = seen.find(key);
+    if (it != seen.end()) { *ok = it->second ? 1u : 0u; return MPM_OK; }
     *ok = 0;
-    if (!(d > 0.0f) || d > 1e30f || d < 1e-30f || !(S > 0.0f) || S > 1e30f) { c->div_verified[key] = false; return MPM_OK; }
+    if (!(d > 0.0f) || d > 1e30f || d < 1e-30f || !(S > 0.0f) || S > 1e30f) { seen[key] = false; return MPM_OK; }
     HIPCHK(c, hipMemsetAsync(c->d_scratch, 0, 4, c->stream));
-    k_verify_fixed_div<<<8192, 256, 0, c->stream>>>(d, 1.0f / d, S, c->d_scratch);
+    k_verify_fixed_div<<<8192, 256, 0, c->stream>>>(d, 1.0f / d, S, mode, c->d_scratch);
     uint32_t bad = 1;
     HIPCHK(c, hipMemcpyAsync(&bad, c->d_scratch, 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->div_verified[key] = bad == 0;
+    seen[key] = bad == 0;
     *ok = bad == 0 ? 1u : 0u;
     return MPM_OK;
 }
@@ -274,7 +278,17 @@ int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, f
     for (int a = 0; a < 3; ++a) P.grav_nz[a] = P.grav_div[a] != 0.0f ? 1u : 0u;
     P.rdt = 1.0f / P.dt;
     int vr;
-    if ((vr = verify_divisor(c, P.dt, P.S, &P.fast_dt))) return vr;
+    if ((vr = verify_divisor(c, P.dt, P.S, 0, &P.fast_dt))) return vr;
+    {
+        int e2 = 0;
+        const bool pow2 = std::frexp(P.S, &e2) == 0.5f && P.S >= 1.0f && P.S <= 65536.0f;
+        bool g_ok = true;
+        for (int a = 0; a < 3; ++a) g_ok = g_ok && std::fabs(P.grav_div[a]) <= 0x1p40f;
+        uint32_t q_ok = 0;
+        if (pow2 && g_ok && !P.interacting && P.dt >= 0x1p-20f && P.dt <= 0x1p20f)
+            if ((vr = verify_divisor(c, P.dt, P.S, 1, &q_ok))) return vr;
+        P.p2g_fold = q_ok;
+    }
     P.wall_mode = g.wall_mode;
     P.wall_cells = g.wall_cells;
     P.n_coll_rec = scene_only ? 0u : (uint32_t)c->n_coll_rec;
@@ -1094,7 +1108,7 @@ int mpm_divisor_fast(mpm_ctx* c, float d, float scale, int* fast) {
     if (!c || !fast) return MPM_E_INVALID;
     HIPCHK(c, hipSetDevice(c->device));
     uint32_t ok = 0;
-    int rc = verify_divisor(c, d, scale, &ok);
+    int rc = verify_divisor(c, d, scale, 0, &ok);
     *fast = (int)ok;
     return rc;
 }

@@ -60,6 +60,8 @@ struct SimParams {
     float grav_div[3];         // grav / force_div (the per-node force of pbmpm.wgsl:137 without a pointer)
     float rdt;                 // RN(1/dt) for fixed_div
     uint32_t fast_dt;          // fixed_div(., dt, rdt, S) verified equal to i32((x/dt) S) for all x
+    uint32_t p2g_fold;         // PBMPM P2G may fold S into the mass (S = 2^k <= 2^16, quot_i32 verified
+                               // for dt, no pointer force, |grav / force_div| <= 2^40, 2^-20 <= dt <= 2^20)
     uint32_t grav_nz[3];       // grav_div component is nonzero (zero ones contribute i32 0 exactly)
     uint32_t wall_mode, wall_cells;
     int32_t k;                 // PBMPM iterations of this substep
@@ -1017,6 +1019,41 @@ MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, const floa
     const bool fast = P.fast_dt != 0;
     const bool inter = P.interacting != 0;
     const int tb = tile_of(s.t0[0] - 1, s.t0[1] - 1, s.t0[2] - 1);
+    // Folded form (S = 2^k): RN(a) 2^k = RN(a 2^k) away from overflow and subnormals, so
+    // i32(RN(RN(w m (dx + aff)) / dt) S) = i32(RN(RN(w (m S) (dx + aff)) / dt)), and likewise the
+    // gravity and mass terms, with the quotient from quot_i32. With |m|, |dx|, |D| <= 2^36 every
+    // dividend stays below 2^100, and a product that would be subnormal unscaled truncates to 0
+    // either way, so the integers are the same bits. (fmaxf passes NaN over; a NaN operand makes
+    // the channels it reaches NaN, i32 0, in both forms.)
+    const float big = fmaxf(fmaxf(fmaxf(fabsf(mass), fabsf(dx.x)), fmaxf(fabsf(dx.y), fabsf(dx.z))),
+                            fmaxf(fmaxf(mabs_max(D.c[0]), mabs_max(D.c[1])), mabs_max(D.c[2])));
+    if (P.p2g_fold && big <= 0x1p36f) {
+        const float mS = mass * P.S;
+#pragma unroll
+        for (int oz = 0; oz < 3; ++oz)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+        for (int ox = 0; ox < 3; ++ox) {
+            const int j = ox + 3 * oy + 9 * oz;
+            const int t = ((ok >> j) & 1u) ? tb + ox + 6 * oy + 36 * oz : 216;
+            const float cw = (s.w[ox].x * s.w[oy].y) * s.w[oz].z;
+            const float wmS = cw * mS;
+            const v3 aff = (ax[ox] + by[oy]) + cz[oz];
+            const v3 num = wmS * (dx + aff);
+            int32_t px = quot_i32(num.x, P.dt, P.rdt);
+            int32_t py = quot_i32(num.y, P.dt, P.rdt);
+            int32_t pz = quot_i32(num.z, P.dt, P.rdt);
+            if (P.grav_nz[0]) px = (int32_t)((uint32_t)px + (uint32_t)f2i((gdiv.x * wmS) * P.dt));
+            if (P.grav_nz[1]) py = (int32_t)((uint32_t)py + (uint32_t)f2i((gdiv.y * wmS) * P.dt));
+            if (P.grav_nz[2]) pz = (int32_t)((uint32_t)pz + (uint32_t)f2i((gdiv.z * wmS) * P.dt));
+            if (MASS) atomicAdd(&T.acc[0][t], f2i(wmS));
+            atomicAdd(&T.acc[1][t], px);
+            atomicAdd(&T.acc[2][t], py);
+            atomicAdd(&T.acc[3][t], pz);
+        }
+        return;
+    }
 #pragma unroll
     for (int oz = 0; oz < 3; ++oz)
 #pragma unroll
@@ -1384,12 +1421,18 @@ __global__ void k_copy_state(Bufs B, uint32_t n) {
 
 // Counts the float inputs x (all 2^32 bit patterns) where fixed_div(x, d, rd, S, true) differs
 // from the reference's i32((x / d) * S).
-__global__ __launch_bounds__(256) void k_verify_fixed_div(float d, float rd, float S, uint32_t* bad) {
+// mode 0: fixed_div's FMA form against i32((x / d) * S) for every float x;
+// mode 1: quot_i32 against i32(x / d) for every float with |x| <= 2^100.
+__global__ __launch_bounds__(256) void k_verify_fixed_div(float d, float rd, float S, int mode, uint32_t* bad) {
     uint32_t cnt = 0;
     const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; u < (1ull << 32); u += gsz) {
         const float x = as_f((uint32_t)u);
-        if (fixed_div(x, d, rd, S, false) != fixed_div(x, d, rd, S, true)) ++cnt;
+        if (mode == 0) {
+            if (fixed_div(x, d, rd, S, false) != fixed_div(x, d, rd, S, true)) ++cnt;
+        } else if (fabs_(x) <= 0x1p100f) {
+            if (f2i(x / d) != quot_i32(x, d, rd)) ++cnt;
+        }
     }
     if (cnt) atomicAdd(bad, cnt);
 }

@@ -78,6 +78,14 @@ MPM_DEV int32_t fixed_div(float x, float d, float rd, float S, bool fast) {
     const float e = __builtin_fmaf(-q0, d, xc);
     return f2i(__builtin_fmaf(e, rd, q0) * S);
 }
+// i32(x / d) for |x| <= 2^100 by the same FMA-corrected quotient without the clamp. The host
+// uses it only for a divisor that a device check over every float with |x| <= 2^100 found equal
+// to i32(x / d) (k_verify_fixed_div, mode 1).
+MPM_DEV int32_t quot_i32(float x, float d, float rd) {
+    const float q0 = x * rd;
+    const float e = __builtin_fmaf(-q0, d, x);
+    return f2i(__builtin_fmaf(e, rd, q0));
+}
 
 MPM_DEV float scale_pow2(float p, int n) {
     if (n > 127) { p = p * as_f(0x7f000000u); n -= 127; if (n > 127) n = 127; }
@@ -152,6 +160,7 @@ MPM_DEV float dot(v3 a, v3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
 MPM_DEV v3 cross(v3 a, v3 b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 MPM_DEV float length(v3 a) { return __builtin_sqrtf(dot(a, a)); }
 MPM_DEV v3 normalize(v3 a) { float l = length(a); return a / l; }
+MPM_DEV float mabs_max(v3 a) { return fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z)); }
 MPM_DEV v3 vmin(v3 a, v3 b) { return mk(a.x < b.x ? a.x : b.x, a.y < b.y ? a.y : b.y, a.z < b.z ? a.z : b.z); }
 MPM_DEV v3 vmax(v3 a, v3 b) { return mk(a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y, a.z > b.z ? a.z : b.z); }
 

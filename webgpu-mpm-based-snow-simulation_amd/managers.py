@@ -54,8 +54,6 @@ class GpuUniformsBufferManager:
     def writeMeshMinCoords(self, mn): self._f(176, mn)
     def writeMeshMaxCoords(self, mx): self._f(192, mx)
     def writeGridResolution(self, res): self._u(208, res)
-    def writeMeshMinCoords(self, mn): self._f(176, mn)
-    def writeMeshMaxCoords(self, mx): self._f(192, mx)
     def writeColliderMinCoords(self, mn): self._f(224, mn)
     def writeColliderMaxCoords(self, mx): self._f(240, mx)
     def writeColliderTransformMat(self, m): self._f(256, m)
