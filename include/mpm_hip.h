@@ -47,7 +47,10 @@ typedef struct mpm_config {
     float    fixed_point_scale;    /* uniforms fixedPointScale @4 (GpuSnowPipelineRunner.svelte.ts:28, 65536) */
     float    dt;                   /* uniforms simulationTimestep @0 */
     uint32_t max_blocks;           /* sparseGridPrelude.wgsl:7; 0 = derived from grid and N */
-    uint32_t hash_map_size;        /* sparseGridPrelude.wgsl:6; 0 = derived (prime > 2 max_blocks) */
+    uint32_t hash_map_size;        /* sparseGridPrelude.wgsl:6; nonzero: open-addressing block table of
+                                      this size with the reference's probing (probe failures possible);
+                                      0: blocks addressed directly by block key (same allocated set,
+                                      no table, no probe failures) */
     uint32_t max_probes;           /* sparseGridPrelude.wgsl:8 (100) */
     int32_t  pbmpm_iterations;     /* GpuMpmPipelineManager.ts:445 (3) */
     float    pbmpm_force_divisor;  /* pbmpm.wgsl:137 (3.0) */

@@ -77,11 +77,13 @@ def test_particles_on_and_outside_the_walls(mpm, orc):
             sim.close()
 
 
-def test_block_capacity_drops_are_counted(mpm):
+@pytest.mark.parametrize("table", [0, 4093])
+def test_block_capacity_drops_are_counted(mpm, table):
     """More stencil blocks than max_blocks: the overflow is dropped and counted (mapAffectedBlocks
-    .wgsl:27-31) and the step still completes with finite state."""
+    .wgsl:27-31) and the step still completes with finite state (direct block addressing and the
+    open-addressing table)."""
     sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=20000, seed=21)
-    with mpm.MpmSimulation(20000, **dict(sc["sim"], max_blocks=200)) as sim:
+    with mpm.MpmSimulation(20000, **dict(sc["sim"], max_blocks=200, hash_map_size=table)) as sim:
         sim.upload(sc["particles"])
         sim.step(1, 2)
         st = sim.stats()

@@ -79,11 +79,12 @@ def _small_pbmpm(mpm, n=4096, seed=11):
     return mpm.scenes.scene("pbmpm_two_spheres", n_particles=n, seed=seed)
 
 
-@pytest.mark.parametrize("k", [1, 3, 4])
-def test_pbmpm_stage_grids_bitexact(mpm, orc, k):
+# hash_map_size 0: blocks addressed directly by key; nonzero: the reference's open-addressing table
+@pytest.mark.parametrize("k,table", [(1, 0), (3, 0), (4, 0), (3, 65537)])
+def test_pbmpm_stage_grids_bitexact(mpm, orc, k, table):
     sc = _small_pbmpm(mpm)
     for stage in range(0, k + 1):
-        sim, o = make_pair(mpm, orc, sc, pbmpm_iterations=k, pbmpm_force_divisor=float(k))
+        sim, o = make_pair(mpm, orc, sc, pbmpm_iterations=k, pbmpm_force_divisor=float(k), hash_map_size=table)
         sim.debug_stage(1, stage)
         o.run_stage(1, stage)
         gb = sim.dump_blocks()
@@ -99,9 +100,10 @@ def test_pbmpm_stage_grids_bitexact(mpm, orc, k):
         sim.close()
 
 
-def test_pbmpm_substeps_bitexact(mpm, orc):
+@pytest.mark.parametrize("table", [0, 65537])
+def test_pbmpm_substeps_bitexact(mpm, orc, table):
     sc = _small_pbmpm(mpm, n=8192, seed=12)
-    sim, o = make_pair(mpm, orc, sc)
+    sim, o = make_pair(mpm, orc, sc, hash_map_size=table)
     for s in range(6):
         sim.step(1, 1)
         o.step(1, 1)

@@ -40,6 +40,8 @@ struct mpm_ctx {
     float* relax = nullptr;
     uint32_t *key = nullptr, *mask = nullptr, *perm = nullptr, *kcount = nullptr, *kmask = nullptr, *kcursor = nullptr;
     uint32_t *tile_sums = nullptr, *tile_ofs = nullptr;
+    uint32_t *kalloc = nullptr, *kslot = nullptr;
+    bool direct_blocks = false;       // block set by direct key addressing (hash_map_size == 0)
     uint32_t *run_key = nullptr, *run_start = nullptr, *run_count = nullptr, *run_mask = nullptr;
     uint2* chunks = nullptr;
     uint32_t *table = nullptr, *slot_val = nullptr, *nbr = nullptr, *block_key = nullptr;
@@ -185,6 +187,7 @@ Bufs make_bufs(mpm_ctx* c) {
     B.relax = c->relax; B.key = c->key; B.mask = c->mask; B.perm = c->perm;
     B.kcount = c->kcount; B.kmask = c->kmask; B.kcursor = c->kcursor;
     B.tile_sums = c->tile_sums; B.tile_ofs = c->tile_ofs;
+    B.kalloc = c->kalloc; B.kslot = c->kslot;
     B.run_key = c->run_key; B.run_start = c->run_start; B.run_count = c->run_count; B.run_mask = c->run_mask;
     B.chunks = c->chunks;
     B.table = c->table; B.slot_val = c->slot_val; B.nbr = c->nbr; B.block_key = c->block_key;
@@ -291,6 +294,7 @@ int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, f
     }
     P.wall_mode = g.wall_mode;
     P.wall_cells = g.wall_cells;
+    P.direct_blocks = (!scene_only && c->direct_blocks) ? 1u : 0u;
     P.n_coll_rec = scene_only ? 0u : (uint32_t)c->n_coll_rec;
     P.n_coll_tri = scene_only ? 0u : (uint32_t)c->n_coll_tri;
     P.has_collider = (!scene_only && c->n_coll_rec > 0 && c->n_coll_words > 0) ? 1u : 0u;
@@ -329,21 +333,26 @@ uint32_t grid_for(uint64_t work, uint32_t cap_wg) {
 // One substep (or a debug prefix of it) enqueued on the stream. Returns the stage reached.
 void enqueue_binning(mpm_ctx* c, const SimParams& P, const Bufs& B) {
     hipStream_t s = c->stream;
-    const uint32_t gk = grid_for(std::max<uint64_t>(c->n, c->cap), 8192);
+    const uint32_t gk = grid_for(P.direct_blocks ? (uint64_t)c->n : std::max<uint64_t>(c->n, c->cap), 8192);
     kt(c, -1);
     k_keys_count<<<gk, 256, 0, s>>>(P, B); kt(c, KID_KEYS);
     k_scan_tiles<<<c->ntiles, 256, 0, s>>>(P, B); kt(c, KID_SCAN_TILES);
-    k_scan_top<<<1, 256, 0, s>>>(P, B, c->ntiles); kt(c, KID_SCAN_TOP);
-    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P, B); kt(c, KID_SCAN_APPLY);
+    const int own_ofs = c->ntiles <= 1024u ? 1 : 0;
+    if (!own_ofs) { k_scan_top<<<1, 256, 0, s>>>(P, B, c->ntiles); kt(c, KID_SCAN_TOP); }
+    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P, B, c->ntiles, own_ofs); kt(c, KID_SCAN_APPLY);
     k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P, B); kt(c, KID_SCATTER);
     const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 4096);
-    k_insert<<<gi, 256, 0, s>>>(P, B); kt(c, KID_INSERT);
-    k_compact<<<grid_for(c->cap, 1u << 30), 256, 0, s>>>(P, B); kt(c, KID_COMPACT);
+    if (!P.direct_blocks) {
+        k_insert<<<gi, 256, 0, s>>>(P, B); kt(c, KID_INSERT);
+        k_compact<<<grid_for(c->cap, 1u << 30), 256, 0, s>>>(P, B); kt(c, KID_COMPACT);
+    }
     k_nbr<<<gi, 256, 0, s>>>(P, B); kt(c, KID_NBR);
 }
 
+// Workgroups of the chunk kernels: four times the resident count (4 per CU at 128 VGPRs), so the
+// dispatcher hands the rest out as workgroups finish and balances uneven chunks.
 uint32_t chunk_grid(mpm_ctx* c) {
-    uint32_t g = c->cfg.grid_workgroups ? c->cfg.grid_workgroups : (uint32_t)c->n_cu * 4u;
+    uint32_t g = c->cfg.grid_workgroups ? c->cfg.grid_workgroups : (uint32_t)c->n_cu * 16u;
     return std::min<uint32_t>(g, std::max<uint32_t>(c->max_chunks, 1u));
 }
 
@@ -461,11 +470,12 @@ int alloc_buffers(mpm_ctx* c) {
         (rc = dalloc(c, &c->relax, c->stride)) || (rc = dalloc(c, &c->key, c->stride)) ||
         (rc = dalloc(c, &c->mask, c->stride)) || (rc = dalloc(c, &c->perm, c->stride)) ||
         (rc = dalloc(c, &c->kcount, K)) || (rc = dalloc(c, &c->kmask, K)) || (rc = dalloc(c, &c->kcursor, K)) ||
-        (rc = dalloc(c, &c->tile_sums, 3 * (size_t)c->ntiles)) || (rc = dalloc(c, &c->tile_ofs, 3 * (size_t)c->ntiles)) ||
+        (rc = dalloc(c, &c->tile_sums, 4 * (size_t)c->ntiles)) || (rc = dalloc(c, &c->tile_ofs, 4 * (size_t)c->ntiles)) ||
+        (rc = dalloc(c, &c->kalloc, K)) || (rc = dalloc(c, &c->kslot, K)) ||
         (rc = dalloc(c, &c->run_key, c->max_runs)) || (rc = dalloc(c, &c->run_start, c->max_runs)) ||
         (rc = dalloc(c, &c->run_count, c->max_runs)) || (rc = dalloc(c, &c->run_mask, c->max_runs)) ||
         (rc = dalloc(c, &c->chunks, c->max_chunks)) ||
-        (rc = dalloc(c, &c->table, c->cap)) || (rc = dalloc(c, &c->slot_val, c->cap)) ||
+        (rc = dalloc(c, &c->table, c->direct_blocks ? 1 : c->cap)) || (rc = dalloc(c, &c->slot_val, c->direct_blocks ? 1 : c->cap)) ||
         (rc = dalloc(c, &c->nbr, 27 * (size_t)c->max_runs)) ||
         (rc = dalloc(c, &c->block_key, c->max_blocks)) ||
         (rc = dalloc(c, &c->grid[0], 64 * (size_t)c->max_blocks)) || (rc = dalloc(c, &c->grid[1], 64 * (size_t)c->max_blocks)) ||
@@ -476,6 +486,7 @@ int alloc_buffers(mpm_ctx* c) {
     // zero-initialised key-space accumulators and counters (kernels keep them zero afterwards)
     k_fill_u32<<<grid_for(K, 4096), 256, 0, c->stream>>>(c->kcount, 0u, (uint32_t)K);
     k_fill_u32<<<grid_for(K, 4096), 256, 0, c->stream>>>(c->kmask, 0u, (uint32_t)K);
+    k_fill_u32<<<grid_for(K, 4096), 256, 0, c->stream>>>(c->kalloc, 0u, (uint32_t)K);
     k_iota<<<grid_for(c->stride, 4096), 256, 0, c->stream>>>(c->pid[0], c->stride);
     k_fill_u32<<<1, 256, 0, c->stream>>>(c->counters, 0u, C_NCOUNTERS);
     if (hipStreamSynchronize(c->stream) != hipSuccess) { (void)hipGetLastError(); return MPM_E_HIP; }
@@ -506,7 +517,7 @@ void enqueue_ens_substep(mpm_ctx* c, bool full) {
     k_ens_keys_count<<<gk, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_KEYS);
     k_scan_tiles<<<c->ntiles, 256, 0, s>>>(P0, B); kt(c, KID_SCAN_TILES);
     k_scan_top<<<1, 256, 0, s>>>(P0, B, c->ntiles); kt(c, KID_SCAN_TOP);
-    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P0, B); kt(c, KID_SCAN_APPLY);
+    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P0, B, c->ntiles, 0); kt(c, KID_SCAN_APPLY);
     k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P0, B); kt(c, KID_SCATTER);
     const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 8192);
     k_ens_insert<<<gi, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_INSERT);
@@ -644,6 +655,7 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     uint64_t mb = cfg->max_blocks ? cfg->max_blocks : std::min<uint64_t>(K, std::max<uint64_t>(8ull * c->n, 64));
     c->max_blocks = (uint32_t)mb;
     c->cap = cfg->hash_map_size ? cfg->hash_map_size : next_prime((uint32_t)std::min<uint64_t>(2ull * mb + 1, 0xFFFFFF00ull));
+    c->direct_blocks = cfg->hash_map_size == 0;
     c->max_runs = (uint32_t)std::min<uint64_t>(c->n, K);
     c->max_chunks = (c->n + c->chunk - 1) / c->chunk + c->max_runs;
     c->ntiles = (uint32_t)((K + 1023) / 1024);

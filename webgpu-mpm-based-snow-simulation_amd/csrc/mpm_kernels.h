@@ -60,6 +60,7 @@ struct SimParams {
     float grav_div[3];         // grav / force_div (the per-node force of pbmpm.wgsl:137 without a pointer)
     float rdt;                 // RN(1/dt) for fixed_div
     uint32_t fast_dt;          // fixed_div(., dt, rdt, S) verified equal to i32((x/dt) S) for all x
+    uint32_t direct_blocks;    // block set by direct key addressing (no hash table)
     uint32_t p2g_fold;         // PBMPM P2G may fold S into the mass (S = 2^k <= 2^16, quot_i32 verified
                                // for dt, no pointer force, |grav / force_div| <= 2^40, 2^-20 <= dt <= 2^20)
     uint32_t grav_nz[3];       // grav_div component is nonzero (zero ones contribute i32 0 exactly)
@@ -83,7 +84,9 @@ struct Bufs {
     float* relax;
     uint32_t* key; uint32_t* mask; uint32_t* perm;
     uint32_t* kcount; uint32_t* kmask; uint32_t* kcursor;
-    uint32_t* tile_sums; uint32_t* tile_ofs;     // 3 per tile
+    uint32_t* tile_sums; uint32_t* tile_ofs;     // 4 per tile
+    uint32_t* kalloc;          // per key: block touched by some particle's stencil (direct addressing)
+    uint32_t* kslot;           // per key: dense block index or EMPTY (direct addressing)
     uint32_t* run_key; uint32_t* run_start; uint32_t* run_count; uint32_t* run_mask;
     uint2* chunks;
     uint32_t* table; uint32_t* slot_val; uint32_t* nbr; uint32_t* block_key;
@@ -207,21 +210,27 @@ MPM_DEV uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 // exclusive scan of three u32 streams over a 256-thread workgroup
-MPM_DEV void block_excl_scan3(uint32_t v[3], uint32_t ex[3], uint32_t tot[3], uint32_t (*wt)[3]) {
+MPM_DEV void block_excl_scan4(uint32_t v[4], uint32_t ex[4], uint32_t tot[4], uint32_t (*wt)[4]) {
     const int w = threadIdx.x >> 6, l = lane_id();
-    uint32_t inc[3];
+    uint32_t inc[4];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) inc[c] = wave_incl_scan(v[c]);
-    if (l == 63) { wt[w][0] = inc[0]; wt[w][1] = inc[1]; wt[w][2] = inc[2]; }
+    for (int c = 0; c < 4; ++c) inc[c] = wave_incl_scan(v[c]);
+    if (l == 63) { wt[w][0] = inc[0]; wt[w][1] = inc[1]; wt[w][2] = inc[2]; wt[w][3] = inc[3]; }
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < 4; ++c) {
         uint32_t pre = 0, all = 0;
         for (int k = 0; k < 4; ++k) { if (k < w) pre += wt[k][c]; all += wt[k][c]; }
         ex[c] = pre + inc[c] - v[c];
         tot[c] = all;
     }
     __syncthreads();
+}
+// Sum over the workgroup of v[4] (every thread gets the totals).
+MPM_DEV void block_sum4(uint32_t v[4], uint32_t (*wt)[4]) {
+    uint32_t ex[4], tot[4];
+    block_excl_scan4(v, ex, tot, wt);
+    for (int c = 0; c < 4; ++c) v[c] = tot[c];
 }
 
 // ================================================================== binning
@@ -273,14 +282,21 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Key of the block at offset j (0..26, x fastest, -1..1 per axis) from block (bx, by, bz).
+MPM_DEV uint32_t block_key_at(const SimParams& P, int bx, int by, int bz, int j) {
+    bx += j % 3 - 1; by += (j / 3) % 3 - 1; bz += j / 9 - 1;
+    return P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+}
 // Per particle: own-block key, touched-neighbour mask; per key: particle count and OR of masks
 // (countParticlesPerBlock.wgsl:7-23 + the block set of mapAffectedBlocks.wgsl:64-86). Atomics are
 // aggregated per wave over lanes that share a key (input is in the previous substep's sorted order).
-// Also resets the hash table (clearHashMap.wgsl:5-19) and the per-substep counters.
+// Hash-table mode also resets the table (clearHashMap.wgsl:5-19); direct mode instead flags every
+// touched block in kalloc (plain stores of 1, one lane per neighbour). Resets the per-substep counters.
 __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
     const uint32_t gsz = gridDim.x * blockDim.x;
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t s = gi; s < P.cap; s += gsz) B.table[s] = EMPTY;
+    if (!P.direct_blocks)
+        for (uint32_t s = gi; s < P.cap; s += gsz) B.table[s] = EMPTY;
     if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; }
     const uint32_t n = P.n;
     const uint32_t nloop = (n + gsz - 1) / gsz;
@@ -296,6 +312,10 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
             if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
                 key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
                 valid = true;
+            } else if (P.direct_blocks && mask) {
+                // own block outside the block grid: the in-range blocks its stencil touches
+                for (int j = 0; j < 27; ++j)
+                    if ((mask >> j) & 1u) B.kalloc[block_key_at(P, bx, by, bz, j)] = 1u;
             }
             B.key[i] = key; B.mask[i] = mask;
         }
@@ -310,54 +330,81 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
                 atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
                 atomicOr(&B.kmask[lk], m);
             }
+            if (P.direct_blocks) {
+                const int j = lane_id();
+                if (j < 27 && ((m >> j) & 1u)) {
+                    int b[3]; block_coords(P, lk, b);
+                    B.kalloc[block_key_at(P, b[0], b[1], b[2], j)] = 1u;
+                }
+            }
             active &= ~peers;
         }
     }
 }
 
-// Scan over the key space, 1024 keys per tile. Streams: particle count, run flag, chunk count.
-__global__ __launch_bounds__(256) void k_scan_tiles(SimParams P, Bufs B) {
-    __shared__ uint32_t wt[4][3];
-    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
-    uint32_t v[3] = {0, 0, 0};
-    for (int q = 0; q < 4; ++q) {
-        uint32_t k = base + q;
-        uint32_t c = k < P.K ? B.kcount[k] : 0u;
-        v[0] += c; v[1] += c ? 1u : 0u; v[2] += (c + P.chunk - 1) / P.chunk;
-    }
-    uint32_t ex[3], tot[3];
-    block_excl_scan3(v, ex, tot, wt);
-    if (threadIdx.x == 0) { B.tile_sums[3 * blockIdx.x] = tot[0]; B.tile_sums[3 * blockIdx.x + 1] = tot[1]; B.tile_sums[3 * blockIdx.x + 2] = tot[2]; }
+// Scan over the key space, 1024 keys per tile. Streams: particle count, run flag, chunk count,
+// allocated-block flag (direct addressing; 0 in hash-table mode).
+MPM_DEV void key_streams(const SimParams& P, const Bufs& B, uint32_t k, uint32_t v[4]) {
+    const uint32_t c = k < P.K ? B.kcount[k] : 0u;
+    v[0] += c; v[1] += c ? 1u : 0u; v[2] += (c + P.chunk - 1) / P.chunk;
+    if (P.direct_blocks && k < P.K) v[3] += B.kalloc[k];
 }
+__global__ __launch_bounds__(256) void k_scan_tiles(SimParams P, Bufs B) {
+    __shared__ uint32_t wt[4][4];
+    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t v[4] = {0, 0, 0, 0};
+    for (int q = 0; q < 4; ++q) key_streams(P, B, base + q, v);
+    block_sum4(v, wt);
+    if (threadIdx.x == 0) for (int c = 0; c < 4; ++c) B.tile_sums[4 * blockIdx.x + c] = v[c];
+}
+// Totals of the four streams: counters for the later passes. Direct addressing also settles the
+// block count and the blocks beyond max_blocks (dropped) here.
+MPM_DEV void publish_totals(const SimParams& P, const Bufs& B, const uint32_t tot[4]) {
+    B.counters[C_NVALID] = tot[0]; B.counters[C_NRUNS] = tot[1]; B.counters[C_NCHUNKS] = tot[2];
+    if (P.direct_blocks) { B.counters[C_NBLOCKS] = tot[3]; B.counters[C_DROPS] = tot[3] > P.max_blocks ? tot[3] - P.max_blocks : 0u; }
+}
+// Tile offsets when there are more tiles than one workgroup of k_scan_apply sums cheaply.
 __global__ __launch_bounds__(256) void k_scan_top(SimParams P, Bufs B, uint32_t ntiles) {
-    __shared__ uint32_t wt[4][3];
+    __shared__ uint32_t wt[4][4];
     const uint32_t per = (ntiles + 255u) / 256u;
     const uint32_t t0 = threadIdx.x * per;
-    uint32_t v[3] = {0, 0, 0};
-    for (uint32_t t = t0; t < t0 + per && t < ntiles; ++t) { v[0] += B.tile_sums[3 * t]; v[1] += B.tile_sums[3 * t + 1]; v[2] += B.tile_sums[3 * t + 2]; }
-    uint32_t ex[3], tot[3];
-    block_excl_scan3(v, ex, tot, wt);
-    for (uint32_t t = t0; t < t0 + per && t < ntiles; ++t) {
-        B.tile_ofs[3 * t] = ex[0]; B.tile_ofs[3 * t + 1] = ex[1]; B.tile_ofs[3 * t + 2] = ex[2];
-        ex[0] += B.tile_sums[3 * t]; ex[1] += B.tile_sums[3 * t + 1]; ex[2] += B.tile_sums[3 * t + 2];
-    }
-    if (threadIdx.x == 0) { B.counters[C_NVALID] = tot[0]; B.counters[C_NRUNS] = tot[1]; B.counters[C_NCHUNKS] = tot[2]; }
+    uint32_t v[4] = {0, 0, 0, 0};
+    for (uint32_t t = t0; t < t0 + per && t < ntiles; ++t) for (int c = 0; c < 4; ++c) v[c] += B.tile_sums[4 * t + c];
+    uint32_t ex[4], tot[4];
+    block_excl_scan4(v, ex, tot, wt);
+    for (uint32_t t = t0; t < t0 + per && t < ntiles; ++t)
+        for (int c = 0; c < 4; ++c) { B.tile_ofs[4 * t + c] = ex[c]; ex[c] += B.tile_sums[4 * t + c]; }
+    if (threadIdx.x == 0) publish_totals(P, B, tot);
 }
-// Writes run descriptors (particle blocks), work chunks and scatter cursors; zeroes the per-key
-// accumulators for the next substep.
-__global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B) {
-    __shared__ uint32_t wt[4][3];
-    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
-    uint32_t cnt[4], v[3] = {0, 0, 0};
-    for (int q = 0; q < 4; ++q) {
-        uint32_t k = base + q;
-        uint32_t c = k < P.K ? B.kcount[k] : 0u;
-        cnt[q] = c;
-        v[0] += c; v[1] += c ? 1u : 0u; v[2] += (c + P.chunk - 1) / P.chunk;
+// Writes run descriptors (particle blocks), work chunks, scatter cursors and (direct addressing)
+// the dense block index of every key; zeroes the per-key accumulators for the next substep.
+// own_ofs: the tile offsets are summed here from tile_sums (few tiles), without k_scan_top.
+__global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B, uint32_t ntiles, int own_ofs) {
+    __shared__ uint32_t wt[4][4];
+    uint32_t tofs[4];
+    if (own_ofs) {
+        uint32_t v[4] = {0, 0, 0, 0}, all[4] = {0, 0, 0, 0};
+        for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x)
+            for (int c = 0; c < 4; ++c) { const uint32_t x = B.tile_sums[4 * t + c]; all[c] += x; if (t < blockIdx.x) v[c] += x; }
+        block_sum4(v, wt);
+        block_sum4(all, wt);
+        for (int c = 0; c < 4; ++c) tofs[c] = v[c];
+        if (blockIdx.x == 0 && threadIdx.x == 0) publish_totals(P, B, all);
+    } else {
+        for (int c = 0; c < 4; ++c) tofs[c] = B.tile_ofs[4 * blockIdx.x + c];
     }
-    uint32_t ex[3], tot[3];
-    block_excl_scan3(v, ex, tot, wt);
-    uint32_t o0 = ex[0] + B.tile_ofs[3 * blockIdx.x], o1 = ex[1] + B.tile_ofs[3 * blockIdx.x + 1], o2 = ex[2] + B.tile_ofs[3 * blockIdx.x + 2];
+    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    uint32_t cnt[4], alc[4], v[4] = {0, 0, 0, 0};
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t k = base + q;
+        const uint32_t c = k < P.K ? B.kcount[k] : 0u;
+        cnt[q] = c;
+        alc[q] = (P.direct_blocks && k < P.K) ? B.kalloc[k] : 0u;
+        v[0] += c; v[1] += c ? 1u : 0u; v[2] += (c + P.chunk - 1) / P.chunk; v[3] += alc[q];
+    }
+    uint32_t ex[4], tot[4];
+    block_excl_scan4(v, ex, tot, wt);
+    uint32_t o0 = ex[0] + tofs[0], o1 = ex[1] + tofs[1], o2 = ex[2] + tofs[2], o3 = ex[3] + tofs[3];
     for (int q = 0; q < 4; ++q) {
         uint32_t k = base + q;
         if (k >= P.K) break;
@@ -369,6 +416,15 @@ __global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B) {
             for (uint32_t j = 0; j < nch; ++j) B.chunks[o2 + j] = make_uint2(o1, j);
             B.kcount[k] = 0u; B.kmask[k] = 0u;
             o0 += c; o1 += 1; o2 += nch;
+        }
+        if (P.direct_blocks) {
+            uint32_t slot = EMPTY;
+            if (alc[q]) {
+                if (o3 < P.max_blocks) { slot = o3; B.block_key[o3] = k; }
+                B.kalloc[k] = 0u;
+                o3 += 1;
+            }
+            B.kslot[k] = slot;
         }
     }
 }
@@ -491,7 +547,13 @@ __global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) {
     for (uint32_t t = gi; t < nruns * 27u; t += gsz) {
         const uint32_t r = t / 27u, j = t % 27u;
         int b[3]; block_coords(P, B.run_key[r], b);
-        B.nbr[t] = table_lookup(P, B, b[0] + (int)(j % 3u) - 1, b[1] + (int)((j / 3u) % 3u) - 1, b[2] + (int)(j / 9u) - 1);
+        const int x = b[0] + (int)(j % 3u) - 1, y = b[1] + (int)((j / 3u) % 3u) - 1, z = b[2] + (int)(j / 9u) - 1;
+        if (P.direct_blocks) {
+            const bool in = x >= 0 && y >= 0 && z >= 0 && x < P.nb[0] && y < P.nb[1] && z < P.nb[2];
+            B.nbr[t] = in ? B.kslot[P.kbase + (uint32_t)x + (uint32_t)P.nb[0] * ((uint32_t)y + (uint32_t)P.nb[1] * (uint32_t)z)] : EMPTY;
+        } else {
+            B.nbr[t] = table_lookup(P, B, x, y, z);
+        }
     }
     const uint32_t ncell = n_blocks_of(P, B) * 64u;
     for (uint32_t c = gi; c < ncell; c += gsz) { B.grid0[c] = make_int4(0, 0, 0, 0); B.gmass[c] = 0; }
