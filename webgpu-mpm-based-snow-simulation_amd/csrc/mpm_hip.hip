@@ -85,6 +85,7 @@ struct mpm_ctx {
     uint64_t resid_substeps = 0;
     std::map<uint64_t, bool> div_verified;   // (divisor, scale) bits -> fixed_div exact for all inputs
     std::map<uint64_t, bool> quot_verified;  // divisor bits -> quot_i32 exact for all |x| <= 2^100
+    std::map<uint64_t, bool> qdiv_verified;  // divisor bits -> qdiv bit-exact for all |x| <= 2^100
     bool per_particle_lame = false;
     uint32_t* d_scratch = nullptr;           // one counter for k_verify_div
     // per-kernel event timing (eager launches, one event after every kernel)
@@ -204,12 +205,13 @@ Bufs make_bufs(mpm_ctx* c) {
 
 // Checks on the device, once per (divisor, fixed-point scale) pair, that fixed_div's FMA form
 // gives i32((x / d) * S) for every float x (mode 0), or once per divisor that quot_i32 gives
-// i32(x / d) for every float with |x| <= 2^100 (mode 1); the kernels use a form only where it passed.
+// i32(x / d) (mode 1) or qdiv gives x / d bit for bit (mode 2) for every float with |x| <= 2^100;
+// the kernels use a form only where it passed.
 int verify_divisor(mpm_ctx* c, float d, float S, int mode, uint32_t* ok) {
-    if (mode == 1) S = 1.0f;
+    if (mode != 0) S = 1.0f;
     uint32_t db, sb; std::memcpy(&db, &d, 4); std::memcpy(&sb, &S, 4);
     const uint64_t key = ((uint64_t)db << 32) | sb;
-    std::map<uint64_t, bool>& seen = mode == 1 ? c->quot_verified : c->div_verified;
+    std::map<uint64_t, bool>& seen = mode == 2 ? c->qdiv_verified : (mode == 1 ? c->quot_verified : c->div_verified);
     auto it = seen.find(key);
     if (it != seen.end()) { *ok = it->second ? 1u : 0u; return MPM_OK; }
     *ok = 0;
@@ -285,12 +287,19 @@ int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, f
     {
         int e2 = 0;
         const bool pow2 = std::frexp(P.S, &e2) == 0.5f && P.S >= 1.0f && P.S <= 65536.0f;
-        bool g_ok = true;
-        for (int a = 0; a < 3; ++a) g_ok = g_ok && std::fabs(P.grav_div[a]) <= 0x1p40f;
+        const bool g_ok = P.grav_div[0] == 0.0f && P.grav_div[2] == 0.0f && std::fabs(P.grav_div[1]) <= 0x1p40f;
         uint32_t q_ok = 0;
         if (pow2 && g_ok && !P.interacting && P.dt >= 0x1p-20f && P.dt <= 0x1p20f)
             if ((vr = verify_divisor(c, P.dt, P.S, 1, &q_ok))) return vr;
         P.p2g_fold = q_ok;
+        uint32_t cd_ok = 1;
+        for (int a = 0; a < 3 && cd_ok; ++a) {
+            P.rcd[a] = 1.0f / P.cd[a];
+            uint32_t ok = 0;
+            if ((vr = verify_divisor(c, P.cd[a], 1.0f, 2, &ok))) return vr;
+            cd_ok = ok;
+        }
+        P.fast_cd = cd_ok;
     }
     P.wall_mode = g.wall_mode;
     P.wall_cells = g.wall_cells;

@@ -61,8 +61,11 @@ struct SimParams {
     float rdt;                 // RN(1/dt) for fixed_div
     uint32_t fast_dt;          // fixed_div(., dt, rdt, S) verified equal to i32((x/dt) S) for all x
     uint32_t direct_blocks;    // block set by direct key addressing (no hash table)
+    float rcd[3];              // RN(1 / cell dims)
+    uint32_t fast_cd;          // qdiv(., cd[a], rcd[a]) verified equal to x / cd[a] for all |x| <= 2^100
     uint32_t p2g_fold;         // PBMPM P2G may fold S into the mass (S = 2^k <= 2^16, quot_i32 verified
-                               // for dt, no pointer force, |grav / force_div| <= 2^40, 2^-20 <= dt <= 2^20)
+                               // for dt, no pointer force, gravity along y only with |grav_y / force_div|
+                               // <= 2^40, 2^-20 <= dt <= 2^20)
     uint32_t grav_nz[3];       // grav_div component is nonzero (zero ones contribute i32 0 exactly)
     uint32_t wall_mode, wall_cells;
     int32_t k;                 // PBMPM iterations of this substep
@@ -134,10 +137,21 @@ MPM_DEV void cell_number(const SimParams& P, v3 pos, int c[3]) {
 MPM_DEV bool in_range(const SimParams& P, int x, int y, int z) {
     return x >= 0 && y >= 0 && z >= 0 && x < P.res[0] && y < P.res[1] && z < P.res[2];
 }
+// a / cell dims for |a| <= 2^100 (offsets inside the grid, weight gradients): the verified FMA
+// quotient where it is exact, else the IEEE division.
+MPM_DEV v3 div_cd(const SimParams& P, v3 a) {
+    if (P.fast_cd) return mk(qdiv(a.x, P.cd[0], P.rcd[0]), qdiv(a.y, P.cd[1], P.rcd[1]), qdiv(a.z, P.cd[2], P.rcd[2]));
+    return cdiv(a, mk(P.cd[0], P.cd[1], P.cd[2]));
+}
+// cell_number for a binned particle (position inside the grid's block range).
+MPM_DEV void cell_number_in(const SimParams& P, v3 pos, int c[3]) {
+    v3 q = div_cd(P, pos - mk(P.gmin[0], P.gmin[1], P.gmin[2]));
+    c[0] = f2i(q.x); c[1] = f2i(q.y); c[2] = f2i(q.z);
+}
 MPM_DEV v3 frac_pos(const SimParams& P, v3 pos, const int c[3]) {
     v3 cdv = mk(P.cd[0], P.cd[1], P.cd[2]);
     v3 minPos = mk(P.gmin[0], P.gmin[1], P.gmin[2]) + cmul(cdv, mk((float)c[0], (float)c[1], (float)c[2]));
-    return cdiv(pos - minPos, cdv);
+    return div_cd(P, pos - minPos);
 }
 MPM_DEV void bspline(v3 f, v3 w[3]) {
     v3 a = mk(1.0f, 1.0f, 1.0f) - f;
@@ -888,7 +902,7 @@ MPM_DEV bool sticky_cell(const SimParams& P, int x, int y, int z) {
 struct Tile {
     float4 vel[216];       // node velocity; w = 0 node outside the grid or not allocated,
                            // 1 allocated without mass, 2 allocated with mass (G2P reads it)
-    int acc[4][217];       // P2G accumulators: mass, px, py, pz (+ one sink entry for skipped nodes)
+    int acc[4][216];       // P2G accumulators: mass, px, py, pz (those of unallocated nodes are never flushed)
     int node[216];         // global cell index of the node, -1 if out of range / not allocated
     uint32_t nbr[27];
     int base[3];
@@ -973,7 +987,7 @@ MPM_DEV void tile_load_vel_updated(const SimParams& P, Tile& T, const int4* __re
     }
 }
 MPM_DEV void tile_zero_acc(Tile& T) {
-    for (int t = threadIdx.x; t < 217; t += blockDim.x) { T.acc[0][t] = 0; T.acc[1][t] = 0; T.acc[2][t] = 0; T.acc[3][t] = 0; }
+    for (int t = threadIdx.x; t < 216; t += blockDim.x) { T.acc[0][t] = 0; T.acc[1][t] = 0; T.acc[2][t] = 0; T.acc[3][t] = 0; }
 }
 MPM_DEV void tile_flush(Tile& T, int4* __restrict__ g) {
     for (int t = threadIdx.x; t < 216; t += blockDim.x) {
@@ -1010,7 +1024,7 @@ struct Stencil {
     v3 w[3];
 };
 MPM_DEV void stencil(const SimParams& P, const Tile& T, v3 pos, Stencil& s) {
-    cell_number(P, pos, s.c0);
+    cell_number_in(P, pos, s.c0);
     s.f = frac_pos(P, pos, s.c0);
     bspline(s.f, s.w);
     s.t0[0] = s.c0[0] - T.base[0]; s.t0[1] = s.c0[1] - T.base[1]; s.t0[2] = s.c0[2] - T.base[2];
@@ -1029,12 +1043,10 @@ MPM_DEV void stencil_offsets(const Stencil& s, float off[3][3]) {
 }
 
 // gridToParticle.cs.wgsl:82-130 from the LDS velocity tile. Nodes are visited in the reference
-// order (z, y, x) so the FP32 sums round identically. Returns the 27-bit set of nodes P2G may
-// deposit to (allocated and in range).
-MPM_DEV uint32_t g2p_pbmpm(const Tile& T, const Stencil& s, const float off[3][3], v3& nv, m3& Bm) {
+// order (z, y, x) so the FP32 sums round identically; nodes without mass are skipped as there.
+MPM_DEV void g2p_pbmpm(const Tile& T, const Stencil& s, const float off[3][3], v3& nv, m3& Bm) {
     nv = mk(0, 0, 0);
     Bm = zero3();
-    uint32_t ok = 0;
     const int tb = tile_of(s.t0[0] - 1, s.t0[1] - 1, s.t0[2] - 1);
 #pragma unroll
     for (int oz = 0; oz < 3; ++oz)
@@ -1043,7 +1055,6 @@ MPM_DEV uint32_t g2p_pbmpm(const Tile& T, const Stencil& s, const float off[3][3
 #pragma unroll
     for (int ox = 0; ox < 3; ++ox) {
         const float4 q = T.vel[tb + ox + 6 * oy + 36 * oz];
-        if (q.w != 0.0f) ok |= 1u << (ox + 3 * oy + 9 * oz);
         if (q.w == 2.0f) {
             const float cw = (s.w[ox].x * s.w[oy].y) * s.w[oz].z;
             const v3 cv = cw * mk(q.x, q.y, q.z);
@@ -1053,25 +1064,16 @@ MPM_DEV uint32_t g2p_pbmpm(const Tile& T, const Stencil& s, const float off[3][3
             Bm.c[2] = Bm.c[2] + cv * off[2][oz];
         }
     }
-    return ok;
-}
-// Deposit set of a particle from the tile flags (when no G2P precedes the P2G).
-MPM_DEV uint32_t deposit_mask(const Tile& T, const Stencil& s) {
-    uint32_t ok = 0;
-    const int tb = tile_of(s.t0[0] - 1, s.t0[1] - 1, s.t0[2] - 1);
-#pragma unroll
-    for (int j = 0; j < 27; ++j)
-        if (T.vel[tb + (j % 3) + 6 * ((j / 3) % 3) + 36 * (j / 9)].w != 0.0f) ok |= 1u << j;
-    return ok;
 }
 
 // pbmpm.wgsl:41-76 + :79-147 into the LDS accumulators. The two i32 contributions of a node's
 // momentum (transfer and force) are added before the atomic: integer addition is associative,
 // so the node sums are the same bits as the reference's separate atomics. Each product keeps the
 // reference operand order; the per-axis factors D.c[a] * off[a][o] are shared by the nodes that
-// use them, which changes no rounding.
+// use them, which changes no rounding. All 27 nodes are deposited to: the reference skips nodes
+// that are not allocated or outside the grid, and their tile accumulators are never flushed.
 template <bool MASS>
-MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, const float off[3][3], uint32_t ok,
+MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, const float off[3][3],
                        float mass, v3 dx, const m3& D) {
     v3 ax[3], by[3], cz[3];
 #pragma unroll
@@ -1097,8 +1099,7 @@ MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, const floa
         for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox) {
-            const int j = ox + 3 * oy + 9 * oz;
-            const int t = ((ok >> j) & 1u) ? tb + ox + 6 * oy + 36 * oz : 216;
+            const int t = tb + ox + 6 * oy + 36 * oz;
             const float cw = (s.w[ox].x * s.w[oy].y) * s.w[oz].z;
             const float wmS = cw * mS;
             const v3 aff = (ax[ox] + by[oy]) + cz[oz];
@@ -1106,9 +1107,9 @@ MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, const floa
             int32_t px = quot_i32(num.x, P.dt, P.rdt);
             int32_t py = quot_i32(num.y, P.dt, P.rdt);
             int32_t pz = quot_i32(num.z, P.dt, P.rdt);
-            if (P.grav_nz[0]) px = (int32_t)((uint32_t)px + (uint32_t)f2i((gdiv.x * wmS) * P.dt));
-            if (P.grav_nz[1]) py = (int32_t)((uint32_t)py + (uint32_t)f2i((gdiv.y * wmS) * P.dt));
-            if (P.grav_nz[2]) pz = (int32_t)((uint32_t)pz + (uint32_t)f2i((gdiv.z * wmS) * P.dt));
+            // gravity along y only in this form (x and z terms are i32(+-0) = 0); a zero y
+            // component likewise adds 0
+            py = (int32_t)((uint32_t)py + (uint32_t)f2i((gdiv.y * wmS) * P.dt));
             if (MASS) atomicAdd(&T.acc[0][t], f2i(wmS));
             atomicAdd(&T.acc[1][t], px);
             atomicAdd(&T.acc[2][t], py);
@@ -1122,8 +1123,7 @@ MPM_DEV void p2g_pbmpm(const SimParams& P, Tile& T, const Stencil& s, const floa
     for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
     for (int ox = 0; ox < 3; ++ox) {
-        const int j = ox + 3 * oy + 9 * oz;
-        const int t = ((ok >> j) & 1u) ? tb + ox + 6 * oy + 36 * oz : 216;
+        const int t = tb + ox + 6 * oy + 36 * oz;
         const float cw = (s.w[ox].x * s.w[oy].y) * s.w[oz].z;
         const float wm = cw * mass;
         const v3 aff = (ax[ox] + by[oy]) + cz[oz];
@@ -1227,7 +1227,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                     solve_constraints(P, Fe, relax, D);
                     Stencil s; stencil(P, T, pos, s);
                     float off[3][3]; stencil_offsets(s, off);
-                    p2g_pbmpm<true>(P, T, s, off, deposit_mask(T, s), mass, dx, D);
+                    p2g_pbmpm<true>(P, T, s, off, mass, dx, D);
                 } else {   // run whose own block is not allocated: carried unchanged to LAST
                     stm(B.nxt, sd, DD, q, D); st3(B.nxt, sd, DXX, q, dx); st3(B.nxt, sd, VX, q, ld3(B.cur, sd, VX, old));
                 }
@@ -1240,7 +1240,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 Stencil s; stencil(P, T, pos, s);
                 float off[3][3]; stencil_offsets(s, off);
                 v3 nv; m3 Bm;
-                const uint32_t dep = g2p_pbmpm(T, s, off, nv, Bm);
+                g2p_pbmpm(T, s, off, nv, Bm);
                 const v3 dx = nv * P.dt;
                 if (B.resid_dx) resid = track_residual(P, B, q, dx, it);
                 m3 D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
@@ -1248,7 +1248,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 // the stencil is recomputed (same values) rather than kept live across the solve
                 Stencil s2; stencil(P, T, pos, s2);
                 float off2[3][3]; stencil_offsets(s2, off2);
-                p2g_pbmpm<false>(P, T, s2, off2, dep, mass, dx, D);
+                p2g_pbmpm<false>(P, T, s2, off2, mass, dx, D);
                 }
             } else {
                 Part p;
@@ -1332,20 +1332,19 @@ MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             const v3 pm = mass * vel;
             Stencil s; stencil(P, T, pos, s);
             v3 d[3]; bspline_d(s.f, d);
-            const v3 cdv = mk(P.cd[0], P.cd[1], P.cd[2]);
 #pragma unroll
             for (int oz = -1; oz <= 1; ++oz)
 #pragma unroll
             for (int oy = -1; oy <= 1; ++oy)
 #pragma unroll
             for (int ox = -1; ox <= 1; ++ox) {
+                // every node is deposited to; accumulators of unallocated nodes are never flushed
                 const int t = tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz);
-                if (T.node[t] < 0) continue;
                 const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
                 v3 g = mk((d[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z,
                           (s.w[ox + 1].x * d[oy + 1].y) * s.w[oz + 1].z,
                           (s.w[ox + 1].x * s.w[oy + 1].y) * d[oz + 1].z);
-                g = cdiv(g, cdv);
+                g = div_cd(P, g);
                 const v3 sm = mvmul(negVPT, g) * P.dt;
                 const v3 mom = cw * pm + sm;
                 atomicAdd(&T.acc[0][t], f2i((cw * mass) * P.S));
@@ -1484,7 +1483,8 @@ __global__ void k_copy_state(Bufs B, uint32_t n) {
 // Counts the float inputs x (all 2^32 bit patterns) where fixed_div(x, d, rd, S, true) differs
 // from the reference's i32((x / d) * S).
 // mode 0: fixed_div's FMA form against i32((x / d) * S) for every float x;
-// mode 1: quot_i32 against i32(x / d) for every float with |x| <= 2^100.
+// mode 1: quot_i32 against i32(x / d) for every float with |x| <= 2^100;
+// mode 2: qdiv against x / d, bit for bit, for every float with |x| <= 2^100.
 __global__ __launch_bounds__(256) void k_verify_fixed_div(float d, float rd, float S, int mode, uint32_t* bad) {
     uint32_t cnt = 0;
     const uint64_t gsz = (uint64_t)gridDim.x * blockDim.x;
@@ -1492,8 +1492,10 @@ __global__ __launch_bounds__(256) void k_verify_fixed_div(float d, float rd, flo
         const float x = as_f((uint32_t)u);
         if (mode == 0) {
             if (fixed_div(x, d, rd, S, false) != fixed_div(x, d, rd, S, true)) ++cnt;
-        } else if (fabs_(x) <= 0x1p100f) {
+        } else if (mode == 1 && fabs_(x) <= 0x1p100f) {
             if (f2i(x / d) != quot_i32(x, d, rd)) ++cnt;
+        } else if (mode == 2 && fabs_(x) <= 0x1p100f) {
+            if (as_u(x / d) != as_u(qdiv(x, d, rd))) ++cnt;
         }
     }
     if (cnt) atomicAdd(bad, cnt);

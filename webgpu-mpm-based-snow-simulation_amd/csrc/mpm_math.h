@@ -86,6 +86,13 @@ MPM_DEV int32_t quot_i32(float x, float d, float rd) {
     const float e = __builtin_fmaf(-q0, d, x);
     return f2i(__builtin_fmaf(e, rd, q0));
 }
+// x / d for |x| <= 2^100 by the same FMA-corrected quotient; used only for a divisor that a device
+// check found bit-identical to the IEEE quotient for every float with |x| <= 2^100 (mode 2).
+MPM_DEV float qdiv(float x, float d, float rd) {
+    const float q0 = x * rd;
+    const float e = __builtin_fmaf(-q0, d, x);
+    return __builtin_fmaf(e, rd, q0);
+}
 
 MPM_DEV float scale_pow2(float p, int n) {
     if (n > 127) { p = p * as_f(0x7f000000u); n -= 127; if (n > 127) n = 127; }
