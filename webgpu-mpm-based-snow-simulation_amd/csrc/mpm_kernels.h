@@ -600,6 +600,27 @@ MPM_DEV void plasticity(const SimParams& P, Part& p) {
     p.Fp = mmul(nFeInv, F);
 }
 
+// plasticityOps.wgsl:7-48 for row q of nxt, which already holds the particle's F_p: F_e is
+// updated in place; F_p is read and rewritten only when a singular value is clamped (otherwise the
+// reference leaves both unchanged). Same bits as plasticity().
+MPM_DEV void plasticity_row(const SimParams& P, const Bufs& B, uint32_t q, m3& Fe) {
+    const m3 dE = Fe;
+    const m3 rot = polar_rotation(dE);
+    const m3 sc = mmul(transpose(rot), dE);
+    const float lo = P.plast_lo, hi = P.plast_hi;
+    const float s0 = sc.c[0].x, s1 = sc.c[1].y, s2 = sc.c[2].z;
+    if (lo <= s0 && lo <= s1 && lo <= s2 && s0 <= hi && s1 <= hi && s2 <= hi) return;
+    float t0 = s0 < lo ? lo : s0; t0 = hi < t0 ? hi : t0;
+    float t1 = s1 < lo ? lo : s1; t1 = hi < t1 ? hi : t1;
+    float t2 = s2 < lo ? lo : s2; t2 = hi < t2 ? hi : t2;
+    m3 ns = zero3(); ns.c[0].x = t0; ns.c[1].y = t1; ns.c[2].z = t2;
+    const m3 nFe = mmul(rot, ns);
+    const m3 nFeInv = inverse3(nFe);
+    const m3 F = mmul(dE, ldm(B.nxt, B.stride, FP, q));
+    Fe = nFe;
+    stm(B.nxt, B.stride, FP, q, mmul(nFeInv, F));
+}
+
 MPM_DEV v3 xform(const float* T, v3 p) {
     v3 c0 = mk(T[0], T[1], T[2]), c1 = mk(T[4], T[5], T[6]), c2 = mk(T[8], T[9], T[10]), c3 = mk(T[12], T[13], T[14]);
     return ((c0 * p.x + c1 * p.y) + c2 * p.z) + c3 * 1.0f;
@@ -797,7 +818,8 @@ MPM_DEV bool integrate_body(const SimParams& P, Part& p) {
 // The motion part (x += dx, walls, collision candidate) touches only x, dx, v and the deformation
 // part (F_e = (I + D) F_e, plasticity) only F_e, F_p, so the two are independent and are done in
 // the order that keeps the fewest values live: F_e update, stores of D, motion and its stores,
-// then plasticity. The results are the same bits as integrate_body.
+// then plasticity (F_p stays in row q unless clamped; the pass before copied it there). The
+// results are the same bits as integrate_body.
 MPM_DEV bool integrate_store(const SimParams& P, const Bufs& B, Part& p, uint32_t q) {
     const uint32_t sd = B.stride;
     p.Fe = mmul(madd(ident(), p.D), p.Fe);
@@ -811,8 +833,8 @@ MPM_DEV bool integrate_store(const SimParams& P, const Bufs& B, Part& p, uint32_
     }
     const bool cand = collision_candidate(P, p.pos, p.dx);
     st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel); st3(B.nxt, sd, DXX, q, p.dx);
-    plasticity(P, p);
-    stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
+    plasticity_row(P, B, q, p.Fe);
+    stm(B.nxt, sd, FE, q, p.Fe);
     return cand;
 }
 // integrateParticles.wgsl:12-72 in one piece.
@@ -1260,7 +1282,6 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 p.pos = ld3(B.nxt, sd, PX, q);
                 p.mass = ld(B.nxt, sd, PM, q);
                 p.Fe = ldm(B.nxt, sd, FE, q);
-                p.Fp = ldm(B.nxt, sd, FP, q);
                 if (ok) {
                     Stencil s; stencil(P, T, p.pos, s);
                     float off[3][3]; stencil_offsets(s, off);
@@ -1410,7 +1431,6 @@ MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             p.pos = ld3(B.nxt, sd, PX, q);
             p.mass = ld(B.nxt, sd, PM, q);
             p.Fe = ldm(B.nxt, sd, FE, q);
-            p.Fp = ldm(B.nxt, sd, FP, q);
             if (ok) {
                 Stencil s; stencil(P, T, p.pos, s);
                 v3 d[3]; bspline_d(s.f, d);
