@@ -70,6 +70,20 @@ def test_batch_equals_scenes_alone(mpm, batch, persistent, chunk, wg):
         assert st["blocks_dropped"] == 0 and st["probe_failures"] == 0
 
 
+def test_batch_with_block_table_equals_scenes_alone(mpm, batch):
+    """hash_map_size given: the batch bins through the open-addressing block table instead of
+    direct key addressing; the results are the same bits."""
+    scenes, ref = batch
+    with mpm.MpmEnsemble([dict(sc["sim"], hash_map_size=400009) for sc in scenes]) as ens:
+        ens.set_tuning(512, 2, False)
+        for i, sc in enumerate(scenes):
+            ens.upload(i, sc["particles"])
+        ens.step(4)
+        for i in range(len(scenes)):
+            assert _same(ens.download(i), ref[i]), f"scene {i}"
+        assert ens.stats()["probe_failures"] == 0
+
+
 def test_batch_eager_equals_graph(mpm, batch):
     scenes, ref = batch
     with mpm.MpmEnsemble([sc["sim"] for sc in scenes], use_graph=0) as ens:

@@ -37,6 +37,8 @@ struct Ens {
     const uint32_t* nphase;       // [S] transfer phases per substep
     const uint32_t* method;       // [S] 0 explicit, 1 PBMPM
     uint32_t* nblk;               // [S] blocks allocated this substep
+    uint32_t* rbase;              // [S] direct addressing: first entry of the scene's blocks in the
+                                  // block list; nullptr: open-addressing block table
     uint32_t* cbeg;               // [S+1] first work chunk of each scene
     uint32_t* seg;                // [R*S+1] first task of each (phase, scene) segment, phase-major
     uint32_t* done;               // [R*S] finished tasks per segment
@@ -55,13 +57,28 @@ MPM_DEV uint32_t seg_find(const uint32_t* base, uint32_t n, uint32_t v) {
     return lo;
 }
 
+// Runs f(params of scene sc). Items are in scene order (particles after the first sort, runs and
+// keys always), so a wave is nearly always inside one scene: then the scene index is made
+// wave-uniform and the parameters are read with scalar loads instead of per-lane loads.
+template <class F>
+MPM_DEV void with_scene(const Ens& E, uint32_t sc, F&& f) {
+    const uint32_t s0 = __builtin_amdgcn_readfirstlane(sc);
+    if (__ballot(sc == s0) == __ballot(true)) f(E.sp[s0]);
+    else f(E.sp[sc]);
+}
+
 // ------------------------------------------------------------------ binning over all scenes
+// Direct addressing (P0.direct_blocks): a scene's dense block range is its key range, so a block's
+// index is its key (kslot[key] = key). The touched blocks are flagged here (per wave and key, one
+// lane per neighbour; unbinned particles per lane), listed in key order by the key-space scan
+// (block_key), and each scene's part of the list is found in k_ens_tasks.
 __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, Ens E) {
-    __shared__ uint32_t pb[ENS_MAX_SCENES + 1];
-    for (uint32_t s = threadIdx.x; s <= E.S; s += blockDim.x) pb[s] = E.part_base[s];
+    __shared__ uint32_t pb[ENS_MAX_SCENES + 1], kb[ENS_MAX_SCENES + 1];
+    for (uint32_t s = threadIdx.x; s <= E.S; s += blockDim.x) { pb[s] = E.part_base[s]; kb[s] = E.key_base[s]; }
     const uint32_t gsz = gridDim.x * blockDim.x;
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
-    for (uint32_t s = gi; s < P0.cap; s += gsz) B.table[s] = EMPTY;
+    if (!P0.direct_blocks)
+        for (uint32_t s = gi; s < P0.cap; s += gsz) B.table[s] = EMPTY;
     for (uint32_t s = gi; s < E.S; s += gsz) E.nblk[s] = 0;
     if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; }
     __syncthreads();
@@ -72,15 +89,19 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
         bool valid = false;
         uint32_t key = EMPTY, mask = 0;
         if (i < n) {
-            const SimParams& P = E.sp[seg_find(pb, E.S, B.pid_cur[i])];
-            v3 pos = ld3(B.cur, B.stride, PX, i);
-            int c[3]; cell_number(P, pos, c);
-            mask = touched_mask(P, c);
-            int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
-            if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
-                key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
-                valid = true;
-            }
+            const v3 pos = ld3(B.cur, B.stride, PX, i);
+            with_scene(E, seg_find(pb, E.S, B.pid_cur[i]), [&](const SimParams& P) {
+                int c[3]; cell_number(P, pos, c);
+                mask = touched_mask(P, c);
+                int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
+                if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
+                    key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+                    valid = true;
+                } else if (P0.direct_blocks && mask) {
+                    for (int jj = 0; jj < 27; ++jj)
+                        if ((mask >> jj) & 1u) B.kalloc[block_key_at(P, bx, by, bz, jj)] = 1u;
+                }
+            });
             B.key[i] = key; B.mask[i] = mask;
         }
         uint64_t active = __ballot(valid);
@@ -93,6 +114,17 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
             if (lane_id() == leader) {
                 atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
                 atomicOr(&B.kmask[lk], m);
+            }
+            if (P0.direct_blocks) {
+                // lk and m are the same in every lane: scene and parameters read once per wave
+                const int j = lane_id();
+                const uint32_t uk = __builtin_amdgcn_readfirstlane(lk);
+                const uint32_t um = __builtin_amdgcn_readfirstlane(m);
+                const SimParams& P = E.sp[__builtin_amdgcn_readfirstlane(seg_find(kb, E.S, uk))];
+                if (j < 27 && ((um >> j) & 1u)) {
+                    int b[3]; block_coords(P, uk, b);
+                    B.kalloc[block_key_at(P, b[0], b[1], b[2], j)] = 1u;
+                }
             }
             active &= ~peers;
         }
@@ -110,9 +142,10 @@ __global__ __launch_bounds__(256) void k_ens_insert(SimParams P0, Bufs B, Ens E)
         const uint32_t r = t / 27u, j = t % 27u;
         if ((B.run_mask[r] >> j) & 1u) {
             const uint32_t key = B.run_key[r];
-            const SimParams& P = E.sp[seg_find(kb, E.S, key)];
-            int b[3]; block_coords(P, key, b);
-            table_insert(P, B, b[0] + (int)(j % 3u) - 1, b[1] + (int)((j / 3u) % 3u) - 1, b[2] + (int)(j / 9u) - 1);
+            with_scene(E, seg_find(kb, E.S, key), [&](const SimParams& P) {
+                int b[3]; block_coords(P, key, b);
+                table_insert(P, B, b[0] + (int)(j % 3u) - 1, b[1] + (int)((j / 3u) % 3u) - 1, b[2] + (int)(j / 9u) - 1);
+            });
         }
     }
     for (uint32_t p = nvalid + blockIdx.x * blockDim.x + threadIdx.x; p < P0.n; p += gsz) {
@@ -166,9 +199,24 @@ __global__ __launch_bounds__(256) void k_ens_nbr(SimParams P0, Bufs B, Ens E) {
     for (uint32_t t = gi; t < nruns * 27u; t += gsz) {
         const uint32_t r = t / 27u, j = t % 27u;
         const uint32_t key = B.run_key[r];
-        const SimParams& P = E.sp[seg_find(kb, E.S, key)];
-        int b[3]; block_coords(P, key, b);
-        B.nbr[t] = table_lookup(P, B, b[0] + (int)(j % 3u) - 1, b[1] + (int)((j / 3u) % 3u) - 1, b[2] + (int)(j / 9u) - 1);
+        with_scene(E, seg_find(kb, E.S, key), [&](const SimParams& P) {
+            int b[3]; block_coords(P, key, b);
+            const int x = b[0] + (int)(j % 3u) - 1, y = b[1] + (int)((j / 3u) % 3u) - 1, z = b[2] + (int)(j / 9u) - 1;
+            if (P0.direct_blocks) {
+                const bool in = x >= 0 && y >= 0 && z >= 0 && x < P.nb[0] && y < P.nb[1] && z < P.nb[2];
+                B.nbr[t] = in ? B.kslot[P.kbase + (uint32_t)x + (uint32_t)P.nb[0] * ((uint32_t)y + (uint32_t)P.nb[1] * (uint32_t)z)] : EMPTY;
+            } else {
+                B.nbr[t] = table_lookup(P, B, x, y, z);
+            }
+        });
+    }
+    if (P0.direct_blocks) {
+        const uint32_t ncell = min(B.counters[C_NBLOCKS], P0.max_blocks) * 64u;
+        for (uint32_t c = gi; c < ncell; c += gsz) {
+            const size_t at = (size_t)B.block_key[c >> 6] * 64u + (c & 63u);
+            B.grid0[at] = make_int4(0, 0, 0, 0); B.gmass[at] = 0;
+        }
+        return;
     }
     for (uint32_t sc = 0; sc < E.S; ++sc) {
         const uint32_t nb = min(E.nblk[sc], E.sp[sc].blk_cap);
@@ -186,6 +234,20 @@ __global__ __launch_bounds__(256) void k_ens_tasks(Bufs B, Ens E) {
     for (uint32_t s = threadIdx.x; s <= E.S; s += blockDim.x) kb[s] = E.key_base[s];
     __syncthreads();
     const uint32_t nch = B.counters[C_NCHUNKS];
+    if (E.rbase) {
+        // direct addressing: scene s's blocks are the block-list entries with keys in its key range
+        const uint32_t nbt = B.counters[C_NBLOCKS];
+        for (uint32_t s = threadIdx.x; s < E.S; s += blockDim.x) {
+            uint32_t b[2];
+            for (int e = 0; e < 2; ++e) {
+                uint32_t lo = 0, hi = nbt;
+                const uint32_t kv = kb[s + e];
+                while (lo < hi) { const uint32_t mid = (lo + hi) >> 1; if (B.block_key[mid] < kv) lo = mid + 1; else hi = mid; }
+                b[e] = lo;
+            }
+            E.rbase[s] = b[0]; E.nblk[s] = b[1] - b[0];
+        }
+    }
     for (uint32_t s = threadIdx.x; s <= E.S; s += blockDim.x) {
         // first chunk whose scene is >= s (chunks are in key order, keys in scene order)
         uint32_t lo = 0, hi = nch;
@@ -224,10 +286,17 @@ __global__ __launch_bounds__(256) void k_ens_tasks(Bufs B, Ens E) {
     }
 }
 
-// Clear of slice j of n of scene P's live blocks in grid buffer g.
-MPM_DEV void ens_clear_slice(const SimParams& P, int4* g, uint32_t nblk, uint32_t j, uint32_t n) {
-    const uint32_t nb = min(nblk, P.blk_cap);
+// Clear of slice j of n of scene sc's live blocks in grid buffer g: a dense range from blk_base
+// (block table), or the scene's entries of the block list, each block at its key (direct).
+MPM_DEV void ens_clear_slice(const SimParams& P, const Bufs& B, const Ens& E, uint32_t sc, int4* g, uint32_t j, uint32_t n) {
+    const uint32_t nb = min(E.nblk[sc], P.blk_cap);
     const uint32_t b0 = (uint32_t)(((uint64_t)nb * j) / n), b1 = (uint32_t)(((uint64_t)nb * (j + 1)) / n);
+    if (E.rbase) {
+        const uint32_t* list = B.block_key + E.rbase[sc] + b0;
+        for (uint32_t c = threadIdx.x; c < (b1 - b0) * 64u; c += blockDim.x)
+            g[(size_t)list[c >> 6] * 64u + (c & 63u)] = make_int4(0, 0, 0, 0);
+        return;
+    }
     int4* base = g + (size_t)(P.blk_base + b0) * 64u;
     for (uint32_t c = threadIdx.x; c < (b1 - b0) * 64u; c += blockDim.x) base[c] = make_int4(0, 0, 0, 0);
 }
@@ -279,10 +348,10 @@ __global__ __launch_bounds__(256, OCC) void k_ens_transfer(Bufs B, Ens E, int ph
         if (E.method[sc] != 0u) {
             const int k = P.k;
             if (r == 0) {
-                if (k > 1) ens_clear_slice(P, B.grid1, E.nblk[sc], j, ntask);
+                if (k > 1) ens_clear_slice(P, B, E, sc, B.grid1, j, ntask);
                 pbmpm_chunk<PB_FIRST>(P, B, T, ch, nullptr, B.grid0, -1);
             } else if ((int)r < k) {
-                ens_clear_slice(P, grid_sel(B, (int)(r + 1) % 3), E.nblk[sc], j, ntask);
+                ens_clear_slice(P, B, E, sc, grid_sel(B, (int)(r + 1) % 3), j, ntask);
                 pbmpm_chunk<PB_MID>(P, B, T, ch, grid_sel(B, (int)(r - 1) % 3), grid_sel(B, (int)r % 3), (int)r - 1);
             } else {
                 pbmpm_chunk<PB_LAST>(P, B, T, ch, grid_sel(B, (k - 1) % 3), nullptr, k - 1);

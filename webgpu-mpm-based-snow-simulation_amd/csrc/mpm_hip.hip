@@ -105,6 +105,7 @@ struct mpm_ctx {
     SimParams ens_P0;
     SimParams* d_sp = nullptr;
     uint32_t *d_part_base = nullptr, *d_key_base = nullptr, *d_nphase = nullptr, *d_method = nullptr, *d_nblk = nullptr;
+    uint32_t* d_rbase = nullptr;      // batched scenes, direct addressing: first block-list entry per scene
     uint32_t *d_cbeg = nullptr, *d_seg = nullptr, *d_done = nullptr, *d_ctl = nullptr;
     int ens_persistent = 1;
     uint32_t ens_wg_per_cu = 2;
@@ -509,6 +510,7 @@ Ens ens_bufs(mpm_ctx* c) {
     Ens E;
     E.sp = c->d_sp; E.part_base = c->d_part_base; E.key_base = c->d_key_base; E.nphase = c->d_nphase; E.method = c->d_method;
     E.nblk = c->d_nblk; E.cbeg = c->d_cbeg; E.seg = c->d_seg; E.done = c->d_done; E.ctl = c->d_ctl;
+    E.rbase = c->direct_blocks ? c->d_rbase : nullptr;
     E.S = c->ens; E.R = c->ens_R;
     return E;
 }
@@ -521,16 +523,19 @@ void enqueue_ens_substep(mpm_ctx* c, bool full) {
     Bufs B = make_bufs(c);
     Ens E = ens_bufs(c);
     if (full) hipEventRecord(c->ev[0], s);
-    const uint32_t gk = grid_for(std::max<uint64_t>(c->n, c->cap), 8192);
+    const uint32_t gk = grid_for(c->direct_blocks ? (uint64_t)c->n : std::max<uint64_t>(c->n, c->cap), 8192);
     kt(c, -1);
     k_ens_keys_count<<<gk, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_KEYS);
     k_scan_tiles<<<c->ntiles, 256, 0, s>>>(P0, B); kt(c, KID_SCAN_TILES);
-    k_scan_top<<<1, 256, 0, s>>>(P0, B, c->ntiles); kt(c, KID_SCAN_TOP);
-    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P0, B, c->ntiles, 0); kt(c, KID_SCAN_APPLY);
+    const int own_ofs = c->ntiles <= 1024u ? 1 : 0;
+    if (!own_ofs) { k_scan_top<<<1, 256, 0, s>>>(P0, B, c->ntiles); kt(c, KID_SCAN_TOP); }
+    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P0, B, c->ntiles, own_ofs); kt(c, KID_SCAN_APPLY);
     k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P0, B); kt(c, KID_SCATTER);
     const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 8192);
-    k_ens_insert<<<gi, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_INSERT);
-    k_ens_compact<<<grid_for(c->cap, 1u << 30), 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_COMPACT);
+    if (!c->direct_blocks) {
+        k_ens_insert<<<gi, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_INSERT);
+        k_ens_compact<<<grid_for(c->cap, 1u << 30), 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_COMPACT);
+    }
     k_ens_nbr<<<gi, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_NBR);
     k_ens_tasks<<<1, 256, 0, s>>>(B, E); kt(c, KID_ENS_TASKS);
     if (full) hipEventRecord(c->ev[1], s);
@@ -1407,7 +1412,9 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
     c->K = (uint32_t)K;
     c->chunk = cfgs[0].chunk_particles ? cfgs[0].chunk_particles : 512u;
     c->max_blocks = (uint32_t)K;
-    c->cap = next_prime((uint32_t)std::min<uint64_t>(2ull * std::min<uint64_t>(K, 8ull * n) + 1, 0xFFFFFF00ull));
+    c->cap = cfgs[0].hash_map_size ? cfgs[0].hash_map_size
+                                   : next_prime((uint32_t)std::min<uint64_t>(2ull * std::min<uint64_t>(K, 8ull * n) + 1, 0xFFFFFF00ull));
+    c->direct_blocks = cfgs[0].hash_map_size == 0;
     c->max_runs = (uint32_t)std::min<uint64_t>(n, K);
     c->max_chunks = (c->n + 255u) / 256u + c->max_runs;      // room for chunks down to 256 particles
     c->ntiles = (uint32_t)((K + 1023) / 1024);
@@ -1416,6 +1423,7 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
         const size_t S = (size_t)n_scenes;
         if ((rc = dalloc(c, &c->d_sp, S)) || (rc = dalloc(c, &c->d_part_base, S + 1)) || (rc = dalloc(c, &c->d_key_base, S + 1)) ||
             (rc = dalloc(c, &c->d_nphase, S)) || (rc = dalloc(c, &c->d_method, S)) || (rc = dalloc(c, &c->d_nblk, S)) ||
+            (rc = dalloc(c, &c->d_rbase, S)) ||
             (rc = dalloc(c, &c->d_cbeg, S + 1)) || (rc = dalloc(c, &c->d_seg, (size_t)R * S + 1)) ||
             (rc = dalloc(c, &c->d_done, (size_t)R * S)) || (rc = dalloc(c, &c->d_ctl, ENS_CTL_PHASE + 128))) {}
     }
@@ -1460,6 +1468,9 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
         std::memset(&c->ens_P0, 0, sizeof(SimParams));
         c->ens_P0.n = c->n; c->ens_P0.K = c->K; c->ens_P0.chunk = c->chunk; c->ens_P0.cap = c->cap;
         c->ens_P0.max_blocks = c->max_blocks; c->ens_P0.max_probes = c->cfg.max_probes ? c->cfg.max_probes : 100u;
+        c->ens_P0.direct_blocks = c->direct_blocks ? 1u : 0u;
+        c->ens_P0.block_identity = 1u;     // a scene's dense block range is its key range
+
         std::vector<uint32_t> meth(n_scenes);
         for (int s = 0; s < n_scenes; ++s) meth[s] = (uint32_t)methods[s];
         hipError_t e = hipMemcpy(c->d_sp, c->ens_sp.data(), n_scenes * sizeof(SimParams), hipMemcpyHostToDevice);

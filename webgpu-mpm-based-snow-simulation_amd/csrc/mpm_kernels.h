@@ -61,6 +61,7 @@ struct SimParams {
     float rdt;                 // RN(1/dt) for fixed_div
     uint32_t fast_dt;          // fixed_div(., dt, rdt, S) verified equal to i32((x/dt) S) for all x
     uint32_t direct_blocks;    // block set by direct key addressing (no hash table)
+    uint32_t block_identity;   // direct addressing with dense block index = key (batched scenes)
     float rcd[3];              // RN(1 / cell dims)
     uint32_t fast_cd;          // qdiv(., cd[a], rcd[a]) verified equal to x / cd[a] for all |x| <= 2^100
     uint32_t p2g_fold;         // PBMPM P2G may fold S into the mass (S = 2^k <= 2^16, quot_i32 verified
@@ -434,7 +435,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B, uint32_
         if (P.direct_blocks) {
             uint32_t slot = EMPTY;
             if (alc[q]) {
-                if (o3 < P.max_blocks) { slot = o3; B.block_key[o3] = k; }
+                if (o3 < P.max_blocks) { slot = P.block_identity ? k : o3; B.block_key[o3] = k; }
                 B.kalloc[k] = 0u;
                 o3 += 1;
             }
