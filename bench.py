@@ -235,6 +235,26 @@ def measured_traffic(kernel):
         return None
 
 
+def measured_atomics(kernel, seconds):
+    """Global / LDS atomic rates of one launch from the committed rocprofv3 --pmc summary."""
+    f = ROOT / "profiles" / "atomics.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        k, pk = d.get(kernel), d.get("_peaks", {})
+        if not k or seconds <= 0:
+            return None
+        g, l = k["tcc_atomic_requests"] / seconds, k["lds_atomic_wave_instr"] / seconds
+        return {"global_requests_per_launch": k["tcc_atomic_requests"], "global_per_s": g,
+                "global_frac": g / pk["global_atomic_requests_per_s"],
+                "lds_wave_instr_per_launch": k["lds_atomic_wave_instr"], "lds_per_s": l,
+                "lds_frac": l / pk["lds_atomic_wave_instr_per_s"],
+                "peaks_assumed": pk}
+    except (ValueError, OSError, KeyError, ZeroDivisionError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,7 +333,8 @@ def main():
                      # the kernel is VALU-bound: issue utilisation = wave-instructions x 4 cycles /
                      # (1,024 SIMDs x 2.4 GHz x launch time), instructions from the committed counters
                      "valu_issue_utilisation": (measured_valu(dom) * 4 / (1024 * 2.4e9 * avg_s)) if measured_valu(dom) else None,
-                     "step_frac_reference_bytes": (STEP_ALGO_BYTES(k) * value / 1e9) / HBM_PEAK_GBS},
+                     "step_frac_reference_bytes": (STEP_ALGO_BYTES(k) * value / 1e9) / HBM_PEAK_GBS,
+                     "atomics": measured_atomics(dom, avg_s)},
         "kernel_ms_per_substep": {nm: round(v, 5) for nm, v in sorted(per_substep.items(), key=lambda x: -x[1])},
         "eager_ms_per_substep": eager_ms,
     }
