@@ -315,6 +315,10 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
     if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; }
     const uint32_t n = P.n;
     const uint32_t nloop = (n + gsz - 1) / gsz;
+    // key offset of the neighbour block this lane flags (lane j < 27: offset j, x fastest); a
+    // touched neighbour is inside the block grid, so key + offset is its key
+    const int jl = lane_id();
+    const uint32_t koff = (uint32_t)((jl % 3 - 1) + P.nb[0] * ((jl / 3) % 3 - 1 + P.nb[1] * (jl / 9 - 1)));
     for (uint32_t it = 0; it < nloop; ++it) {
         uint32_t i = it * gsz + gi;
         bool valid = false;
@@ -345,13 +349,7 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
                 atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
                 atomicOr(&B.kmask[lk], m);
             }
-            if (P.direct_blocks) {
-                const int j = lane_id();
-                if (j < 27 && ((m >> j) & 1u)) {
-                    int b[3]; block_coords(P, lk, b);
-                    B.kalloc[block_key_at(P, b[0], b[1], b[2], j)] = 1u;
-                }
-            }
+            if (P.direct_blocks && jl < 27 && ((m >> jl) & 1u)) B.kalloc[lk + koff] = 1u;
             active &= ~peers;
         }
     }
