@@ -390,12 +390,16 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
         c->last_pbmpm = true;
         if (stage == 1) return false;
         for (int i = 1; i < k; ++i) {
-            k_pbmpm<PB_MID><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3, i - 1); kt(c, KID_PB_MID);
+            if (B.resid_dx) k_pbmpm<PB_MID, true><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3, i - 1);
+            else k_pbmpm<PB_MID><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3, i - 1);
+            kt(c, KID_PB_MID);
             c->last_grid = i % 3;
             if (stage == i + 1) return false;
         }
         if (full) hipEventRecord(c->ev[2], s);
-        k_pbmpm<PB_LAST><<<G, 256, 0, s>>>(P, B, (k - 1) % 3, -1, -1, k - 1); kt(c, KID_PB_LAST);
+        if (B.resid_dx) k_pbmpm<PB_LAST, true><<<G, 256, 0, s>>>(P, B, (k - 1) % 3, -1, -1, k - 1);
+        else k_pbmpm<PB_LAST><<<G, 256, 0, s>>>(P, B, (k - 1) % 3, -1, -1, k - 1);
+        kt(c, KID_PB_LAST);
         if (P.learn_flags & 2u) k_resid_roll<<<1, 64, 0, s>>>(B, k, c->resid_peak);
     } else {
         k_explicit_p2g<0><<<G, 256, 0, s>>>(P, B); kt(c, KID_EX_P2G);

@@ -1218,7 +1218,7 @@ __global__ void k_resid_roll(Bufs B, int k, uint32_t* peak) {
 
 // One work chunk of a PBMPM transfer pass (the body of k_pbmpm; the batched-scene kernel calls
 // it with the chunk's scene parameters).
-template <int MODE>
+template <int MODE, bool TRACK = false>
 MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, const int4* __restrict__ gi,
                          int4* __restrict__ go, int it) {
     const uint32_t sd = B.stride;
@@ -1268,7 +1268,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 v3 nv; m3 Bm;
                 g2p_pbmpm(T, s, off, nv, Bm);
                 const v3 dx = nv * P.dt;
-                if (B.resid_dx) resid = track_residual(P, B, q, dx, it);
+                if (TRACK) resid = track_residual(P, B, q, dx, it);
                 m3 D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                 solve_constraints(P, Fe, relax, D);
                 // the stencil is recomputed (same values) rather than kept live across the solve
@@ -1287,7 +1287,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                     v3 nv; m3 Bm;
                     g2p_pbmpm(T, s, off, nv, Bm);
                     p.dx = nv * P.dt;
-                    if (B.resid_dx) resid = track_residual(P, B, q, p.dx, it);
+                    if (TRACK) resid = track_residual(P, B, q, p.dx, it);
                     p.D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                     p.vel = nv;
                 } else {
@@ -1296,7 +1296,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 cand = integrate_store(P, B, p, q);
             }
             if (MODE == PB_LAST) queue_collider(B, cand, q);
-            if (MODE != PB_FIRST && B.resid_dx && it > 0) publish_residual(B, resid, it);
+            if (MODE != PB_FIRST && TRACK && it > 0) publish_residual(B, resid, it);
         }
         __syncthreads();
         if (MODE == PB_FIRST) tile_flush_pb<true>(T, go, B.gmass);
@@ -1307,7 +1307,9 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
 
 // Resident workgroups per CU: MID runs at 5 (96 VGPRs, a few spilled registers, measured faster),
 // FIRST and LAST at 4 (their spills at 5 cost more than the occupancy gains).
-template <int MODE>
+// TRACK: the residual bookkeeping of the learned warm start (B.resid_dx set); a separate
+// instantiation so the untracked passes carry none of its registers.
+template <int MODE, bool TRACK = false>
 __global__ __launch_bounds__(256, MODE == PB_MID ? 5 : 4) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
     __shared__ Tile T;
     const uint32_t nchunks = B.counters[C_NCHUNKS];
@@ -1318,7 +1320,7 @@ __global__ __launch_bounds__(256, MODE == PB_MID ? 5 : 4) void k_pbmpm(SimParams
     }
     const int4* gi = MODE != PB_FIRST ? grid_sel(B, gin) : nullptr;
     int4* go = MODE != PB_LAST ? grid_sel(B, gout) : nullptr;
-    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) pbmpm_chunk<MODE>(P, B, T, B.chunks[ch], gi, go, it);
+    for (uint32_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) pbmpm_chunk<MODE, TRACK>(P, B, T, B.chunks[ch], gi, go, it);
 }
 
 // ------------------------------------------------------------------ explicit MPM
