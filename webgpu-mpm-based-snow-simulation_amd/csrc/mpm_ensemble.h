@@ -111,10 +111,13 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
             bool peer = valid && key == lk;
             uint64_t peers = __ballot(peer);
             uint32_t m = wave_or(peer ? mask : 0u);
+            uint32_t b = 0;
             if (lane_id() == leader) {
-                atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
+                b = atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
                 atomicOr(&B.kmask[lk], m);
             }
+            b = __shfl(b, leader, 64);
+            if (peer) B.lofs[i] = b + (uint32_t)__popcll(peers & lanemask_lt());
             if (P0.direct_blocks) {
                 // lk and m are the same in every lane: scene and parameters read once per wave
                 const int j = lane_id();

@@ -39,6 +39,7 @@ struct mpm_ctx {
     int cur = 0;
     float* relax = nullptr;
     uint32_t *key = nullptr, *mask = nullptr, *perm = nullptr, *kcount = nullptr, *kmask = nullptr, *kcursor = nullptr;
+    uint32_t* lofs = nullptr;
     uint32_t *tile_sums = nullptr, *tile_ofs = nullptr;
     uint32_t *kalloc = nullptr, *kslot = nullptr;
     bool direct_blocks = false;       // block set by direct key addressing (hash_map_size == 0)
@@ -186,7 +187,7 @@ Bufs make_bufs(mpm_ctx* c) {
     B.cur = c->planes[c->cur]; B.nxt = c->planes[c->cur ^ 1];
     B.pid_cur = c->pid[c->cur]; B.pid_nxt = c->pid[c->cur ^ 1];
     B.stride = c->stride;
-    B.relax = c->relax; B.key = c->key; B.mask = c->mask; B.perm = c->perm;
+    B.relax = c->relax; B.key = c->key; B.mask = c->mask; B.perm = c->perm; B.lofs = c->lofs;
     B.kcount = c->kcount; B.kmask = c->kmask; B.kcursor = c->kcursor;
     B.tile_sums = c->tile_sums; B.tile_ofs = c->tile_ofs;
     B.kalloc = c->kalloc; B.kslot = c->kslot;
@@ -481,7 +482,7 @@ int alloc_buffers(mpm_ctx* c) {
     const size_t ps = (size_t)NPLANES * c->stride;
     if ((rc = dalloc(c, &c->planes[0], ps)) || (rc = dalloc(c, &c->planes[1], ps)) ||
         (rc = dalloc(c, &c->pid[0], c->stride)) || (rc = dalloc(c, &c->pid[1], c->stride)) ||
-        (rc = dalloc(c, &c->relax, c->stride)) || (rc = dalloc(c, &c->key, c->stride)) ||
+        (rc = dalloc(c, &c->relax, c->stride)) || (rc = dalloc(c, &c->key, c->stride)) || (rc = dalloc(c, &c->lofs, c->stride)) ||
         (rc = dalloc(c, &c->mask, c->stride)) || (rc = dalloc(c, &c->perm, c->stride)) ||
         (rc = dalloc(c, &c->kcount, K)) || (rc = dalloc(c, &c->kmask, K)) || (rc = dalloc(c, &c->kcursor, K)) ||
         (rc = dalloc(c, &c->tile_sums, 4 * (size_t)c->ntiles)) || (rc = dalloc(c, &c->tile_ofs, 4 * (size_t)c->ntiles)) ||

@@ -87,6 +87,7 @@ struct Bufs {
     float* cur; float* nxt; uint32_t* pid_cur; uint32_t* pid_nxt; uint32_t stride;
     float* relax;
     uint32_t* key; uint32_t* mask; uint32_t* perm;
+    uint32_t* lofs;            // per particle: its offset among the particles of its key (from the count atomic)
     uint32_t* kcount; uint32_t* kmask; uint32_t* kcursor;
     uint32_t* tile_sums; uint32_t* tile_ofs;     // 4 per tile
     uint32_t* kalloc;          // per key: block touched by some particle's stencil (direct addressing)
@@ -302,9 +303,10 @@ MPM_DEV uint32_t block_key_at(const SimParams& P, int bx, int by, int bz, int j)
     bx += j % 3 - 1; by += (j / 3) % 3 - 1; bz += j / 9 - 1;
     return P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
 }
-// Per particle: own-block key, touched-neighbour mask; per key: particle count and OR of masks
-// (countParticlesPerBlock.wgsl:7-23 + the block set of mapAffectedBlocks.wgsl:64-86). Atomics are
-// aggregated per wave over lanes that share a key (input is in the previous substep's sorted order).
+// Per particle: own-block key, touched-neighbour mask, offset among its key's particles; per key:
+// particle count and OR of masks (countParticlesPerBlock.wgsl:7-23 + the block set of
+// mapAffectedBlocks.wgsl:64-86). Atomics are aggregated per wave over lanes that share a key (input
+// is in the previous substep's sorted order); the count atomic's old value gives the offsets.
 // Hash-table mode also resets the table (clearHashMap.wgsl:5-19); direct mode instead flags every
 // touched block in kalloc (plain stores of 1, one lane per neighbour). Resets the per-substep counters.
 __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
@@ -345,10 +347,13 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
             bool peer = valid && key == lk;
             uint64_t peers = __ballot(peer);
             uint32_t m = wave_or(peer ? mask : 0u);
+            uint32_t b = 0;
             if (lane_id() == leader) {
-                atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
+                b = atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
                 atomicOr(&B.kmask[lk], m);
             }
+            b = __shfl(b, leader, 64);
+            if (peer) B.lofs[i] = b + (uint32_t)__popcll(peers & lanemask_lt());
             if (P.direct_blocks && jl < 27 && ((m >> jl) & 1u)) B.kalloc[lk + koff] = 1u;
             active &= ~peers;
         }
@@ -441,29 +446,21 @@ __global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B, uint32_
         }
     }
 }
-// binParticles.wgsl:8-25 — scatter of particle indices by key; one cursor atomic per wave and key.
-// Particles with no valid own block go to the tail [n_valid, N). The per-wave reservations leave a
-// block's particles as 64-particle pieces in arrival order; measured against one reservation per
-// workgroup and key (contiguous 1,024-particle pieces), this order makes the transfer passes
-// faster (fewer lanes of a wave in one cell, so fewer conflicting LDS atomics).
+// binParticles.wgsl:8-25 — scatter of particle indices by key: a particle's sorted position is its
+// key's start plus the offset the count atomic gave it in keys_count (lanes of a wave that share a
+// key take consecutive offsets; waves in arrival order, so a block's particles are 64-particle
+// pieces in arrival order — measured faster for the transfer passes than one reservation per
+// workgroup and key, which gives contiguous 1,024-particle pieces: fewer lanes of a wave in one
+// cell, so fewer conflicting LDS atomics). Particles with no valid own block go to the tail
+// [n_valid, N).
 __global__ __launch_bounds__(256) void k_scatter(SimParams P, Bufs B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nvalid = B.counters[C_NVALID];
-    uint32_t key = i < P.n ? B.key[i] : EMPTY;
-    bool valid = i < P.n && key != EMPTY;
-    bool strag = i < P.n && key == EMPTY;
-    uint64_t active = __ballot(valid);
-    while (active) {
-        int leader = __ffsll((unsigned long long)active) - 1;
-        uint32_t lk = __shfl(key, leader, 64);
-        bool peer = valid && key == lk;
-        uint64_t peers = __ballot(peer);
-        uint32_t b = 0;
-        if (lane_id() == leader) b = atomicAdd(&B.kcursor[lk], (uint32_t)__popcll(peers));
-        b = __shfl(b, leader, 64);
-        const uint32_t dst = b + (uint32_t)__popcll(peers & lanemask_lt());
-        if (peer && dst < nvalid) B.perm[dst] = i;     // always true when keys and counts agree
-        active &= ~peers;
+    const uint32_t key = i < P.n ? B.key[i] : EMPTY;
+    const bool strag = i < P.n && key == EMPTY;
+    if (i < P.n && key != EMPTY) {
+        const uint32_t dst = B.kcursor[key] + B.lofs[i];
+        if (dst < nvalid) B.perm[dst] = i;     // always true when keys and counts agree
     }
     uint64_t sm = __ballot(strag);
     if (sm) {
