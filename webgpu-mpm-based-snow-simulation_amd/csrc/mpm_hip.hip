@@ -351,13 +351,16 @@ void enqueue_binning(mpm_ctx* c, const SimParams& P, const Bufs& B) {
     const int own_ofs = c->ntiles <= 1024u ? 1 : 0;
     if (!own_ofs) { k_scan_top<<<1, 256, 0, s>>>(P, B, c->ntiles); kt(c, KID_SCAN_TOP); }
     k_scan_apply<<<c->ntiles, 256, 0, s>>>(P, B, c->ntiles, own_ofs); kt(c, KID_SCAN_APPLY);
-    k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P, B); kt(c, KID_SCATTER);
     const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 4096);
-    if (!P.direct_blocks) {
+    if (P.direct_blocks) {
+        // the neighbour table needs only the scan here: one launch with the scatter
+        k_scatter<true><<<std::max(grid_for(c->n, 1u << 30), gi), 256, 0, s>>>(P, B); kt(c, KID_SCATTER);
+    } else {
+        k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P, B); kt(c, KID_SCATTER);
         k_insert<<<gi, 256, 0, s>>>(P, B); kt(c, KID_INSERT);
         k_compact<<<grid_for(c->cap, 1u << 30), 256, 0, s>>>(P, B); kt(c, KID_COMPACT);
+        k_nbr<<<gi, 256, 0, s>>>(P, B); kt(c, KID_NBR);
     }
-    k_nbr<<<gi, 256, 0, s>>>(P, B); kt(c, KID_NBR);
 }
 
 // Workgroups of the chunk kernels: four times the resident count (4 per CU at 128 VGPRs), so the
@@ -416,9 +419,10 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
         const uint32_t gc = std::min<uint32_t>(grid_for(c->n, 1u << 30), (uint32_t)c->n_cu * 8u);
         if (P.n_coll_rec <= COLL_LDS_REC && P.n_coll_tri <= COLL_LDS_TRI) k_collide<true><<<gc, 256, 0, s>>>(P, B);
         else k_collide<false><<<gc, 256, 0, s>>>(P, B);
-        kt(c, KID_COLLIDE);
+        kt(c, KID_COLLIDE);     // also the unbinned particles
+    } else {
+        k_straggle<<<gs, 256, 0, s>>>(P, B); kt(c, KID_STRAGGLE);
     }
-    k_straggle<<<gs, 256, 0, s>>>(P, B); kt(c, KID_STRAGGLE);
     if (full) hipEventRecord(c->ev[3], s);
     return true;
 }

@@ -453,7 +453,12 @@ __global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B, uint32_
 // workgroup and key, which gives contiguous 1,024-particle pieces: fewer lanes of a wave in one
 // cell, so fewer conflicting LDS atomics). Particles with no valid own block go to the tail
 // [n_valid, N).
+// NBR: also the neighbour table / grid clear / collider triangles of k_nbr (direct addressing,
+// where they depend on the scan only), in the same launch.
+MPM_DEV void nbr_pass(const SimParams& P, const Bufs& B);
+template <bool NBR = false>
 __global__ __launch_bounds__(256) void k_scatter(SimParams P, Bufs B) {
+    if (NBR) nbr_pass(P, B);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nvalid = B.counters[C_NVALID];
     const uint32_t key = i < P.n ? B.key[i] : EMPTY;
@@ -555,7 +560,7 @@ MPM_DEV uint32_t table_lookup(const SimParams& P, const Bufs& B, int bx, int by,
 // clear of grid buffer 0 over the allocated blocks (clearMappedBlocks.wgsl:9-26 restricted to
 // live blocks); also the world-space collider triangles of this substep.
 MPM_DEV void collider_prep_triangle(const SimParams& P, const Bufs& B, uint32_t t);
-__global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) {
+MPM_DEV void nbr_pass(const SimParams& P, const Bufs& B) {
     const uint32_t nruns = B.counters[C_NRUNS];
     const uint32_t gsz = gridDim.x * blockDim.x;
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -575,6 +580,7 @@ __global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) {
     if (P.has_collider)
         for (uint32_t t = gi; t < P.n_coll_tri; t += gsz) collider_prep_triangle(P, B, t);
 }
+__global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) { nbr_pass(P, B); }
 
 // ================================================================== particle update helpers
 // plasticityOps.wgsl:7-48
@@ -851,10 +857,15 @@ MPM_DEV void queue_collider(const Bufs& B, bool cand, uint32_t q) {
 // writes x, v, dx of the new-order state. LDS = true: the records and triangles are first copied
 // into LDS (colliders of at most COLL_LDS_REC records and COLL_LDS_TRI triangles).
 constexpr uint32_t COLL_LDS_REC = 64, COLL_LDS_TRI = 128;
+MPM_DEV void straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_t old);
+// Also integrates the unbinned particles (k_straggle's work) after the queue.
 template <bool LDS>
 __global__ __launch_bounds__(256) void k_collide(SimParams P, Bufs B) {
     const uint32_t nq = B.counters[C_NCOLL];
     const uint32_t sd = B.stride;
+    const uint32_t nvalid = B.counters[C_NVALID];
+    for (uint32_t q = nvalid + blockIdx.x * blockDim.x + threadIdx.x; q < P.n; q += gridDim.x * blockDim.x)
+        straggle_one(P, B, q, B.perm[q]);
     if (blockIdx.x * blockDim.x >= nq) return;
     const float4* rec = B.crec;
     const float4* tri = B.ctri;
