@@ -32,7 +32,7 @@ struct mpm_ctx {
     hipStream_t stream = nullptr;
     uint8_t uni[MPM_UNIFORMS_BYTES];
     std::string err;
-    uint32_t n = 0, stride = 0, K = 0, nb[3] = {0, 0, 0}, cap = 0, max_blocks = 0, max_runs = 0, max_chunks = 0, ntiles = 0;
+    uint32_t n = 0, stride = 0, K = 0, nb[3] = {0, 0, 0}, cap = 0, max_blocks = 0, max_runs = 0, max_chunks = 0, ntiles = 0, scan_kpt = 1;
     uint32_t chunk = 512;
     float* planes[2] = {nullptr, nullptr};
     uint32_t* pid[2] = {nullptr, nullptr};
@@ -138,6 +138,23 @@ void kt(mpm_ctx* c, int id) {
     c->kt_id[c->kt_n] = id;
     hipEventRecord(c->kt_ev[c->kt_n], c->stream);
     c->kt_n++;
+}
+// Scan tiling over K keys: one key per thread while that keeps the tile count within SCAN_OWN_MAX,
+// four per thread beyond (fewer tiles for k_scan_top to combine).
+void set_scan_tiles(mpm_ctx* c, uint64_t K) {
+    c->scan_kpt = (K + 255u) / 256u <= SCAN_OWN_MAX ? 1u : 4u;
+    c->ntiles = (uint32_t)((K + 256u * c->scan_kpt - 1) / (256u * c->scan_kpt));
+}
+// k_scan_tiles, k_scan_top when there are many tiles, k_scan_apply.
+void enqueue_scan(mpm_ctx* c, const SimParams& P, const Bufs& B, hipStream_t s) {
+    const int own_ofs = c->ntiles <= SCAN_OWN_MAX ? 1 : 0;
+    if (c->scan_kpt == 1) k_scan_tiles<1><<<c->ntiles, 256, 0, s>>>(P, B);
+    else k_scan_tiles<4><<<c->ntiles, 256, 0, s>>>(P, B);
+    kt(c, KID_SCAN_TILES);
+    if (!own_ofs) { k_scan_top<<<1, 256, 0, s>>>(P, B, c->ntiles); kt(c, KID_SCAN_TOP); }
+    if (c->scan_kpt == 1) k_scan_apply<1><<<c->ntiles, 256, 0, s>>>(P, B, c->ntiles, own_ofs);
+    else k_scan_apply<4><<<c->ntiles, 256, 0, s>>>(P, B, c->ntiles, own_ofs);
+    kt(c, KID_SCAN_APPLY);
 }
 void kt_collect(mpm_ctx* c) {
     if (!c->ktiming || c->kt_n == 0) return;
@@ -347,10 +364,7 @@ void enqueue_binning(mpm_ctx* c, const SimParams& P, const Bufs& B) {
     const uint32_t gk = grid_for(P.direct_blocks ? (uint64_t)c->n : std::max<uint64_t>(c->n, c->cap), 8192);
     kt(c, -1);
     k_keys_count<<<gk, 256, 0, s>>>(P, B); kt(c, KID_KEYS);
-    k_scan_tiles<<<c->ntiles, 256, 0, s>>>(P, B); kt(c, KID_SCAN_TILES);
-    const int own_ofs = c->ntiles <= 1024u ? 1 : 0;
-    if (!own_ofs) { k_scan_top<<<1, 256, 0, s>>>(P, B, c->ntiles); kt(c, KID_SCAN_TOP); }
-    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P, B, c->ntiles, own_ofs); kt(c, KID_SCAN_APPLY);
+    enqueue_scan(c, P, B, s);
     const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 4096);
     if (P.direct_blocks) {
         // the neighbour table needs only the scan here: one launch with the scatter
@@ -535,10 +549,7 @@ void enqueue_ens_substep(mpm_ctx* c, bool full) {
     const uint32_t gk = grid_for(c->direct_blocks ? (uint64_t)c->n : std::max<uint64_t>(c->n, c->cap), 8192);
     kt(c, -1);
     k_ens_keys_count<<<gk, 256, 0, s>>>(P0, B, E); kt(c, KID_ENS_KEYS);
-    k_scan_tiles<<<c->ntiles, 256, 0, s>>>(P0, B); kt(c, KID_SCAN_TILES);
-    const int own_ofs = c->ntiles <= 1024u ? 1 : 0;
-    if (!own_ofs) { k_scan_top<<<1, 256, 0, s>>>(P0, B, c->ntiles); kt(c, KID_SCAN_TOP); }
-    k_scan_apply<<<c->ntiles, 256, 0, s>>>(P0, B, c->ntiles, own_ofs); kt(c, KID_SCAN_APPLY);
+    enqueue_scan(c, P0, B, s);
     k_scatter<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P0, B); kt(c, KID_SCATTER);
     const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 8192);
     if (!c->direct_blocks) {
@@ -681,7 +692,7 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     c->direct_blocks = cfg->hash_map_size == 0;
     c->max_runs = (uint32_t)std::min<uint64_t>(c->n, K);
     c->max_chunks = (c->n + c->chunk - 1) / c->chunk + c->max_runs;
-    c->ntiles = (uint32_t)((K + 1023) / 1024);
+    set_scan_tiles(c, K);
     int rc = alloc_buffers(c);
     if (rc) { release(c); delete c; return rc; }
     // uniform fields the runner writes at construction (GpuSnowPipelineRunner.svelte.ts:136-144,171-177)
@@ -1426,7 +1437,7 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
     c->direct_blocks = cfgs[0].hash_map_size == 0;
     c->max_runs = (uint32_t)std::min<uint64_t>(n, K);
     c->max_chunks = (c->n + 255u) / 256u + c->max_runs;      // room for chunks down to 256 particles
-    c->ntiles = (uint32_t)((K + 1023) / 1024);
+    set_scan_tiles(c, K);
     int rc = alloc_buffers(c);
     if (!rc) {
         const size_t S = (size_t)n_scenes;

@@ -132,10 +132,6 @@ MPM_DEV void stm(float* s, uint32_t stride, int plane, uint32_t i, const m3& m) 
 }
 
 // ------------------------------------------------------------------ mpmOps.wgsl:14-53
-MPM_DEV void cell_number(const SimParams& P, v3 pos, int c[3]) {
-    v3 q = cdiv(pos - mk(P.gmin[0], P.gmin[1], P.gmin[2]), mk(P.cd[0], P.cd[1], P.cd[2]));
-    c[0] = f2i(q.x); c[1] = f2i(q.y); c[2] = f2i(q.z);
-}
 MPM_DEV bool in_range(const SimParams& P, int x, int y, int z) {
     return x >= 0 && y >= 0 && z >= 0 && x < P.res[0] && y < P.res[1] && z < P.res[2];
 }
@@ -144,6 +140,13 @@ MPM_DEV bool in_range(const SimParams& P, int x, int y, int z) {
 MPM_DEV v3 div_cd(const SimParams& P, v3 a) {
     if (P.fast_cd) return mk(qdiv(a.x, P.cd[0], P.rcd[0]), qdiv(a.y, P.cd[1], P.rcd[1]), qdiv(a.z, P.cd[2], P.rcd[2]));
     return cdiv(a, mk(P.cd[0], P.cd[1], P.cd[2]));
+}
+// Any position: the verified quotient when every |offset| <= 2^100 (a NaN component, which fmaxf
+// passes over, gives NaN by either quotient), else the IEEE division.
+MPM_DEV void cell_number(const SimParams& P, v3 pos, int c[3]) {
+    const v3 a = pos - mk(P.gmin[0], P.gmin[1], P.gmin[2]);
+    const v3 q = mabs_max(a) <= 0x1p100f ? div_cd(P, a) : cdiv(a, mk(P.cd[0], P.cd[1], P.cd[2]));
+    c[0] = f2i(q.x); c[1] = f2i(q.y); c[2] = f2i(q.z);
 }
 // cell_number for a binned particle (position inside the grid's block range).
 MPM_DEV void cell_number_in(const SimParams& P, v3 pos, int c[3]) {
@@ -184,13 +187,13 @@ MPM_DEV uint32_t touched_mask(const SimParams& P, const int c[3]) {
         }
         s[a] = m;
     }
-    uint32_t mask = 0;
-#pragma unroll
-    for (int j = 0; j < 27; ++j) {
-        int ox = j % 3, oy = (j / 3) % 3, oz = j / 9;
-        if (((s[0] >> ox) & 1u) && ((s[1] >> oy) & 1u) && ((s[2] >> oz) & 1u)) mask |= 1u << j;
-    }
-    return mask;
+    // bit ox + 3 oy + 9 oz is set when bit ox of s[0], oy of s[1] and oz of s[2] are: s[0] repeated
+    // every 3 bits; each bit of s[1] widened to 3 bits, the 9-bit result repeated every 9; each bit
+    // of s[2] widened to 9 bits (the products have no carries)
+    const uint32_t mx = s[0] * 0x1249249u;
+    const uint32_t my = ((s[1] & 1u) | ((s[1] & 2u) << 2) | ((s[1] & 4u) << 4)) * 7u * 0x40201u;
+    const uint32_t mz = ((s[2] & 1u) | ((s[2] & 2u) << 8) | ((s[2] & 4u) << 16)) * 0x1FFu;
+    return mx & my & mz;
 }
 
 MPM_DEV void block_coords(const SimParams& P, uint32_t key, int b[3]) {
@@ -360,18 +363,22 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
     }
 }
 
-// Scan over the key space, 1024 keys per tile. Streams: particle count, run flag, chunk count,
-// allocated-block flag (direct addressing; 0 in hash-table mode).
+// Scan over the key space, 256·KPT keys per tile (KPT consecutive keys per thread). Streams:
+// particle count, run flag, chunk count, allocated-block flag (direct addressing; 0 in hash-table mode).
+// KPT is 1 when that keeps the tile count within SCAN_OWN_MAX (k_scan_apply then sums the tile
+// offsets itself and k_scan_top is not launched), else 4.
+enum : uint32_t { SCAN_OWN_MAX = 512u };
 MPM_DEV void key_streams(const SimParams& P, const Bufs& B, uint32_t k, uint32_t v[4]) {
     const uint32_t c = k < P.K ? B.kcount[k] : 0u;
     v[0] += c; v[1] += c ? 1u : 0u; v[2] += (c + P.chunk - 1) / P.chunk;
     if (P.direct_blocks && k < P.K) v[3] += B.kalloc[k];
 }
+template <uint32_t KPT>
 __global__ __launch_bounds__(256) void k_scan_tiles(SimParams P, Bufs B) {
     __shared__ uint32_t wt[4][4];
-    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+    const uint32_t base = blockIdx.x * 256u * KPT + threadIdx.x * KPT;
     uint32_t v[4] = {0, 0, 0, 0};
-    for (int q = 0; q < 4; ++q) key_streams(P, B, base + q, v);
+    for (uint32_t q = 0; q < KPT; ++q) key_streams(P, B, base + q, v);
     block_sum4(v, wt);
     if (threadIdx.x == 0) for (int c = 0; c < 4; ++c) B.tile_sums[4 * blockIdx.x + c] = v[c];
 }
@@ -397,6 +404,7 @@ __global__ __launch_bounds__(256) void k_scan_top(SimParams P, Bufs B, uint32_t 
 // Writes run descriptors (particle blocks), work chunks, scatter cursors and (direct addressing)
 // the dense block index of every key; zeroes the per-key accumulators for the next substep.
 // own_ofs: the tile offsets are summed here from tile_sums (few tiles), without k_scan_top.
+template <uint32_t KPT>
 __global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B, uint32_t ntiles, int own_ofs) {
     __shared__ uint32_t wt[4][4];
     uint32_t tofs[4];
@@ -411,9 +419,9 @@ __global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B, uint32_
     } else {
         for (int c = 0; c < 4; ++c) tofs[c] = B.tile_ofs[4 * blockIdx.x + c];
     }
-    const uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
-    uint32_t cnt[4], alc[4], v[4] = {0, 0, 0, 0};
-    for (int q = 0; q < 4; ++q) {
+    const uint32_t base = blockIdx.x * 256u * KPT + threadIdx.x * KPT;
+    uint32_t cnt[KPT], alc[KPT], v[4] = {0, 0, 0, 0};
+    for (uint32_t q = 0; q < KPT; ++q) {
         const uint32_t k = base + q;
         const uint32_t c = k < P.K ? B.kcount[k] : 0u;
         cnt[q] = c;
@@ -423,7 +431,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B, uint32_
     uint32_t ex[4], tot[4];
     block_excl_scan4(v, ex, tot, wt);
     uint32_t o0 = ex[0] + tofs[0], o1 = ex[1] + tofs[1], o2 = ex[2] + tofs[2], o3 = ex[3] + tofs[3];
-    for (int q = 0; q < 4; ++q) {
+    for (uint32_t q = 0; q < KPT; ++q) {
         uint32_t k = base + q;
         if (k >= P.K) break;
         uint32_t c = cnt[q];
