@@ -27,14 +27,16 @@ PKG_DIR = ROOT / "webgpu-mpm-based-snow-simulation_amd"
 
 HBM_PEAK_GBS = 8000.0
 # Compulsory particle-state bytes per particle for each transfer kernel of the implemented layout
-# (FP32 SoA planes; reads + writes):
-#   k_pbmpm<FIRST>: read perm, id, x, m, F_e, F_p, D, dx (144 B); write id, x, m, F_e, F_p, relax (96 B)
-#   k_pbmpm<MID>:   read x, m, F_e, relax (56 B)
-#   k_pbmpm<LAST>:  read x, m, F_e, F_p (88 B); write x, v, F_e, F_p, dx, D (144 B)
+# (FP32 SoA planes; reads + writes; F_p lives per particle id and is touched only on a plastic
+# clamp, so the passes carry det F_p instead):
+#   k_pbmpm<FIRST>:    read perm, id, x, m, F_e, det F_p, D, dx (112 B); write id, x, m, F_e, det F_p, relax (64 B)
+#   k_pbmpm<MID>:      read x, m, F_e, relax (56 B)
+#   k_pbmpm<LAST>:     read x, m, F_e (52 B); write x, v, F_e, dx, D (108 B)
+#   k_explicit_p2g:    read perm, id, x, m, v, F_e, det F_p (76 B); write id, x, m, F_e, det F_p (60 B)
+#   k_explicit_g2p:    as k_pbmpm<LAST>
 KERNEL_BYTES_PER_PARTICLE = {
-    "k_pbmpm<FIRST>": 240, "k_pbmpm<MID>": 56, "k_pbmpm<LAST>": 232,
-    "k_explicit_p2g": 4 + 4 + 12 + 4 + 12 + 36 + 36 + 4 + 12 + 4 + 36 + 36,
-    "k_explicit_g2p": 12 + 4 + 36 + 36 + 12 + 12 + 36 + 36 + 12 + 36,
+    "k_pbmpm<FIRST>": 176, "k_pbmpm<MID>": 56, "k_pbmpm<LAST>": 160,
+    "k_explicit_p2g": 136, "k_explicit_g2p": 160,
 }
 # Reference formulation (one substep of PBMPM with k iterations): 268 + 172 k bytes per particle.
 STEP_ALGO_BYTES = lambda k: 268 + 172 * k

@@ -49,6 +49,50 @@ def _scene(mpm, pos, vel, **sim):
     return dict(particles=p, sim=sim, collider=None)
 
 
+def _random_deformations(rng, n, lo, hi):
+    """n FP32 3x3 matrices U diag(s) V^T with s uniform in [lo, hi] (column-major record layout)."""
+    u, _ = np.linalg.qr(rng.normal(size=(n, 3, 3)))
+    v, _ = np.linalg.qr(rng.normal(size=(n, 3, 3)))
+    s = rng.uniform(lo, hi, (n, 3))
+    m = np.einsum("nij,nj,nkj->nik", u, s, v).astype(np.float32)
+    return np.swapaxes(m, 1, 2)
+
+
+@pytest.mark.parametrize("collider", [False, True])
+def test_plasticity_clamps_bitexact(mpm, orc, collider):
+    """Particles whose F_e lies outside the plastic band [1 - theta_c, 1 + theta_s] and whose F_p
+    is not the identity, some of them outside the grid: every substep clamps, so F_p (kept per
+    particle id) and det F_p (which the relax factor and the hardening read) change while the
+    particles are re-sorted. Both methods, with and without the collider (whose launch then
+    integrates the unbinned particles)."""
+    rng = np.random.default_rng(31)
+    n = 4000
+    pos = rng.uniform(0.1, 0.9, (n, 3)).astype(np.float32)
+    pos[:40, 1] = -0.1                     # unbinned: integrated outside the transfer passes
+    vel = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    for method in (1, 0):
+        p = mpm.make_particles(pos, vel=vel)
+        p["Fe"][:, :, :3] = _random_deformations(rng, n, 0.9, 1.1)
+        p["Fp"][:, :, :3] = _random_deformations(rng, n, 0.95, 1.05)
+        sc = dict(particles=p, sim=dict(grid_res=(32, 32, 32), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1.0 / 600,
+                                        gravity=(0.0, -9.8, 0.0), pbmpm_iterations=3, pbmpm_force_divisor=3.0),
+                  collider=mpm.scenes.box_mesh((0.5, 0.5, 0.5), 0.2) if collider else None)
+        if method == 0:
+            sc["sim"]["dt"] = 1e-4
+        sim, o = make_pair(mpm, orc, sc)
+        fp0 = p["Fp"].copy()
+        for s in range(3):
+            sim.step(method, 1)
+            o.step(method, 1)
+            out = sim.download()
+            assert_particles_equal(mpm, out, o.particles(), f"method {method} collider {collider} substep {s}")
+            if s == 0:      # the walls clamp them into the grid for the later substeps
+                assert sim.stats()["particles_unbinned"] == 40
+        changed = np.any(out["Fp"].reshape(n, -1) != fp0.reshape(n, -1), axis=1)
+        assert changed.mean() > 0.5, changed.mean()
+        sim.close()
+
+
 def test_particles_on_and_outside_the_walls(mpm, orc):
     """Particles at exactly gridMax (cell == res, finding 13), outside the grid and a
     non-multiple-of-4 resolution: binned / unbinned sets, grids and states match the oracle."""

@@ -385,39 +385,20 @@ __global__ __launch_bounds__(256) void k_ens_straggle(SimParams P0, Bufs B, Ens 
 
 // Per-scene particle exchange: rows whose original id lies in [base, base + cnt).
 __global__ void k_upload_range(const float* __restrict__ aos, float* __restrict__ s, const uint32_t* __restrict__ pid,
-                               uint32_t nrows, uint32_t stride, uint32_t base, uint32_t cnt) {
+                               float4* __restrict__ fpid, uint32_t nrows, uint32_t stride, uint32_t base, uint32_t cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrows) return;
     const uint32_t id = pid[i] - base;
     if (id >= cnt) return;
-    const float* q = aos + (size_t)id * 48;
-    st(s, stride, PX, i, q[0]); st(s, stride, PY, i, q[1]); st(s, stride, PZ, i, q[2]);
-    st(s, stride, VX, i, q[4]); st(s, stride, VY, i, q[5]); st(s, stride, VZ, i, q[6]); st(s, stride, PM, i, q[7]);
-    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) {
-        st(s, stride, FE + 3 * c + r, i, q[8 + 4 * c + r]);
-        st(s, stride, FP + 3 * c + r, i, q[20 + 4 * c + r]);
-        st(s, stride, DD + 3 * c + r, i, q[36 + 4 * c + r]);
-    }
-    st(s, stride, DXX, i, q[32]); st(s, stride, DXY, i, q[33]); st(s, stride, DXZ, i, q[34]);
+    put_record(aos + (size_t)id * 48, s, stride, fpid, i, pid[i]);
 }
-__global__ void k_download_range(const float* __restrict__ s, const uint32_t* __restrict__ pid, float* __restrict__ aos,
-                                 uint32_t nrows, uint32_t stride, uint32_t base, uint32_t cnt) {
+__global__ void k_download_range(const float* __restrict__ s, const uint32_t* __restrict__ pid, const float4* __restrict__ fpid,
+                                 float* __restrict__ aos, uint32_t nrows, uint32_t stride, uint32_t base, uint32_t cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nrows) return;
     const uint32_t id = pid[i] - base;
     if (id >= cnt) return;
-    float* q = aos + (size_t)id * 48;
-    q[0] = ld(s, stride, PX, i); q[1] = ld(s, stride, PY, i); q[2] = ld(s, stride, PZ, i); q[3] = 1.0f;
-    q[4] = ld(s, stride, VX, i); q[5] = ld(s, stride, VY, i); q[6] = ld(s, stride, VZ, i); q[7] = ld(s, stride, PM, i);
-    for (int c = 0; c < 3; ++c) {
-        for (int r = 0; r < 3; ++r) {
-            q[8 + 4 * c + r] = ld(s, stride, FE + 3 * c + r, i);
-            q[20 + 4 * c + r] = ld(s, stride, FP + 3 * c + r, i);
-            q[36 + 4 * c + r] = ld(s, stride, DD + 3 * c + r, i);
-        }
-        q[11 + 4 * c] = 0.0f; q[23 + 4 * c] = 0.0f; q[39 + 4 * c] = 0.0f;
-    }
-    q[32] = ld(s, stride, DXX, i); q[33] = ld(s, stride, DXY, i); q[34] = ld(s, stride, DXZ, i); q[35] = 0.0f;
+    get_record(aos + (size_t)id * 48, s, stride, fpid, i, pid[i]);
 }
 __global__ void k_iota(uint32_t* p, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;

@@ -36,6 +36,7 @@ struct mpm_ctx {
     uint32_t chunk = 512;
     float* planes[2] = {nullptr, nullptr};
     uint32_t* pid[2] = {nullptr, nullptr};
+    float4* fpid = nullptr;                   // F_p by particle id (3 columns each)
     int cur = 0;
     float* relax = nullptr;
     uint32_t *key = nullptr, *mask = nullptr, *perm = nullptr, *kcount = nullptr, *kmask = nullptr, *kcursor = nullptr;
@@ -113,6 +114,7 @@ struct mpm_ctx {
     bool ens_graph_ok[2] = {false, false};
     float* snap_planes = nullptr;
     uint32_t* snap_pid = nullptr;
+    float4* snap_fpid = nullptr;
     // render-side splat grid
     uint32_t* splat = nullptr;
     size_t splat_cells = 0;
@@ -202,7 +204,7 @@ MlpBufs mlp_bufs(mpm_ctx* c) {
 Bufs make_bufs(mpm_ctx* c) {
     Bufs B;
     B.cur = c->planes[c->cur]; B.nxt = c->planes[c->cur ^ 1];
-    B.pid_cur = c->pid[c->cur]; B.pid_nxt = c->pid[c->cur ^ 1];
+    B.pid_cur = c->pid[c->cur]; B.pid_nxt = c->pid[c->cur ^ 1]; B.fpid = c->fpid;
     B.stride = c->stride;
     B.relax = c->relax; B.key = c->key; B.mask = c->mask; B.perm = c->perm; B.lofs = c->lofs;
     B.kcount = c->kcount; B.kmask = c->kmask; B.kcursor = c->kcursor;
@@ -500,6 +502,7 @@ int alloc_buffers(mpm_ctx* c) {
     const size_t ps = (size_t)NPLANES * c->stride;
     if ((rc = dalloc(c, &c->planes[0], ps)) || (rc = dalloc(c, &c->planes[1], ps)) ||
         (rc = dalloc(c, &c->pid[0], c->stride)) || (rc = dalloc(c, &c->pid[1], c->stride)) ||
+        (rc = dalloc(c, &c->fpid, 3 * (size_t)c->stride)) ||
         (rc = dalloc(c, &c->relax, c->stride)) || (rc = dalloc(c, &c->key, c->stride)) || (rc = dalloc(c, &c->lofs, c->stride)) ||
         (rc = dalloc(c, &c->mask, c->stride)) || (rc = dalloc(c, &c->perm, c->stride)) ||
         (rc = dalloc(c, &c->kcount, K)) || (rc = dalloc(c, &c->kmask, K)) || (rc = dalloc(c, &c->kcursor, K)) ||
@@ -848,7 +851,7 @@ int mpm_upload_particles(mpm_ctx* c, const void* aos, size_t n) {
     HIPCHK(c, hipMalloc(&tmp, n * MPM_PARTICLE_BYTES));
     hipError_t e = hipMemcpyAsync(tmp, aos, n * MPM_PARTICLE_BYTES, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        k_upload<<<grid_for(n, 1u << 30), 256, 0, c->stream>>>(tmp, c->planes[c->cur], c->pid[c->cur], c->n, c->stride);
+        k_upload<<<grid_for(n, 1u << 30), 256, 0, c->stream>>>(tmp, c->planes[c->cur], c->pid[c->cur], c->fpid, c->n, c->stride);
         e = hipStreamSynchronize(c->stream);
     }
     hipFree(tmp);
@@ -869,7 +872,7 @@ int mpm_scatter_particles(mpm_ctx* c, const float* mesh_xyzw, size_t n_vertices)
     hipError_t e = hipMemcpyAsync(dm, mesh_xyzw, n_vertices * 16, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
         k_scatter_init<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(dm, (uint32_t)(n_vertices / 3), mn[0], mn[1], mn[2], mx[0], mx[1], mx[2],
-                                                                        c->planes[c->cur], c->pid[c->cur], c->n, c->stride);
+                                                                        c->planes[c->cur], c->pid[c->cur], c->fpid, c->n, c->stride);
         e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     }
@@ -981,7 +984,7 @@ int mpm_download_particles(mpm_ctx* c, void* aos, size_t n) {
     HIPCHK(c, hipSetDevice(c->device));
     float* tmp = nullptr;
     HIPCHK(c, hipMalloc(&tmp, n * MPM_PARTICLE_BYTES));
-    k_download<<<grid_for(n, 1u << 30), 256, 0, c->stream>>>(c->planes[c->cur], c->pid[c->cur], tmp, c->n, c->stride);
+    k_download<<<grid_for(n, 1u << 30), 256, 0, c->stream>>>(c->planes[c->cur], c->pid[c->cur], c->fpid, tmp, c->n, c->stride);
     hipError_t e = hipMemcpyAsync(aos, tmp, n * MPM_PARTICLE_BYTES, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(tmp);
@@ -1519,7 +1522,7 @@ int mpm_ensemble_upload(mpm_ctx* c, int scene, const void* aos, size_t n) {
     HIPCHK(c, hipMalloc(&tmp, n * MPM_PARTICLE_BYTES));
     hipError_t e = hipMemcpyAsync(tmp, aos, n * MPM_PARTICLE_BYTES, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) {
-        k_upload_range<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(tmp, c->planes[c->cur], c->pid[c->cur], c->n, c->stride, base, cnt);
+        k_upload_range<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(tmp, c->planes[c->cur], c->pid[c->cur], c->fpid, c->n, c->stride, base, cnt);
         e = hipStreamSynchronize(c->stream);
     }
     hipFree(tmp);
@@ -1537,7 +1540,7 @@ int mpm_ensemble_download(mpm_ctx* c, int scene, void* aos, size_t n) {
     HIPCHK(c, hipSetDevice(c->device));
     float* tmp = nullptr;
     HIPCHK(c, hipMalloc(&tmp, n * MPM_PARTICLE_BYTES));
-    k_download_range<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(c->planes[c->cur], c->pid[c->cur], tmp, c->n, c->stride, base, cnt);
+    k_download_range<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(c->planes[c->cur], c->pid[c->cur], c->fpid, tmp, c->n, c->stride, base, cnt);
     hipError_t e = hipMemcpyAsync(aos, tmp, n * MPM_PARTICLE_BYTES, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     hipFree(tmp);
@@ -1614,10 +1617,12 @@ int mpm_ensemble_autotune(mpm_ctx* c, int trial_substeps, mpm_tuning* out, int c
     int rc;
     const size_t ps = (size_t)NPLANES * c->stride;
     if (!c->snap_planes) {
-        if ((rc = dalloc(c, &c->snap_planes, ps)) || (rc = dalloc(c, &c->snap_pid, c->stride))) return rc;
+        if ((rc = dalloc(c, &c->snap_planes, ps)) || (rc = dalloc(c, &c->snap_pid, c->stride)) ||
+            (rc = dalloc(c, &c->snap_fpid, 3 * (size_t)c->stride))) return rc;
     }
     HIPCHK(c, hipMemcpy(c->snap_planes, c->planes[c->cur], ps * 4, hipMemcpyDeviceToDevice));
     HIPCHK(c, hipMemcpy(c->snap_pid, c->pid[c->cur], (size_t)c->stride * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(c, hipMemcpy(c->snap_fpid, c->fpid, (size_t)c->stride * 48, hipMemcpyDeviceToDevice));
     const int cur0 = c->cur;
     const uint64_t sub0 = c->substeps;
     const uint32_t chunks[] = {256, 512, 1024};
@@ -1634,10 +1639,12 @@ int mpm_ensemble_autotune(mpm_ctx* c, int trial_substeps, mpm_tuning* out, int c
         if ((rc = mpm_ensemble_set_tuning(c, &cand[i]))) break;
         HIPCHK(c, hipMemcpy(c->planes[cur0], c->snap_planes, ps * 4, hipMemcpyDeviceToDevice));
         HIPCHK(c, hipMemcpy(c->pid[cur0], c->snap_pid, (size_t)c->stride * 4, hipMemcpyDeviceToDevice));
+        HIPCHK(c, hipMemcpy(c->fpid, c->snap_fpid, (size_t)c->stride * 48, hipMemcpyDeviceToDevice));
         c->cur = cur0;
         if ((rc = mpm_ensemble_step(c, 2))) break;                   // builds both graph parities
         HIPCHK(c, hipMemcpy(c->planes[cur0], c->snap_planes, ps * 4, hipMemcpyDeviceToDevice));
         HIPCHK(c, hipMemcpy(c->pid[cur0], c->snap_pid, (size_t)c->stride * 4, hipMemcpyDeviceToDevice));
+        HIPCHK(c, hipMemcpy(c->fpid, c->snap_fpid, (size_t)c->stride * 48, hipMemcpyDeviceToDevice));
         c->cur = cur0;
         HIPCHK(c, hipEventRecord(e0, c->stream));
         if ((rc = mpm_ensemble_step(c, trial_substeps))) break;
@@ -1654,6 +1661,7 @@ int mpm_ensemble_autotune(mpm_ctx* c, int trial_substeps, mpm_tuning* out, int c
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(c->planes[cur0], c->snap_planes, ps * 4, hipMemcpyDeviceToDevice));
     HIPCHK(c, hipMemcpy(c->pid[cur0], c->snap_pid, (size_t)c->stride * 4, hipMemcpyDeviceToDevice));
+    HIPCHK(c, hipMemcpy(c->fpid, c->snap_fpid, (size_t)c->stride * 48, hipMemcpyDeviceToDevice));
     c->cur = cur0;
     c->substeps = sub0;
     if (rc) return rc;

@@ -29,11 +29,14 @@ namespace mpm {
 
 enum : uint32_t { EMPTY = 0xFFFFFFFFu };
 
+// Particle state in sorted order, one FP32 plane per component. F_p is not among them: it changes
+// only when plasticity clamps a singular value, so it is kept per particle id (Bufs::fpid) and the
+// passes that need only det F_p read the JP plane, which holds det3(F_p) of the same bits.
 enum Plane : int {
     PX = 0, PY, PZ, PM, VX, VY, VZ, DXX, DXY, DXZ,
-    FE = 10, FP = 19, DD = 28,
-    MU0 = 37, LA0 = 38,        // optional per-particle Lame parameters (explicit method)
-    NPLANES = 39
+    FE = 10, JP = 19, DD = 20,
+    MU0 = 29, LA0 = 30,        // optional per-particle Lame parameters (explicit method)
+    NPLANES = 31
 };
 
 enum Counter : int {
@@ -85,6 +88,7 @@ struct SimParams {
 
 struct Bufs {
     float* cur; float* nxt; uint32_t* pid_cur; uint32_t* pid_nxt; uint32_t stride;
+    float4* fpid;              // F_p by particle id: 3 columns per id (w = 0)
     float* relax;
     uint32_t* key; uint32_t* mask; uint32_t* perm;
     uint32_t* lofs;            // per particle: its offset among the particles of its key (from the count atomic)
@@ -112,7 +116,7 @@ MPM_DEV int4* grid_sel(const Bufs& B, int i) { return i == 0 ? B.grid0 : (i == 1
 // ------------------------------------------------------------------ particle record in registers
 struct Part {
     v3 pos, vel; float mass;
-    m3 Fe, Fp, D;
+    m3 Fe, D;
     v3 dx;
 };
 
@@ -129,6 +133,16 @@ MPM_DEV m3 ldm(const float* s, uint32_t stride, int plane, uint32_t i) {
 }
 MPM_DEV void stm(float* s, uint32_t stride, int plane, uint32_t i, const m3& m) {
     st3(s, stride, plane, i, m.c[0]); st3(s, stride, plane + 3, i, m.c[1]); st3(s, stride, plane + 6, i, m.c[2]);
+}
+MPM_DEV m3 ld_fp(const float4* fpid, uint32_t id) {
+    m3 m;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { const float4 v = fpid[3u * id + c]; m.c[c] = mk(v.x, v.y, v.z); }
+    return m;
+}
+MPM_DEV void st_fp(float4* fpid, uint32_t id, const m3& m) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fpid[3u * id + c] = make_float4(m.c[c].x, m.c[c].y, m.c[c].z, 0.0f);
 }
 
 // ------------------------------------------------------------------ mpmOps.wgsl:14-53
@@ -251,38 +265,52 @@ MPM_DEV void block_sum4(uint32_t v[4], uint32_t (*wt)[4]) {
 }
 
 // ================================================================== binning
-// Upload: 192-byte ParticleData records (mpmOps.wgsl:1-12) -> SoA planes, ids = record index.
-__global__ void k_upload(const float* __restrict__ aos, float* __restrict__ s, uint32_t* __restrict__ pid, uint32_t n, uint32_t stride) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float* q = aos + (size_t)i * 48;
+// One 192-byte ParticleData record (mpmOps.wgsl:1-12) into row i of the planes and F_p into id.
+MPM_DEV void put_record(const float* q, float* s, uint32_t stride, float4* fpid, uint32_t i, uint32_t id) {
     st(s, stride, PX, i, q[0]); st(s, stride, PY, i, q[1]); st(s, stride, PZ, i, q[2]);
     st(s, stride, VX, i, q[4]); st(s, stride, VY, i, q[5]); st(s, stride, VZ, i, q[6]); st(s, stride, PM, i, q[7]);
-    for (int c = 0; c < 3; ++c) for (int r = 0; r < 3; ++r) {
-        st(s, stride, FE + 3 * c + r, i, q[8 + 4 * c + r]);
-        st(s, stride, FP + 3 * c + r, i, q[20 + 4 * c + r]);
-        st(s, stride, DD + 3 * c + r, i, q[36 + 4 * c + r]);
+    m3 Fp;
+    for (int c = 0; c < 3; ++c) {
+        Fp.c[c] = mk(q[20 + 4 * c], q[21 + 4 * c], q[22 + 4 * c]);
+        for (int r = 0; r < 3; ++r) {
+            st(s, stride, FE + 3 * c + r, i, q[8 + 4 * c + r]);
+            st(s, stride, DD + 3 * c + r, i, q[36 + 4 * c + r]);
+        }
     }
+    st_fp(fpid, id, Fp);
+    st(s, stride, JP, i, det3(Fp));
     st(s, stride, DXX, i, q[32]); st(s, stride, DXY, i, q[33]); st(s, stride, DXZ, i, q[34]);
+}
+// Row i of the planes and F_p of id into one ParticleData record.
+MPM_DEV void get_record(float* q, const float* s, uint32_t stride, const float4* fpid, uint32_t i, uint32_t id) {
+    q[0] = ld(s, stride, PX, i); q[1] = ld(s, stride, PY, i); q[2] = ld(s, stride, PZ, i); q[3] = 1.0f;
+    q[4] = ld(s, stride, VX, i); q[5] = ld(s, stride, VY, i); q[6] = ld(s, stride, VZ, i); q[7] = ld(s, stride, PM, i);
+    const m3 Fp = ld_fp(fpid, id);
+    for (int c = 0; c < 3; ++c) {
+        for (int r = 0; r < 3; ++r) {
+            q[8 + 4 * c + r] = ld(s, stride, FE + 3 * c + r, i);
+            q[36 + 4 * c + r] = ld(s, stride, DD + 3 * c + r, i);
+        }
+        q[20 + 4 * c] = Fp.c[c].x; q[21 + 4 * c] = Fp.c[c].y; q[22 + 4 * c] = Fp.c[c].z;
+        q[11 + 4 * c] = 0.0f; q[23 + 4 * c] = 0.0f; q[39 + 4 * c] = 0.0f;
+    }
+    q[32] = ld(s, stride, DXX, i); q[33] = ld(s, stride, DXY, i); q[34] = ld(s, stride, DXZ, i); q[35] = 0.0f;
+}
+// Upload: ParticleData records -> SoA planes, ids = record index.
+__global__ void k_upload(const float* __restrict__ aos, float* __restrict__ s, uint32_t* __restrict__ pid,
+                         float4* __restrict__ fpid, uint32_t n, uint32_t stride) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    put_record(aos + (size_t)i * 48, s, stride, fpid, i, i);
     pid[i] = i;
 }
-__global__ void k_download(const float* __restrict__ s, const uint32_t* __restrict__ pid, float* __restrict__ aos, uint32_t n, uint32_t stride) {
+__global__ void k_download(const float* __restrict__ s, const uint32_t* __restrict__ pid, const float4* __restrict__ fpid,
+                           float* __restrict__ aos, uint32_t n, uint32_t stride) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t id = pid[i];
     if (id >= n) return;
-    float* q = aos + (size_t)id * 48;
-    q[0] = ld(s, stride, PX, i); q[1] = ld(s, stride, PY, i); q[2] = ld(s, stride, PZ, i); q[3] = 1.0f;
-    q[4] = ld(s, stride, VX, i); q[5] = ld(s, stride, VY, i); q[6] = ld(s, stride, VZ, i); q[7] = ld(s, stride, PM, i);
-    for (int c = 0; c < 3; ++c) {
-        for (int r = 0; r < 3; ++r) {
-            q[8 + 4 * c + r] = ld(s, stride, FE + 3 * c + r, i);
-            q[20 + 4 * c + r] = ld(s, stride, FP + 3 * c + r, i);
-            q[36 + 4 * c + r] = ld(s, stride, DD + 3 * c + r, i);
-        }
-        q[11 + 4 * c] = 0.0f; q[23 + 4 * c] = 0.0f; q[39 + 4 * c] = 0.0f;
-    }
-    q[32] = ld(s, stride, DXX, i); q[33] = ld(s, stride, DXY, i); q[34] = ld(s, stride, DXZ, i); q[35] = 0.0f;
+    get_record(aos + (size_t)id * 48, s, stride, fpid, i, id);
 }
 
 // Per-particle Lame parameters given by original particle id, written to the rows holding them.
@@ -589,28 +617,9 @@ MPM_DEV void nbr_pass(const SimParams& P, const Bufs& B) {
 __global__ __launch_bounds__(256) void k_nbr(SimParams P, Bufs B) { nbr_pass(P, B); }
 
 // ================================================================== particle update helpers
-// plasticityOps.wgsl:7-48
-MPM_DEV void plasticity(const SimParams& P, Part& p) {
-    const m3 dE = p.Fe, dP = p.Fp;
-    const m3 rot = polar_rotation(dE);
-    const m3 sc = mmul(transpose(rot), dE);
-    const float lo = P.plast_lo, hi = P.plast_hi;
-    const float s0 = sc.c[0].x, s1 = sc.c[1].y, s2 = sc.c[2].z;
-    if (lo <= s0 && lo <= s1 && lo <= s2 && s0 <= hi && s1 <= hi && s2 <= hi) return;
-    float t0 = s0 < lo ? lo : s0; t0 = hi < t0 ? hi : t0;
-    float t1 = s1 < lo ? lo : s1; t1 = hi < t1 ? hi : t1;
-    float t2 = s2 < lo ? lo : s2; t2 = hi < t2 ? hi : t2;
-    m3 ns = zero3(); ns.c[0].x = t0; ns.c[1].y = t1; ns.c[2].z = t2;
-    const m3 nFe = mmul(rot, ns);
-    const m3 nFeInv = inverse3(nFe);
-    const m3 F = mmul(dE, dP);
-    p.Fe = nFe;
-    p.Fp = mmul(nFeInv, F);
-}
-
-// plasticityOps.wgsl:7-48 for row q of nxt, which already holds the particle's F_p: F_e is
-// updated in place; F_p is read and rewritten only when a singular value is clamped (otherwise the
-// reference leaves both unchanged). Same bits as plasticity().
+// plasticityOps.wgsl:7-48 for the particle at row q of nxt (its id already in pid_nxt[q]): F_e is
+// updated in place; F_p (by id) and its determinant (JP plane, row q) are read and rewritten only
+// when a singular value is clamped, as the reference leaves both unchanged otherwise.
 MPM_DEV void plasticity_row(const SimParams& P, const Bufs& B, uint32_t q, m3& Fe) {
     const m3 dE = Fe;
     const m3 rot = polar_rotation(dE);
@@ -624,9 +633,12 @@ MPM_DEV void plasticity_row(const SimParams& P, const Bufs& B, uint32_t q, m3& F
     m3 ns = zero3(); ns.c[0].x = t0; ns.c[1].y = t1; ns.c[2].z = t2;
     const m3 nFe = mmul(rot, ns);
     const m3 nFeInv = inverse3(nFe);
-    const m3 F = mmul(dE, ldm(B.nxt, B.stride, FP, q));
+    const uint32_t id = B.pid_nxt[q];
+    const m3 F = mmul(dE, ld_fp(B.fpid, id));
     Fe = nFe;
-    stm(B.nxt, B.stride, FP, q, mmul(nFeInv, F));
+    const m3 nFp = mmul(nFeInv, F);
+    st_fp(B.fpid, id, nFp);
+    st(B.nxt, B.stride, JP, q, det3(nFp));
 }
 
 MPM_DEV v3 xform(const float* T, v3 p) {
@@ -808,12 +820,13 @@ MPM_DEV void resolve_collision_narrow(const SimParams& P, const float4* __restri
         pt.pos = cl_p + pd * margin;
     }
 }
-// integrateParticles.wgsl:12-66 — everything before resolveParticleCollision. Returns whether the
-// particle is a collider candidate (its narrow phase then runs in k_collide).
+// integrateParticles.wgsl:12-66 — everything before resolveParticleCollision except plasticity,
+// which touches only F_e and F_p and is applied by plasticity_row (the motion part touches only x,
+// dx and v, so the order of the two does not matter). Returns whether the particle is a collider
+// candidate.
 MPM_DEV bool integrate_body(const SimParams& P, Part& p) {
     p.pos = p.pos + p.dx;
     p.Fe = mmul(madd(ident(), p.D), p.Fe);
-    plasticity(P, p);
     float* pp = &p.pos.x; float* dd = &p.dx.x; float* vv = &p.vel.x;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -826,8 +839,7 @@ MPM_DEV bool integrate_body(const SimParams& P, Part& p) {
 // The motion part (x += dx, walls, collision candidate) touches only x, dx, v and the deformation
 // part (F_e = (I + D) F_e, plasticity) only F_e, F_p, so the two are independent and are done in
 // the order that keeps the fewest values live: F_e update, stores of D, motion and its stores,
-// then plasticity (F_p stays in row q unless clamped; the pass before copied it there). The
-// results are the same bits as integrate_body.
+// then plasticity.
 MPM_DEV bool integrate_store(const SimParams& P, const Bufs& B, Part& p, uint32_t q) {
     const uint32_t sd = B.stride;
     p.Fe = mmul(madd(ident(), p.D), p.Fe);
@@ -845,9 +857,11 @@ MPM_DEV bool integrate_store(const SimParams& P, const Bufs& B, Part& p, uint32_
     stm(B.nxt, sd, FE, q, p.Fe);
     return cand;
 }
-// integrateParticles.wgsl:12-72 in one piece.
-MPM_DEV void integrate(const SimParams& P, const Bufs& B, Part& p) {
-    if (integrate_body(P, p)) resolve_collision_narrow(P, B.crec, B.ctri, p);
+// integrateParticles.wgsl:12-72 in one piece for the particle at row q of nxt.
+MPM_DEV void integrate(const SimParams& P, const Bufs& B, Part& p, uint32_t q) {
+    const bool cand = integrate_body(P, p);
+    plasticity_row(P, B, q, p.Fe);
+    if (cand) resolve_collision_narrow(P, B.crec, B.ctri, p);
 }
 // Appends sorted index q to the collider queue (one atomic per wave).
 MPM_DEV void queue_collider(const Bufs& B, bool cand, uint32_t q) {
@@ -905,8 +919,7 @@ MPM_DEV void solve_constraints(const SimParams& P, const m3& Fe, float relax, m3
     const m3 diff = msub(corr, D);
     D = madd(D, mscale(relax, diff));
 }
-MPM_DEV float pbmpm_relax(const SimParams& P, const m3& Fp) {
-    const float Jp = det3(Fp);
+MPM_DEV float pbmpm_relax(const SimParams& P, float Jp) {
     return 0.5f + 0.5f * (1.0f - fexp(-P.xi_p / Jp));
 }
 
@@ -1254,14 +1267,14 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 const v3 pos = ld3(B.cur, sd, PX, old);
                 const float mass = ld(B.cur, sd, PM, old);
                 const m3 Fe = ldm(B.cur, sd, FE, old);
-                const m3 Fp = ldm(B.cur, sd, FP, old);
+                const float Jp = ld(B.cur, sd, JP, old);
                 m3 D = ldm(B.cur, sd, DD, old);
                 const v3 dx = ld3(B.cur, sd, DXX, old);
                 B.pid_nxt[q] = B.pid_cur[old];
                 st3(B.nxt, sd, PX, q, pos); st(B.nxt, sd, PM, q, mass);
-                stm(B.nxt, sd, FE, q, Fe); stm(B.nxt, sd, FP, q, Fp);
+                stm(B.nxt, sd, FE, q, Fe); st(B.nxt, sd, JP, q, Jp);
                 if (P.per_particle_lame) { st(B.nxt, sd, MU0, q, ld(B.cur, sd, MU0, old)); st(B.nxt, sd, LA0, q, ld(B.cur, sd, LA0, old)); }
-                const float relax = pbmpm_relax(P, Fp);
+                const float relax = pbmpm_relax(P, Jp);
                 B.relax[q] = relax;
                 if (ok) {
                     solve_constraints(P, Fe, relax, D);
@@ -1353,10 +1366,10 @@ MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             const float mass = ld(B.cur, sd, PM, old);
             const v3 vel = ld3(B.cur, sd, VX, old);
             const m3 Fe = ldm(B.cur, sd, FE, old);
-            const m3 Fp = ldm(B.cur, sd, FP, old);
+            const float Jp = ld(B.cur, sd, JP, old);
             B.pid_nxt[q] = B.pid_cur[old];
             st3(B.nxt, sd, PX, q, pos); st(B.nxt, sd, PM, q, mass);
-            stm(B.nxt, sd, FE, q, Fe); stm(B.nxt, sd, FP, q, Fp);
+            stm(B.nxt, sd, FE, q, Fe); st(B.nxt, sd, JP, q, Jp);
             float mu = P.mu0, la = P.la0;
             if (P.per_particle_lame) {
                 mu = ld(B.cur, sd, MU0, old); la = ld(B.cur, sd, LA0, old);
@@ -1366,7 +1379,7 @@ MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
                 st3(B.nxt, sd, VX, q, vel); st3(B.nxt, sd, DXX, q, ld3(B.cur, sd, DXX, old)); stm(B.nxt, sd, DD, q, ldm(B.cur, sd, DD, old));
                 continue;
             }
-            const float ef = fexp(P.xi_e * (1.0f - det3(Fp)));
+            const float ef = fexp(P.xi_e * (1.0f - Jp));
             mu = mu * ef; la = la * ef;
             const float J = det3(Fe);
             const m3 R = polar_rotation(Fe);
@@ -1499,13 +1512,14 @@ MPM_DEV void straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_
         B.pid_nxt[q] = B.pid_cur[old];
         Part p;
         p.pos = ld3(B.cur, sd, PX, old); p.vel = ld3(B.cur, sd, VX, old); p.mass = ld(B.cur, sd, PM, old);
-        p.Fe = ldm(B.cur, sd, FE, old); p.Fp = ldm(B.cur, sd, FP, old); p.D = ldm(B.cur, sd, DD, old);
+        p.Fe = ldm(B.cur, sd, FE, old); p.D = ldm(B.cur, sd, DD, old);
         p.dx = ld3(B.cur, sd, DXX, old);
         st(B.nxt, sd, PM, q, p.mass);
+        st(B.nxt, sd, JP, q, ld(B.cur, sd, JP, old));
         if (P.per_particle_lame) { st(B.nxt, sd, MU0, q, ld(B.cur, sd, MU0, old)); st(B.nxt, sd, LA0, q, ld(B.cur, sd, LA0, old)); }
-        integrate(P, B, p);
+        integrate(P, B, p, q);
         st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
-        stm(B.nxt, sd, FE, q, p.Fe); stm(B.nxt, sd, FP, q, p.Fp);
+        stm(B.nxt, sd, FE, q, p.Fe);
         st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
     }
 }

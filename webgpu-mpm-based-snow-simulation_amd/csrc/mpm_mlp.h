@@ -395,11 +395,11 @@ __global__ __launch_bounds__(256) void k_mlp_grad_fixed(MlpBufs M, const float* 
 }
 
 // Features of a particle at the start of a substep (see the header comment).
-MPM_DEV void mlp_features(const m3& Fe, const m3& Fp, const m3& D, v3 vel, float mass, float f[16]) {
+MPM_DEV void mlp_features(const m3& Fe, float Jp, const m3& D, v3 vel, float mass, float f[16]) {
     const m3 R = polar_rotation(Fe);
     const m3 S = mmul(transpose(R), Fe);
     f[0] = S.c[0].x; f[1] = S.c[1].y; f[2] = S.c[2].z;
-    f[3] = det3(Fp);
+    f[3] = Jp;
     for (int c = 0; c < 3; ++c) { f[4 + 3 * c] = D.c[c].x; f[5 + 3 * c] = D.c[c].y; f[6 + 3 * c] = D.c[c].z; }
     f[13] = length(vel);
     f[14] = mass;
@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_mlp_log_features(Bufs B, uint32_t n, fl
     if (i >= n) return;
     const uint32_t sd = B.stride;
     float f[16];
-    mlp_features(ldm(B.cur, sd, FE, i), ldm(B.cur, sd, FP, i), ldm(B.cur, sd, DD, i), ld3(B.cur, sd, VX, i), ld(B.cur, sd, PM, i), f);
+    mlp_features(ldm(B.cur, sd, FE, i), ld(B.cur, sd, JP, i), ldm(B.cur, sd, DD, i), ld3(B.cur, sd, VX, i), ld(B.cur, sd, PM, i), f);
     float4* o = reinterpret_cast<float4*>(feats + (base + B.pid_cur[i]) * 16);
     o[0] = make_float4(f[0], f[1], f[2], f[3]); o[1] = make_float4(f[4], f[5], f[6], f[7]);
     o[2] = make_float4(f[8], f[9], f[10], f[11]); o[3] = make_float4(f[12], f[13], f[14], f[15]);
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(256) void k_mlp_warmstart(MlpBufs M, Bufs B, uint32
         m3 D = zero3();
         if (i < n) {
             D = ldm(B.cur, sd, DD, i);
-            mlp_features(ldm(B.cur, sd, FE, i), ldm(B.cur, sd, FP, i), D, ld3(B.cur, sd, VX, i), ld(B.cur, sd, PM, i), f);
+            mlp_features(ldm(B.cur, sd, FE, i), ld(B.cur, sd, JP, i), D, ld3(B.cur, sd, VX, i), ld(B.cur, sd, PM, i), f);
         } else {
             for (int c = 0; c < 16; ++c) f[c] = L.mu_x[c];
         }

@@ -71,7 +71,7 @@ MPM_DEV float fbm(v3 x) {
 // One particle record per thread, written to SoA row i (original order, id i).
 __global__ __launch_bounds__(256) void k_scatter_init(const float4* __restrict__ mesh, uint32_t ntri, float mn_x, float mn_y,
                                                       float mn_z, float mx_x, float mx_y, float mx_z, float* __restrict__ s,
-                                                      uint32_t* __restrict__ pid, uint32_t n, uint32_t stride) {
+                                                      uint32_t* __restrict__ pid, float4* __restrict__ fpid, uint32_t n, uint32_t stride) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t seed = hash4(i, 100u, 200u, 300u);
@@ -93,7 +93,8 @@ __global__ __launch_bounds__(256) void k_scatter_init(const float4* __restrict__
     st3(s, stride, VX, i, mk(0.0f, 0.0f, 0.0f));
     st(s, stride, PM, i, m1 * m1);
     stm(s, stride, FE, i, ident());
-    stm(s, stride, FP, i, ident());
+    st_fp(fpid, i, ident());
+    st(s, stride, JP, i, det3(ident()));
     st3(s, stride, DXX, i, mk(0.0f, 0.0f, 0.0f));
     stm(s, stride, DD, i, zero3());
     pid[i] = i;
