@@ -127,6 +127,43 @@ def test_explicit_stages_and_substeps_bitexact(mpm, orc):
     sim.close()
 
 
+@pytest.mark.parametrize("case", ["scale_not_pow2", "huge_momentum_lanes"])
+def test_explicit_p2g_general_form_bitexact(mpm, orc, case):
+    """Explicit P2G folds the fixed-point scale into momentum, mass and dt only for S = 2^k and
+    particles with |m|, |m v|, |V P F^T| <= 2^40; everything else takes the general form, per lane.
+    Deformed particles (F_e off the identity, so the stress term is live) against the oracle: a
+    scale of 10^5 (every particle general), and a few lanes of momentum 2^41 among folded ones."""
+    rng = np.random.default_rng(15)
+    sc = mpm.scenes.scene("explicit_small", n_particles=6000, seed=15)
+    p = sc["particles"].copy()
+    n = p.shape[0]
+    u, _ = np.linalg.qr(rng.normal(size=(n, 3, 3)))
+    sv = rng.uniform(0.97, 1.01, (n, 3))
+    p["Fe"][:, :, :3] = np.swapaxes(np.einsum("nij,nj,nkj->nik", u, sv, u).astype(np.float32), 1, 2)
+    if case == "huge_momentum_lanes":
+        p["vel"][::97, 0] = np.float32(2.0 ** 41)
+    sc = dict(sc, particles=p)
+
+    def pair():
+        sim, o = make_pair(mpm, orc, sc)
+        if case == "scale_not_pow2":
+            sim.uniformsManager.writeFixedPointScale(1e5)
+            o.set_uniforms(sim.uniform_bytes())
+        return sim, o
+
+    sim, o = pair()
+    sim.debug_stage(0, 1)
+    o.run_stage(0, 1)
+    assert_grid_equal(sim, o, f"{case} P2G")
+    sim.close()
+    sim, o = pair()
+    for s in range(3):
+        sim.step(0, 1)
+        o.step(0, 1)
+        assert_particles_equal(mpm, sim.download(), o.particles(), f"{case} substep {s + 1}")
+    sim.close()
+
+
 def test_graph_replay_matches_eager(mpm, orc):
     sc = _small_pbmpm(mpm, n=6000, seed=13)
     a, _ = make_pair(mpm, orc, sc, use_graph=1)

@@ -70,6 +70,9 @@ struct SimParams {
     uint32_t p2g_fold;         // PBMPM P2G may fold S into the mass (S = 2^k <= 2^16, quot_i32 verified
                                // for dt, no pointer force, gravity along y only with |grav_y / force_div|
                                // <= 2^40, 2^-20 <= dt <= 2^20)
+    uint32_t ex_fold;          // explicit P2G may fold S into momentum, mass and dt (S = 2^k <= 2^16,
+                               // dt S <= 2^20, cell dims >= 2^-20)
+    float dtS;                 // dt S (exact for S = 2^k)
     uint32_t grav_nz[3];       // grav_div component is nonzero (zero ones contribute i32 0 exactly)
     uint32_t wall_mode, wall_cells;
     int32_t k;                 // PBMPM iterations of this substep
@@ -1352,6 +1355,45 @@ __global__ __launch_bounds__(256, MODE == PB_MID ? 5 : 4) void k_pbmpm(SimParams
 
 // ------------------------------------------------------------------ explicit MPM
 // particleToGrid.cs.wgsl:27-79 (hardening stressTensorOps.wgsl:39-56, PK1 stress :9-22)
+// particleToGrid.cs.wgsl:47-77 for one particle into the LDS accumulators: node mass and momentum
+// (transfer plus stress force), each channel i32(RN(value) S). FOLD (S = 2^k <= 2^16): RN(a) 2^k =
+// RN(a 2^k) away from overflow and subnormals, so i32(RN(RN(w p) + RN(M dt)) S) = i32(RN(RN(w (p S))
+// + RN(M (dt S)))) and i32(RN(w m) S) = i32(RN(w (m S))): pm and mass arrive multiplied by S and no
+// product by S follows. With |m|, |p|, |-V P F^T| <= 2^40 (per lane) and the host's bounds on dt S
+// and the cell size nothing overflows; a subnormal term is either below half an ulp of the other
+// term (which then is the sum in both forms) or leaves a sum that truncates to 0 in both forms.
+// (fmaxf passes NaN over; a NaN reaches the same channels in both forms, i32 0.)
+template <bool FOLD>
+MPM_DEV void explicit_deposit(const SimParams& P, Tile& T, const Stencil& s, const v3 d[3], const m3& negVPT, v3 pm, float mass) {
+    const float dtf = FOLD ? P.dtS : P.dt;
+#pragma unroll
+    for (int oz = -1; oz <= 1; ++oz)
+#pragma unroll
+    for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll
+    for (int ox = -1; ox <= 1; ++ox) {
+        // every node is deposited to; accumulators of unallocated nodes are never flushed
+        const int t = tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz);
+        const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
+        v3 g = mk((d[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z,
+                  (s.w[ox + 1].x * d[oy + 1].y) * s.w[oz + 1].z,
+                  (s.w[ox + 1].x * s.w[oy + 1].y) * d[oz + 1].z);
+        g = div_cd(P, g);
+        const v3 sm = mvmul(negVPT, g) * dtf;
+        const v3 mom = cw * pm + sm;
+        if (FOLD) {
+            atomicAdd(&T.acc[0][t], f2i(cw * mass));
+            atomicAdd(&T.acc[1][t], f2i(mom.x));
+            atomicAdd(&T.acc[2][t], f2i(mom.y));
+            atomicAdd(&T.acc[3][t], f2i(mom.z));
+        } else {
+            atomicAdd(&T.acc[0][t], f2i((cw * mass) * P.S));
+            atomicAdd(&T.acc[1][t], f2i(mom.x * P.S));
+            atomicAdd(&T.acc[2][t], f2i(mom.y * P.S));
+            atomicAdd(&T.acc[3][t], f2i(mom.z * P.S));
+        }
+    }
+}
 MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, int4* __restrict__ go) {
     const uint32_t sd = B.stride;
     {
@@ -1388,26 +1430,10 @@ MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             const v3 pm = mass * vel;
             Stencil s; stencil(P, T, pos, s);
             v3 d[3]; bspline_d(s.f, d);
-#pragma unroll
-            for (int oz = -1; oz <= 1; ++oz)
-#pragma unroll
-            for (int oy = -1; oy <= 1; ++oy)
-#pragma unroll
-            for (int ox = -1; ox <= 1; ++ox) {
-                // every node is deposited to; accumulators of unallocated nodes are never flushed
-                const int t = tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz);
-                const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
-                v3 g = mk((d[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z,
-                          (s.w[ox + 1].x * d[oy + 1].y) * s.w[oz + 1].z,
-                          (s.w[ox + 1].x * s.w[oy + 1].y) * d[oz + 1].z);
-                g = div_cd(P, g);
-                const v3 sm = mvmul(negVPT, g) * P.dt;
-                const v3 mom = cw * pm + sm;
-                atomicAdd(&T.acc[0][t], f2i((cw * mass) * P.S));
-                atomicAdd(&T.acc[1][t], f2i(mom.x * P.S));
-                atomicAdd(&T.acc[2][t], f2i(mom.y * P.S));
-                atomicAdd(&T.acc[3][t], f2i(mom.z * P.S));
-            }
+            const float big = fmaxf(fmaxf(fabsf(mass), mabs_max(pm)),
+                                    fmaxf(fmaxf(mabs_max(negVPT.c[0]), mabs_max(negVPT.c[1])), mabs_max(negVPT.c[2])));
+            if (P.ex_fold && big <= 0x1p40f) explicit_deposit<true>(P, T, s, d, negVPT, pm * P.S, mass * P.S);
+            else explicit_deposit<false>(P, T, s, d, negVPT, pm, mass);
         }
         __syncthreads();
         tile_flush(T, go);
