@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256, OCC) void k_ens_transfer(Bufs B, Ens E, int ph
                 pbmpm_chunk<PB_LAST>(P, B, T, ch, grid_sel(B, (k - 1) % 3), nullptr, k - 1);
             }
         } else {
-            if (r == 0) explicit_p2g_chunk(P, B, T, ch, B.grid0);
+            if (r == 0) explicit_p2g_chunk<false>(P, B, T, ch, B.grid0);
             else explicit_g2p_chunk<true>(P, B, T, ch, B.grid0);
         }
         __syncthreads();

@@ -316,13 +316,20 @@ int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, f
         P.dtS = P.dt * P.S;
         P.ex_fold = (pow2 && P.dtS <= 0x1p20f && P.cd[0] >= 0x1p-20f && P.cd[1] >= 0x1p-20f && P.cd[2] >= 0x1p-20f) ? 1u : 0u;
         uint32_t cd_ok = 1;
+        for (int a = 0; a < 3; ++a) P.rcd[a] = 1.0f / P.cd[a];
         for (int a = 0; a < 3 && cd_ok; ++a) {
-            P.rcd[a] = 1.0f / P.cd[a];
             uint32_t ok = 0;
             if ((vr = verify_divisor(c, P.cd[a], 1.0f, 2, &ok))) return vr;
             cd_ok = ok;
         }
         P.fast_cd = cd_ok;
+        // a power-of-two cell of at most 1 divides exactly as a product by 2^k (|x| <= 2^100 and
+        // k <= 126 cannot overflow, and scaling up leaves no rounding)
+        P.pow2_cd = 1u;
+        for (int a = 0; a < 3; ++a) {
+            int ea = 0;
+            if (!(std::frexp(P.cd[a], &ea) == 0.5f && P.cd[a] <= 1.0f && P.cd[a] >= 0x1p-26f)) P.pow2_cd = 0u;
+        }
     }
     P.wall_mode = g.wall_mode;
     P.wall_cells = g.wall_cells;

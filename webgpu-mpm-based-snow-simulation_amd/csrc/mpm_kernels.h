@@ -67,6 +67,8 @@ struct SimParams {
     uint32_t block_identity;   // direct addressing with dense block index = key (batched scenes)
     float rcd[3];              // RN(1 / cell dims)
     uint32_t fast_cd;          // qdiv(., cd[a], rcd[a]) verified equal to x / cd[a] for all |x| <= 2^100
+    uint32_t pow2_cd;          // every cell dim is 2^-k (k >= 0): x / cd[a] = x rcd[a] exactly for |x| <= 2^100
+                               // (used by the explicit deposit's weight gradients)
     uint32_t p2g_fold;         // PBMPM P2G may fold S into the mass (S = 2^k <= 2^16, quot_i32 verified
                                // for dt, no pointer force, gravity along y only with |grav_y / force_div|
                                // <= 2^40, 2^-20 <= dt <= 2^20)
@@ -1363,7 +1365,10 @@ __global__ __launch_bounds__(256, MODE == PB_MID ? 5 : 4) void k_pbmpm(SimParams
 // and the cell size nothing overflows; a subnormal term is either below half an ulp of the other
 // term (which then is the sum in both forms) or leaves a sum that truncates to 0 in both forms.
 // (fmaxf passes NaN over; a NaN reaches the same channels in both forms, i32 0.)
-template <bool FOLD>
+// POW2: cells of 2^-k (P.pow2_cd) take the quotient of the weight gradient as the product by 2^k
+// (exact for these |g| <= 2^100); the batched-scene kernel is built without it (the extra path
+// costs that kernel's register allocation more than it saves).
+template <bool FOLD, bool POW2>
 MPM_DEV void explicit_deposit(const SimParams& P, Tile& T, const Stencil& s, const v3 d[3], const m3& negVPT, v3 pm, float mass) {
     const float dtf = FOLD ? P.dtS : P.dt;
 #pragma unroll
@@ -1378,7 +1383,7 @@ MPM_DEV void explicit_deposit(const SimParams& P, Tile& T, const Stencil& s, con
         v3 g = mk((d[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z,
                   (s.w[ox + 1].x * d[oy + 1].y) * s.w[oz + 1].z,
                   (s.w[ox + 1].x * s.w[oy + 1].y) * d[oz + 1].z);
-        g = div_cd(P, g);
+        g = (POW2 && P.pow2_cd) ? mk(g.x * P.rcd[0], g.y * P.rcd[1], g.z * P.rcd[2]) : div_cd(P, g);
         const v3 sm = mvmul(negVPT, g) * dtf;
         const v3 mom = cw * pm + sm;
         if (FOLD) {
@@ -1394,6 +1399,7 @@ MPM_DEV void explicit_deposit(const SimParams& P, Tile& T, const Stencil& s, con
         }
     }
 }
+template <bool POW2 = true>
 MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, int4* __restrict__ go) {
     const uint32_t sd = B.stride;
     {
@@ -1432,8 +1438,8 @@ MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             v3 d[3]; bspline_d(s.f, d);
             const float big = fmaxf(fmaxf(fabsf(mass), mabs_max(pm)),
                                     fmaxf(fmaxf(mabs_max(negVPT.c[0]), mabs_max(negVPT.c[1])), mabs_max(negVPT.c[2])));
-            if (P.ex_fold && big <= 0x1p40f) explicit_deposit<true>(P, T, s, d, negVPT, pm * P.S, mass * P.S);
-            else explicit_deposit<false>(P, T, s, d, negVPT, pm, mass);
+            if (P.ex_fold && big <= 0x1p40f) explicit_deposit<true, POW2>(P, T, s, d, negVPT, pm * P.S, mass * P.S);
+            else explicit_deposit<false, POW2>(P, T, s, d, negVPT, pm, mass);
         }
         __syncthreads();
         tile_flush(T, go);
