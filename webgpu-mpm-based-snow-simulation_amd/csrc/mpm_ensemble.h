@@ -352,16 +352,16 @@ __global__ __launch_bounds__(256, OCC) void k_ens_transfer(Bufs B, Ens E, int ph
             const int k = P.k;
             if (r == 0) {
                 if (k > 1) ens_clear_slice(P, B, E, sc, B.grid1, j, ntask);
-                pbmpm_chunk<PB_FIRST>(P, B, T, ch, nullptr, B.grid0, -1);
+                pbmpm_chunk<PB_FIRST, false, false>(P, B, T, ch, nullptr, B.grid0, -1);
             } else if ((int)r < k) {
                 ens_clear_slice(P, B, E, sc, grid_sel(B, (int)(r + 1) % 3), j, ntask);
-                pbmpm_chunk<PB_MID>(P, B, T, ch, grid_sel(B, (int)(r - 1) % 3), grid_sel(B, (int)r % 3), (int)r - 1);
+                pbmpm_chunk<PB_MID, false, false>(P, B, T, ch, grid_sel(B, (int)(r - 1) % 3), grid_sel(B, (int)r % 3), (int)r - 1);
             } else {
-                pbmpm_chunk<PB_LAST>(P, B, T, ch, grid_sel(B, (k - 1) % 3), nullptr, k - 1);
+                pbmpm_chunk<PB_LAST, false, false>(P, B, T, ch, grid_sel(B, (k - 1) % 3), nullptr, k - 1);
             }
         } else {
             if (r == 0) explicit_p2g_chunk<false>(P, B, T, ch, B.grid0);
-            else explicit_g2p_chunk<true>(P, B, T, ch, B.grid0);
+            else explicit_g2p_chunk<true, false>(P, B, T, ch, B.grid0);
         }
         __syncthreads();
         if (threadIdx.x == 0 && phase < 0) {

@@ -155,8 +155,12 @@ MPM_DEV bool in_range(const SimParams& P, int x, int y, int z) {
     return x >= 0 && y >= 0 && z >= 0 && x < P.res[0] && y < P.res[1] && z < P.res[2];
 }
 // a / cell dims for |a| <= 2^100 (offsets inside the grid, weight gradients): the verified FMA
-// quotient where it is exact, else the IEEE division.
+// quotient where it is exact, else the IEEE division. POW2: cells of 2^-k (P.pow2_cd) give the
+// quotient as the product by 2^k, exact for these |a|. (The batched-scene kernel is built without
+// that path: there it cost more in register allocation than it saved.)
+template <bool POW2 = false>
 MPM_DEV v3 div_cd(const SimParams& P, v3 a) {
+    if (POW2 && P.pow2_cd) return mk(a.x * P.rcd[0], a.y * P.rcd[1], a.z * P.rcd[2]);
     if (P.fast_cd) return mk(qdiv(a.x, P.cd[0], P.rcd[0]), qdiv(a.y, P.cd[1], P.rcd[1]), qdiv(a.z, P.cd[2], P.rcd[2]));
     return cdiv(a, mk(P.cd[0], P.cd[1], P.cd[2]));
 }
@@ -166,14 +170,16 @@ MPM_DEV void cell_number(const SimParams& P, v3 pos, int c[3]) {
     c[0] = f2i(q.x); c[1] = f2i(q.y); c[2] = f2i(q.z);
 }
 // cell_number for a binned particle (position inside the grid's block range).
+template <bool POW2 = false>
 MPM_DEV void cell_number_in(const SimParams& P, v3 pos, int c[3]) {
-    v3 q = div_cd(P, pos - mk(P.gmin[0], P.gmin[1], P.gmin[2]));
+    v3 q = div_cd<POW2>(P, pos - mk(P.gmin[0], P.gmin[1], P.gmin[2]));
     c[0] = f2i(q.x); c[1] = f2i(q.y); c[2] = f2i(q.z);
 }
+template <bool POW2 = false>
 MPM_DEV v3 frac_pos(const SimParams& P, v3 pos, const int c[3]) {
     v3 cdv = mk(P.cd[0], P.cd[1], P.cd[2]);
     v3 minPos = mk(P.gmin[0], P.gmin[1], P.gmin[2]) + cmul(cdv, mk((float)c[0], (float)c[1], (float)c[2]));
-    return div_cd(P, pos - minPos);
+    return div_cd<POW2>(P, pos - minPos);
 }
 MPM_DEV void bspline(v3 f, v3 w[3]) {
     v3 a = mk(1.0f, 1.0f, 1.0f) - f;
@@ -1081,9 +1087,10 @@ struct Stencil {
     v3 f;           // fractional position
     v3 w[3];
 };
+template <bool POW2 = false>
 MPM_DEV void stencil(const SimParams& P, const Tile& T, v3 pos, Stencil& s) {
-    cell_number_in(P, pos, s.c0);
-    s.f = frac_pos(P, pos, s.c0);
+    cell_number_in<POW2>(P, pos, s.c0);
+    s.f = frac_pos<POW2>(P, pos, s.c0);
     bspline(s.f, s.w);
     s.t0[0] = s.c0[0] - T.base[0]; s.t0[1] = s.c0[1] - T.base[1]; s.t0[2] = s.c0[2] - T.base[2];
 }
@@ -1250,7 +1257,7 @@ __global__ void k_resid_roll(Bufs B, int k, uint32_t* peak) {
 
 // One work chunk of a PBMPM transfer pass (the body of k_pbmpm; the batched-scene kernel calls
 // it with the chunk's scene parameters).
-template <int MODE, bool TRACK = false>
+template <int MODE, bool TRACK = false, bool POW2 = true>
 MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, const int4* __restrict__ gi,
                          int4* __restrict__ go, int it) {
     const uint32_t sd = B.stride;
@@ -1283,7 +1290,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 B.relax[q] = relax;
                 if (ok) {
                     solve_constraints(P, Fe, relax, D);
-                    Stencil s; stencil(P, T, pos, s);
+                    Stencil s; stencil<POW2>(P, T, pos, s);
                     float off[3][3]; stencil_offsets(s, off);
                     p2g_pbmpm<true>(P, T, s, off, mass, dx, D);
                 } else {   // run whose own block is not allocated: carried unchanged to LAST
@@ -1295,7 +1302,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 const float mass = ld(B.nxt, sd, PM, q);
                 const m3 Fe = ldm(B.nxt, sd, FE, q);
                 const float relax = B.relax[q];
-                Stencil s; stencil(P, T, pos, s);
+                Stencil s; stencil<POW2>(P, T, pos, s);
                 float off[3][3]; stencil_offsets(s, off);
                 v3 nv; m3 Bm;
                 g2p_pbmpm(T, s, off, nv, Bm);
@@ -1304,7 +1311,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 m3 D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                 solve_constraints(P, Fe, relax, D);
                 // the stencil is recomputed (same values) rather than kept live across the solve
-                Stencil s2; stencil(P, T, pos, s2);
+                Stencil s2; stencil<POW2>(P, T, pos, s2);
                 float off2[3][3]; stencil_offsets(s2, off2);
                 p2g_pbmpm<false>(P, T, s2, off2, mass, dx, D);
                 }
@@ -1314,7 +1321,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 p.mass = ld(B.nxt, sd, PM, q);
                 p.Fe = ldm(B.nxt, sd, FE, q);
                 if (ok) {
-                    Stencil s; stencil(P, T, p.pos, s);
+                    Stencil s; stencil<POW2>(P, T, p.pos, s);
                     float off[3][3]; stencil_offsets(s, off);
                     v3 nv; m3 Bm;
                     g2p_pbmpm(T, s, off, nv, Bm);
@@ -1365,9 +1372,6 @@ __global__ __launch_bounds__(256, MODE == PB_MID ? 5 : 4) void k_pbmpm(SimParams
 // and the cell size nothing overflows; a subnormal term is either below half an ulp of the other
 // term (which then is the sum in both forms) or leaves a sum that truncates to 0 in both forms.
 // (fmaxf passes NaN over; a NaN reaches the same channels in both forms, i32 0.)
-// POW2: cells of 2^-k (P.pow2_cd) take the quotient of the weight gradient as the product by 2^k
-// (exact for these |g| <= 2^100); the batched-scene kernel is built without it (the extra path
-// costs that kernel's register allocation more than it saves).
 template <bool FOLD, bool POW2>
 MPM_DEV void explicit_deposit(const SimParams& P, Tile& T, const Stencil& s, const v3 d[3], const m3& negVPT, v3 pm, float mass) {
     const float dtf = FOLD ? P.dtS : P.dt;
@@ -1383,7 +1387,7 @@ MPM_DEV void explicit_deposit(const SimParams& P, Tile& T, const Stencil& s, con
         v3 g = mk((d[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z,
                   (s.w[ox + 1].x * d[oy + 1].y) * s.w[oz + 1].z,
                   (s.w[ox + 1].x * s.w[oy + 1].y) * d[oz + 1].z);
-        g = (POW2 && P.pow2_cd) ? mk(g.x * P.rcd[0], g.y * P.rcd[1], g.z * P.rcd[2]) : div_cd(P, g);
+        g = div_cd<POW2>(P, g);
         const v3 sm = mvmul(negVPT, g) * dtf;
         const v3 mom = cw * pm + sm;
         if (FOLD) {
@@ -1434,7 +1438,7 @@ MPM_DEV void explicit_p2g_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             const m3 Pk = madd(mscale(2.0f * mu, msub(Fe, R)), mscale((la * (J - 1.0f)) * J, transpose(inverse3(Fe))));
             const m3 negVPT = mscale(-(mass * P.inv_density), transpose(Pk));
             const v3 pm = mass * vel;
-            Stencil s; stencil(P, T, pos, s);
+            Stencil s; stencil<POW2>(P, T, pos, s);
             v3 d[3]; bspline_d(s.f, d);
             const float big = fmaxf(fmaxf(fabsf(mass), mabs_max(pm)),
                                     fmaxf(fmaxf(mabs_max(negVPT.c[0]), mabs_max(negVPT.c[1])), mabs_max(negVPT.c[2])));
@@ -1474,7 +1478,7 @@ __global__ __launch_bounds__(256) void k_grid_update(SimParams P, Bufs B) {
 // gridToParticle.cs.wgsl:28-80 + integrateParticles.wgsl:12-72
 // One work chunk of the explicit G2P + integrate. FUSED: the tile applies the grid update
 // (gridUpdate.cs.wgsl:35-53) to its nodes itself, reading the P2G sums instead of the updated grid.
-template <bool FUSED>
+template <bool FUSED, bool POW2 = true>
 MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, const int4* __restrict__ g) {
     const uint32_t sd = B.stride;
     {
@@ -1494,7 +1498,7 @@ MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             p.mass = ld(B.nxt, sd, PM, q);
             p.Fe = ldm(B.nxt, sd, FE, q);
             if (ok) {
-                Stencil s; stencil(P, T, p.pos, s);
+                Stencil s; stencil<POW2>(P, T, p.pos, s);
                 v3 d[3]; bspline_d(s.f, d);
                 v3 nv = mk(0, 0, 0);
                 m3 G = zero3();
