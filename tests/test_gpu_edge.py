@@ -93,6 +93,35 @@ def test_plasticity_clamps_bitexact(mpm, orc, collider):
         sim.close()
 
 
+def test_plastic_band_edges_bitexact(mpm, orc):
+    """F_e with singular values on both sides of the band's edges and of the inner edges where the
+    passes skip the polar-decomposition check (lo + 5e-4, hi - 5e-4), arbitrary rotations and
+    reflections, some axis-aligned: both methods, bit for bit against the oracle, which always
+    runs the full check."""
+    rng = np.random.default_rng(33)
+    n = 8000
+    lo, hi = 1.0 - 2.5e-2, 1.0 + 7.5e-3
+    centres = np.array([lo, lo + 5e-4, hi - 5e-4, hi, 1.0])
+    sv = centres[rng.integers(0, centres.size, (n, 3))] + rng.uniform(-3e-4, 3e-4, (n, 3))
+    q1, _ = np.linalg.qr(rng.normal(size=(n, 3, 3)))
+    q2, _ = np.linalg.qr(rng.normal(size=(n, 3, 3)))
+    q2[1::3] = np.eye(3)
+    fe = np.einsum("nij,nj,nkj->nik", q1, sv, q2).astype(np.float32)
+    pos = rng.uniform(0.3, 0.7, (n, 3)).astype(np.float32)
+    for method in (1, 0):
+        p = mpm.make_particles(pos, vel=rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32))
+        p["Fe"][:, :, :3] = np.swapaxes(fe, 1, 2)
+        sc = dict(particles=p, sim=dict(grid_res=(32, 32, 32), grid_min=(0, 0, 0), grid_max=(1, 1, 1),
+                                        dt=1.0 / 600 if method == 1 else 1e-4, gravity=(0.0, -9.8, 0.0),
+                                        pbmpm_iterations=3, pbmpm_force_divisor=3.0), collider=None)
+        sim, o = make_pair(mpm, orc, sc)
+        for s in range(2):
+            sim.step(method, 1)
+            o.step(method, 1)
+            assert_particles_equal(mpm, sim.download(), o.particles(), f"method {method} substep {s}")
+        sim.close()
+
+
 def test_particles_on_and_outside_the_walls(mpm, orc):
     """Particles at exactly gridMax (cell == res, finding 13), outside the grid and a
     non-multiple-of-4 resolution: binned / unbinned sets, grids and states match the oracle."""

@@ -289,6 +289,15 @@ int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, f
     P.vol_exp = g.pbmpm_volume_exponent;
     P.plast_lo = (float)(1.0 - g.theta_c);
     P.plast_hi = (float)(1.0 + g.theta_s);
+    {
+        // squared inner band of band_clear, rounded toward its inside
+        const double blo = (double)P.plast_lo + BAND_MARGIN, bhi = (double)P.plast_hi - BAND_MARGIN;
+        float lo2 = (float)(blo * blo), hi2 = (float)(bhi * bhi);
+        if ((double)lo2 < blo * blo) lo2 = std::nextafter(lo2, 2.0f);
+        if ((double)hi2 > bhi * bhi) hi2 = std::nextafter(hi2, 0.0f);
+        P.band_lo2 = lo2; P.band_hi2 = hi2;
+        P.band_skip = (blo >= 0.8 && bhi <= 1.25 && blo < bhi) ? 1u : 0u;
+    }
     P.inv_density = (float)(1.0 / g.density);
     P.grad_scale = g.grad_scale;
     P.force_div = force_div;
