@@ -1,27 +1,33 @@
 // Hand-written gfx950 kernels of the MPM substep.
 //
 // Data layout in HBM (one GPU, everything resident):
-//   particle state  37 FP32 planes (SoA, plane-major, stride = padded N) kept in *sorted* order:
-//                   pos xyz, mass, vel xyz, dx xyz, F_e[9], F_p[9], D[9] (matrices column-major),
-//                   plus a u32 plane of original particle ids. Two copies (cur / nxt): each
-//                   substep's first transfer pass gathers cur through the sort permutation and
-//                   writes nxt, so every later pass streams contiguous, coalesced planes.
+//   particle state  31 FP32 planes (SoA, plane-major, stride = padded N) kept in *sorted* order:
+//                   pos xyz, mass, vel xyz, dx xyz, F_e[9], det F_p, D[9] (matrices column-major),
+//                   optional per-particle Lame planes, plus a u32 plane of original particle ids.
+//                   Two copies (cur / nxt): each substep's first transfer pass gathers cur through
+//                   the sort permutation and writes nxt, so every later pass streams contiguous,
+//                   coalesced planes. F_p itself is kept per original id (fpid) and touched only
+//                   when plasticity clamps.
 //   binning         key space = all 4^3 blocks of the grid (linearised block coordinate); per-key
 //                   counts / OR of touched-neighbour masks, one scan over the key space yields
-//                   run (particle-block) descriptors, work chunks and scatter cursors.
-//   block table     open-addressing hash (hash3 of the block coordinate, linear probing,
-//                   atomicCAS insert) compacted to dense block indices.
+//                   run (particle-block) descriptors, work chunks, scatter cursors and (direct
+//                   addressing, the default) dense block indices of the touched blocks.
+//   block table     with an explicit hash_map_size: open-addressing hash (hash3 of the block
+//                   coordinate, linear probing, atomicCAS insert) compacted to dense block indices.
 //   grid            int4 per cell {mass, px, py, pz} i32 fixed point, 64 cells per block,
 //                   three buffers so PBMPM iteration i reads grid i-1, writes grid i, and clears
 //                   the buffer iteration i+1 will write.
 // Reference passes (src/lib/gpu/mpm/*.wgsl) and the kernels that replace them:
-//   clearHashMap, mapAffectedBlocks                 -> k_keys_count (table clear), k_insert, k_compact
+//   clearHashMap, mapAffectedBlocks                 -> k_keys_count (touched flags / table clear),
+//                                                      k_scan_apply (dense slots); k_insert, k_compact
 //   clearBlockParticleCounts, countParticlesPerBlock,
 //   computeBlockOffsets (1 thread), binParticles    -> k_keys_count, k_scan_*, k_scatter
-//   clearMappedBlocks (whole capacity)              -> k_nbr (active blocks only), fused clears
+//   clearMappedBlocks (whole capacity)              -> neighbour pass (in k_scatter or k_nbr; active
+//                                                      blocks only), fused clears
 //   pbmpm (solve+P2G+gravity) and doGridToParticle  -> k_pbmpm<FIRST|MID|LAST> (LDS-tiled)
 //   doParticleToGrid / doGridUpdate / doGridToParticle (explicit) -> k_explicit_p2g / k_grid_update / k_explicit_g2p
-//   integrateParticles                              -> fused into k_pbmpm<LAST>, k_explicit_g2p, k_straggle
+//   integrateParticles                              -> fused into k_pbmpm<LAST>, k_explicit_g2p;
+//                                                      unbinned particles in k_collide / k_straggle
 #pragma once
 #include "mpm_math.h"
 
