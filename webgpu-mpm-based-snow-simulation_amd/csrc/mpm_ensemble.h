@@ -111,18 +111,20 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
             bool peer = valid && key == lk;
             uint64_t peers = __ballot(peer);
             uint32_t m = wave_or(peer ? mask : 0u);
-            uint32_t b = 0;
+            uint32_t b = 0, fresh = 0;
             if (lane_id() == leader) {
                 b = atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
-                atomicOr(&B.kmask[lk], m);
+                fresh = m & ~atomicOr(&B.kmask[lk], m);
             }
             b = __shfl(b, leader, 64);
             if (peer) B.lofs[i] = b + (uint32_t)__popcll(peers & lanemask_lt());
-            if (P0.direct_blocks) {
-                // lk and m are the same in every lane: scene and parameters read once per wave
+            fresh = __shfl(fresh, leader, 64);
+            if (P0.direct_blocks && fresh) {
+                // lk and the new bits are the same in every lane: scene and parameters read once
+                // per wave; each (key, neighbour) pair is flagged once
                 const int j = lane_id();
                 const uint32_t uk = __builtin_amdgcn_readfirstlane(lk);
-                const uint32_t um = __builtin_amdgcn_readfirstlane(m);
+                const uint32_t um = __builtin_amdgcn_readfirstlane(fresh);
                 const SimParams& P = E.sp[__builtin_amdgcn_readfirstlane(seg_find(kb, E.S, uk))];
                 if (j < 27 && ((um >> j) & 1u)) {
                     int b[3]; block_coords(P, uk, b);

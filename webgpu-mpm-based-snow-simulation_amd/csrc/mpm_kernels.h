@@ -356,7 +356,8 @@ MPM_DEV uint32_t block_key_at(const SimParams& P, int bx, int by, int bz, int j)
 // mapAffectedBlocks.wgsl:64-86). Atomics are aggregated per wave over lanes that share a key (input
 // is in the previous substep's sorted order); the count atomic's old value gives the offsets.
 // Hash-table mode also resets the table (clearHashMap.wgsl:5-19); direct mode instead flags every
-// touched block in kalloc (plain stores of 1, one lane per neighbour). Resets the per-substep counters.
+// touched block in kalloc (plain stores of 1, one lane per neighbour bit that the wave's OR added
+// to its key's mask, so each pair is stored once). Resets the per-substep counters.
 __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
     const uint32_t gsz = gridDim.x * blockDim.x;
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -395,14 +396,16 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
             bool peer = valid && key == lk;
             uint64_t peers = __ballot(peer);
             uint32_t m = wave_or(peer ? mask : 0u);
-            uint32_t b = 0;
+            uint32_t b = 0, fresh = 0;
             if (lane_id() == leader) {
                 b = atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
-                atomicOr(&B.kmask[lk], m);
+                fresh = m & ~atomicOr(&B.kmask[lk], m);
             }
             b = __shfl(b, leader, 64);
             if (peer) B.lofs[i] = b + (uint32_t)__popcll(peers & lanemask_lt());
-            if (P.direct_blocks && jl < 27 && ((m >> jl) & 1u)) B.kalloc[lk + koff] = 1u;
+            // each (key, neighbour) pair is flagged once: by the wave whose OR first set its bit
+            fresh = __shfl(fresh, leader, 64);
+            if (P.direct_blocks && jl < 27 && ((fresh >> jl) & 1u)) B.kalloc[lk + koff] = 1u;
             active &= ~peers;
         }
     }
