@@ -323,6 +323,8 @@ int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, f
             if ((vr = verify_divisor(c, P.dt, P.S, 1, &q_ok))) return vr;
         P.p2g_fold = q_ok;
         P.dtS = P.dt * P.S;
+        P.pow2_S = pow2 ? 1u : 0u;
+        P.rS = 1.0f / P.S;
         P.ex_fold = (pow2 && P.dtS <= 0x1p20f && P.cd[0] >= 0x1p-20f && P.cd[1] >= 0x1p-20f && P.cd[2] >= 0x1p-20f) ? 1u : 0u;
         uint32_t cd_ok = 1;
         for (int a = 0; a < 3; ++a) P.rcd[a] = 1.0f / P.cd[a];

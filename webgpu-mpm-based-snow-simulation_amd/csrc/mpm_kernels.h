@@ -73,6 +73,8 @@ struct SimParams {
     uint32_t block_identity;   // direct addressing with dense block index = key (batched scenes)
     float rcd[3];              // RN(1 / cell dims)
     uint32_t fast_cd;          // qdiv(., cd[a], rcd[a]) verified equal to x / cd[a] for all |x| <= 2^100
+    uint32_t pow2_S;           // S = 2^k, 0 <= k <= 16: q / S = q rS exactly for every integer-valued q
+    float rS;                  // 1 / S
     uint32_t pow2_cd;          // every cell dim is 2^-k (k >= 0): x / cd[a] = x rcd[a] exactly for |x| <= 2^100
                                // (used by the explicit deposit's weight gradients)
     uint32_t p2g_fold;         // PBMPM P2G may fold S into the mass (S = 2^k <= 2^16, quot_i32 verified
@@ -1027,8 +1029,15 @@ MPM_DEV void tile_setup(const SimParams& P, const Bufs& B, Tile& T, uint2 ch) {
         T.vel[t] = make_float4(0.f, 0.f, 0.f, idx >= 0 ? 1.0f : 0.0f);
     }
 }
+// A fixed-point node sum as a float, (float)q / S. POW2 with S = 2^k (k <= 16): the product by 2^-k,
+// exact since the quotient of a nonzero integer-valued float is at least 2^-16, a normal number.
+template <bool POW2>
+MPM_DEV float unscale(const SimParams& P, int32_t q) {
+    return (POW2 && P.pow2_S) ? (float)q * P.rS : (float)q / P.S;
+}
 // node velocity as gridToParticle.cs.wgsl:43-49 / :102-108 computes it; the node mass comes from
 // the separate PBMPM mass array when gm is given, else from the grid's mass channel
+template <bool POW2>
 MPM_DEV void tile_load_vel(const SimParams& P, Tile& T, const int4* __restrict__ g, const int* __restrict__ gm = nullptr) {
     for (int t = threadIdx.x; t < 216; t += blockDim.x) {
         const int idx = T.node[t];
@@ -1036,11 +1045,11 @@ MPM_DEV void tile_load_vel(const SimParams& P, Tile& T, const int4* __restrict__
         if (idx >= 0) {
             int4 q = g[idx];
             if (gm) q.x = gm[idx];
-            const float m = (float)q.x / P.S;
+            const float m = unscale<POW2>(P, q.x);
             if (m > 0.0f) {
-                v.x = ((float)q.y / P.S) / m;
-                v.y = ((float)q.z / P.S) / m;
-                v.z = ((float)q.w / P.S) / m;
+                v.x = unscale<POW2>(P, q.y) / m;
+                v.y = unscale<POW2>(P, q.z) / m;
+                v.z = unscale<POW2>(P, q.w) / m;
                 v.w = 2.0f;
                 if (P.wall_mode & 1u) {
                     const int tx = t % 6, ty = (t / 6) % 6, tz = t / 36;
@@ -1291,7 +1300,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
     const uint32_t sd = B.stride;
     {
         tile_setup(P, B, T, chunk);
-        if (MODE != PB_FIRST) tile_load_vel(P, T, gi, B.gmass);
+        if (MODE != PB_FIRST) tile_load_vel<POW2>(P, T, gi, B.gmass);
         if (MODE != PB_LAST) tile_zero_acc(T);
         __syncthreads();
         const bool ok = T.center_ok != 0;
@@ -1489,9 +1498,9 @@ __global__ __launch_bounds__(256) void k_grid_update(SimParams P, Bufs B) {
     const uint32_t ncell = n_blocks_of(P, B) * 64u;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < ncell; c += gridDim.x * blockDim.x) {
         int4 q = B.grid0[c];
-        const float m = (float)q.x / P.S;
+        const float m = unscale<true>(P, q.x);
         if (m <= 0.0f) continue;
-        v3 v = mk((float)q.y / P.S, (float)q.z / P.S, (float)q.w / P.S) / m;
+        v3 v = mk(unscale<true>(P, q.y), unscale<true>(P, q.z), unscale<true>(P, q.w)) / m;
         v = v + mk(P.grav[0], P.grav[1], P.grav[2]) * P.dt;
         if (P.wall_mode & 1u) {
             int b[3]; block_coords(P, B.block_key[c >> 6], b);
@@ -1512,7 +1521,7 @@ MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
     {
         tile_setup(P, B, T, chunk);
         if (FUSED) tile_load_vel_updated(P, T, g);
-        else tile_load_vel(P, T, g);
+        else tile_load_vel<POW2>(P, T, g);
         __syncthreads();
         const bool ok = T.center_ok != 0;
         const uint32_t rounds = (T.count + blockDim.x - 1) / blockDim.x;
