@@ -1001,12 +1001,19 @@ struct Tile {
     uint32_t nbr[27];
     int base[3];
     uint32_t start, count, center_ok;
+    // MID round order (cell-interleaved): particles per cell, and per rank level the cells holding a
+    // particle of that rank, the particles before that level, the resulting lane -> row map
+    uint32_t ocnt[64];
+    unsigned long long oball[256];
+    uint32_t obase[256];
+    uint32_t operm[256];
 };
 
 MPM_DEV int tile_of(int tx, int ty, int tz) { return tx + 6 * (ty + 6 * tz); }
 
 MPM_DEV void tile_setup(const SimParams& P, const Bufs& B, Tile& T, uint2 ch) {
     if (threadIdx.x < 27) T.nbr[threadIdx.x] = B.nbr[ch.x * 27u + threadIdx.x];
+    if (threadIdx.x < 64) T.ocnt[threadIdx.x] = 0u;
     if (threadIdx.x == 0) {
         int b[3]; block_coords(P, B.run_key[ch.x], b);
         T.base[0] = 4 * b[0] - 1; T.base[1] = 4 * b[1] - 1; T.base[2] = 4 * b[2] - 1;
@@ -1292,8 +1299,43 @@ __global__ void k_resid_roll(Bufs B, int k, uint32_t* peak) {
     B.resid_max[it] = 0u;
 }
 
+// Order of the particles of round r of a MID pass (blockDim.x rows): rank-major by cell, so wave w
+// takes the w-th particle of each cell and the lanes of a wave deposit to distinct cells (distinct
+// LDS accumulator addresses; same-address LDS atomics serialise). Returns the row, within the
+// round, of this thread's particle. (FIRST, which gathers its rows through the sort permutation,
+// measured slower with it: the extra gathered read outweighs the deposits it speeds up.)
+MPM_DEV uint32_t round_order(const SimParams& P, const Bufs& B, Tile& T, uint32_t r) {
+    const uint32_t r0 = r * blockDim.x;
+    const uint32_t nr = T.count - r0 < blockDim.x ? T.count - r0 : blockDim.x;
+    const bool mine = threadIdx.x < nr;
+    uint32_t cell = 0, rank = 0;
+    if (mine) {
+        int c[3]; cell_number_in<true>(P, ld3(B.nxt, B.stride, PX, T.start + r0 + threadIdx.x), c);
+        cell = (uint32_t)((c[0] - T.base[0] - 1) + 4 * ((c[1] - T.base[1] - 1) + 4 * (c[2] - T.base[2] - 1)));
+        cell = cell < 64u ? cell : 63u;           // the chunk's particles lie in its block
+        rank = atomicAdd(&T.ocnt[cell], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t cnt = T.ocnt[threadIdx.x];
+        uint32_t base = 0;
+        for (uint32_t k = 0; k < 256u; ++k) {
+            const uint64_t ball = __ballot(cnt > k);
+            if (!ball) break;
+            if (threadIdx.x == 0) { T.oball[k] = ball; T.obase[k] = base; }
+            base += (uint32_t)__popcll(ball);
+        }
+        T.ocnt[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    if (mine) T.operm[T.obase[rank] + (uint32_t)__popcll(T.oball[rank] & ((1ull << cell) - 1ull))] = threadIdx.x;
+    __syncthreads();
+    return mine ? T.operm[threadIdx.x] : threadIdx.x;
+}
+
 // One work chunk of a PBMPM transfer pass (the body of k_pbmpm; the batched-scene kernel calls
-// it with the chunk's scene parameters).
+// it with the chunk's scene parameters and POW2 = false: no power-of-two cell products and no
+// cell-interleaved round order, which there cost more than they saved).
 template <int MODE, bool TRACK = false, bool POW2 = true>
 MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, const int4* __restrict__ gi,
                          int4* __restrict__ go, int it) {
@@ -1306,7 +1348,8 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
         const bool ok = T.center_ok != 0;
         const uint32_t rounds = (T.count + blockDim.x - 1) / blockDim.x;
         for (uint32_t r = 0; r < rounds; ++r) {
-            const uint32_t i = r * blockDim.x + threadIdx.x;
+            uint32_t i = r * blockDim.x + threadIdx.x;
+            if (MODE == PB_MID && POW2 && ok) i = r * blockDim.x + round_order(P, B, T, r);
             const uint32_t q = T.start + i;
             bool cand = false;
             float resid = 0.0f;
