@@ -991,13 +991,12 @@ struct Tile {
     uint32_t nbr[27];
     int base[3];
     uint32_t start, count, center_ok;
-    // MID round order (cell-interleaved): particles per cell, and per rank level the cells holding a
-    // particle of that rank, the particles before that level, the resulting lane -> row map
+    // FIRST's row order (cell-interleaved): particles per cell, and per rank level the cells holding
+    // a particle of that rank and the particles before that level
     uint32_t ocnt[64];
     uint32_t qcnt, qbase;  // LAST: collider-queue entries of the round, and their first slot
     unsigned long long oball[256];
     uint32_t obase[256];
-    uint32_t operm[256];
 };
 
 MPM_DEV int tile_of(int tx, int ty, int tz) { return tx + 6 * (ty + 6 * tz); }
@@ -1291,18 +1290,17 @@ __global__ void k_resid_roll(Bufs B, int k, uint32_t* peak) {
     B.resid_max[it] = 0u;
 }
 
-// Order of the particles of round r of a MID pass (blockDim.x rows): rank-major by cell, so wave w
-// takes the w-th particle of each cell and the lanes of a wave deposit to distinct cells (distinct
-// LDS accumulator addresses; same-address LDS atomics serialise). Returns the row, within the
-// round, of this thread's particle. (FIRST, which gathers its rows through the sort permutation,
-// measured slower with it: the extra gathered read outweighs the deposits it speeds up.)
-MPM_DEV uint32_t round_order(const SimParams& P, const Bufs& B, Tile& T, uint32_t r) {
-    const uint32_t r0 = r * blockDim.x;
-    const uint32_t nr = T.count - r0 < blockDim.x ? T.count - r0 : blockDim.x;
-    const bool mine = threadIdx.x < nr;
+// Slot, within a round of FIRST (blockDim.x rows), of this thread's particle when the round's rows
+// are written rank-major by cell: first one particle of every cell, then a second one of every cell
+// that has two, and so on. The later passes take consecutive rows per wave, so the lanes of a wave
+// lie in distinct cells and their P2G deposits hit distinct LDS accumulators (same-address LDS
+// atomics serialise). Positions do not change before LAST, so the order holds for every MID pass.
+// Called by every thread of the workgroup; `mine`: this thread has a particle in the round, at pos.
+template <bool POW2>
+MPM_DEV uint32_t round_slot(const SimParams& P, Tile& T, bool mine, v3 pos) {
     uint32_t cell = 0, rank = 0;
     if (mine) {
-        int c[3]; cell_number_in<true>(P, ld3(B.nxt, B.stride, PX, T.start + r0 + threadIdx.x), c);
+        int c[3]; cell_number_in<POW2>(P, pos, c);
         cell = (uint32_t)((c[0] - T.base[0] - 1) + 4 * ((c[1] - T.base[1] - 1) + 4 * (c[2] - T.base[2] - 1)));
         cell = cell < 64u ? cell : 63u;           // the chunk's particles lie in its block
         rank = atomicAdd(&T.ocnt[cell], 1u);
@@ -1320,9 +1318,8 @@ MPM_DEV uint32_t round_order(const SimParams& P, const Bufs& B, Tile& T, uint32_
         T.ocnt[threadIdx.x] = 0u;
     }
     __syncthreads();
-    if (mine) T.operm[T.obase[rank] + (uint32_t)__popcll(T.oball[rank] & ((1ull << cell) - 1ull))] = threadIdx.x;
-    __syncthreads();
-    return mine ? T.operm[threadIdx.x] : threadIdx.x;
+    // (the next round's counts start after every thread has passed that round's first barrier)
+    return mine ? T.obase[rank] + (uint32_t)__popcll(T.oball[rank] & ((1ull << cell) - 1ull)) : threadIdx.x;
 }
 
 // Appends sorted index q to the collider queue, one atomic per wave (the explicit G2P pass, whose
@@ -1370,34 +1367,41 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
         const bool ok = T.center_ok != 0;
         const uint32_t rounds = (T.count + blockDim.x - 1) / blockDim.x;
         for (uint32_t r = 0; r < rounds; ++r) {
-            uint32_t i = r * blockDim.x + threadIdx.x;
-            if (MODE == PB_MID && POW2 && ok) i = r * blockDim.x + round_order(P, B, T, r);
+            const uint32_t i = r * blockDim.x + threadIdx.x;
             const uint32_t q = T.start + i;
             bool cand = false;
             float resid = 0.0f;
-            if (i >= T.count) {
-            } else if (MODE == PB_FIRST) {
-                const uint32_t old = B.perm[q];
-                const v3 pos = ld3(B.cur, sd, PX, old);
+            if (MODE == PB_FIRST) {
+                const bool mine = i < T.count;
+                const uint32_t old = mine ? B.perm[q] : 0u;
+                const v3 pos = mine ? ld3(B.cur, sd, PX, old) : mk(0, 0, 0);
+                // row of this particle in nxt: its slot in the cell-interleaved order of the round
+                // (single-scene kernels, allocated runs), else its sorted row
+                uint32_t qd = q;
+                if (POW2 && ok) qd = T.start + r * blockDim.x + round_slot<POW2>(P, T, mine, pos);
+                if (!mine) continue;
                 const float mass = ld(B.cur, sd, PM, old);
                 const m3 Fe = ldm(B.cur, sd, FE, old);
                 const float Jp = ld(B.cur, sd, JP, old);
                 m3 D = ldm(B.cur, sd, DD, old);
                 const v3 dx = ld3(B.cur, sd, DXX, old);
-                B.pid_nxt[q] = B.pid_cur[old];
-                st3(B.nxt, sd, PX, q, pos); st(B.nxt, sd, PM, q, mass);
-                stm(B.nxt, sd, FE, q, Fe); st(B.nxt, sd, JP, q, Jp);
-                if (P.per_particle_lame) { st(B.nxt, sd, MU0, q, ld(B.cur, sd, MU0, old)); st(B.nxt, sd, LA0, q, ld(B.cur, sd, LA0, old)); }
+                B.pid_nxt[qd] = B.pid_cur[old];
+                st3(B.nxt, sd, PX, qd, pos); st(B.nxt, sd, PM, qd, mass);
+                stm(B.nxt, sd, FE, qd, Fe); st(B.nxt, sd, JP, qd, Jp);
+                if (P.per_particle_lame) { st(B.nxt, sd, MU0, qd, ld(B.cur, sd, MU0, old)); st(B.nxt, sd, LA0, qd, ld(B.cur, sd, LA0, old)); }
                 const float relax = pbmpm_relax(P, Jp);
-                B.relax[q] = relax;
+                B.relax[qd] = relax;
                 if (ok) {
                     solve_constraints(P, Fe, relax, D);
                     Stencil s; stencil<POW2>(P, T, pos, s);
                     float off[3][3]; stencil_offsets(s, off);
                     p2g_pbmpm<true>(P, T, s, off, mass, dx, D);
-                } else {   // run whose own block is not allocated: carried unchanged to LAST
-                    stm(B.nxt, sd, DD, q, D); st3(B.nxt, sd, DXX, q, dx); st3(B.nxt, sd, VX, q, ld3(B.cur, sd, VX, old));
+                } else {   // run whose own block is not allocated: carried unchanged to LAST (qd = q)
+                    stm(B.nxt, sd, DD, qd, D); st3(B.nxt, sd, DXX, qd, dx); st3(B.nxt, sd, VX, qd, ld3(B.cur, sd, VX, old));
                 }
+                continue;
+            }
+            if (i >= T.count) {
             } else if (MODE == PB_MID) {
                 if (ok) {
                 const v3 pos = ld3(B.nxt, sd, PX, q);
