@@ -991,19 +991,16 @@ struct Tile {
     uint32_t nbr[27];
     int base[3];
     uint32_t start, count, center_ok;
-    // FIRST's row order (cell-interleaved): particles per cell, and per rank level the cells holding
-    // a particle of that rank and the particles before that level
-    uint32_t ocnt[64];
+    // FIRST's row order (cell-interleaved): particles per cell, one buffer per round modulo 3
+    uint32_t ocnt[3][64];
     uint32_t qcnt, qbase;  // LAST: collider-queue entries of the round, and their first slot
-    unsigned long long oball[256];
-    uint32_t obase[256];
 };
 
 MPM_DEV int tile_of(int tx, int ty, int tz) { return tx + 6 * (ty + 6 * tz); }
 
 MPM_DEV void tile_setup(const SimParams& P, const Bufs& B, Tile& T, uint2 ch) {
     if (threadIdx.x < 27) T.nbr[threadIdx.x] = B.nbr[ch.x * 27u + threadIdx.x];
-    if (threadIdx.x < 64) T.ocnt[threadIdx.x] = 0u;
+    if (threadIdx.x < 64) { T.ocnt[0][threadIdx.x] = 0u; T.ocnt[1][threadIdx.x] = 0u; T.ocnt[2][threadIdx.x] = 0u; }
     if (threadIdx.x == 0) T.qcnt = 0u;
     if (threadIdx.x == 0) {
         int b[3]; block_coords(P, B.run_key[ch.x], b);
@@ -1297,29 +1294,31 @@ __global__ void k_resid_roll(Bufs B, int k, uint32_t* peak) {
 // atomics serialise). Positions do not change before LAST, so the order holds for every MID pass.
 // Called by every thread of the workgroup; `mine`: this thread has a particle in the round, at pos.
 template <bool POW2>
-MPM_DEV uint32_t round_slot(const SimParams& P, Tile& T, bool mine, v3 pos) {
+MPM_DEV uint32_t round_slot(const SimParams& P, Tile& T, uint32_t r, bool mine, v3 pos) {
+    uint32_t* cnt = T.ocnt[r % 3u];
     uint32_t cell = 0, rank = 0;
     if (mine) {
         int c[3]; cell_number_in<POW2>(P, pos, c);
         cell = (uint32_t)((c[0] - T.base[0] - 1) + 4 * ((c[1] - T.base[1] - 1) + 4 * (c[2] - T.base[2] - 1)));
         cell = cell < 64u ? cell : 63u;           // the chunk's particles lie in its block
-        rank = atomicAdd(&T.ocnt[cell], 1u);
+        rank = atomicAdd(&cnt[cell], 1u);
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const uint32_t cnt = T.ocnt[threadIdx.x];
-        uint32_t base = 0;
-        for (uint32_t k = 0; k < 256u; ++k) {
-            const uint64_t ball = __ballot(cnt > k);
-            if (!ball) break;
-            if (threadIdx.x == 0) { T.oball[k] = ball; T.obase[k] = base; }
-            base += (uint32_t)__popcll(ball);
-        }
-        T.ocnt[threadIdx.x] = 0u;
+    // every wave walks the rank levels: level k holds one particle of each cell with more than k;
+    // a lane keeps the level of its own rank (cells of that level, particles before it)
+    const uint32_t mc = cnt[lane_id()];
+    uint64_t myball = 0;
+    uint32_t base = 0, mybase = 0;
+    for (uint32_t k = 0; k < 256u; ++k) {
+        const uint64_t ball = __ballot(mc > k);
+        if (!ball) break;
+        if (rank == k) { myball = ball; mybase = base; }
+        base += (uint32_t)__popcll(ball);
     }
-    __syncthreads();
-    // (the next round's counts start after every thread has passed that round's first barrier)
-    return mine ? T.obase[rank] + (uint32_t)__popcll(T.oball[rank] & ((1ull << cell) - 1ull)) : threadIdx.x;
+    // this round's counts were read before the next barrier; the buffer is cleared for round r + 3
+    // after it (round r + 1's barrier), before round r + 2's
+    if (r > 0 && threadIdx.x < 64) T.ocnt[(r - 1u) % 3u][threadIdx.x] = 0u;
+    return mine ? mybase + (uint32_t)__popcll(myball & ((1ull << cell) - 1ull)) : threadIdx.x;
 }
 
 // Appends sorted index q to the collider queue, one atomic per wave (the explicit G2P pass, whose
@@ -1378,7 +1377,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 // row of this particle in nxt: its slot in the cell-interleaved order of the round
                 // (single-scene kernels, allocated runs), else its sorted row
                 uint32_t qd = q;
-                if (POW2 && ok) qd = T.start + r * blockDim.x + round_slot<POW2>(P, T, mine, pos);
+                if (POW2 && ok) qd = T.start + r * blockDim.x + round_slot<POW2>(P, T, r, mine, pos);
                 if (!mine) continue;
                 const float mass = ld(B.cur, sd, PM, old);
                 const m3 Fe = ldm(B.cur, sd, FE, old);
