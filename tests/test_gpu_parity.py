@@ -111,6 +111,32 @@ def test_pbmpm_substeps_bitexact(mpm, orc, table):
     sim.close()
 
 
+@pytest.mark.parametrize("chunk", [256, 2048])
+def test_pbmpm_chunk_sizes_bitexact(mpm, orc, chunk):
+    """Blocks of ~1,000 particles (16 per cell) in work chunks of 256 (one round each) and of 2,048
+    (four or five rounds per chunk: FIRST's cell-interleaved row order cycles through all three of
+    its count buffers). The grid sums are integers, so the results match the oracle for any chunking."""
+    rng = np.random.default_rng(17)
+    n = 8192
+    pos = (np.array([0.375, 0.5, 0.375]) + rng.uniform(0.0, 0.25, (n, 3))).astype(np.float32)
+    vel = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    sc = dict(particles=mpm.make_particles(pos, vel=vel),
+              sim=dict(grid_res=(32, 32, 32), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1.0 / 600,
+                       gravity=(0.0, -9.8, 0.0), pbmpm_iterations=4, pbmpm_force_divisor=4.0),
+              collider=None)
+    sim, o = make_pair(mpm, orc, sc, chunk_particles=chunk)
+    for s in range(3):
+        sim.step(1, 1)
+        o.step(1, 1)
+        assert_particles_equal(mpm, sim.download(), o.particles(), f"chunk {chunk} substep {s + 1}")
+    st = sim.stats()
+    if chunk == 2048:     # one chunk per particle block, the eight dense ones of ~1,000 particles
+        assert st["work_chunks"] == st["particle_blocks"]
+    else:
+        assert st["work_chunks"] >= 32
+    sim.close()
+
+
 def test_explicit_stages_and_substeps_bitexact(mpm, orc):
     sc = mpm.scenes.scene("explicit_small", n_particles=8192, seed=3)
     for stage in (1, 2):
