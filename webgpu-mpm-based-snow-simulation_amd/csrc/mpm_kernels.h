@@ -1449,12 +1449,12 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
     }
 }
 
-// Resident workgroups per CU: MID runs at 5 (96 VGPRs, a few spilled registers, measured faster),
-// FIRST and LAST at 4 (their spills at 5 cost more than the occupancy gains).
+// Resident workgroups per CU: 4, without spills. (FIRST at 5 spills and is slower; MID at 5, with
+// 24 B/lane spilled, measured the same once its rows came in cell-interleaved order.)
 // TRACK: the residual bookkeeping of the learned warm start (B.resid_dx set); a separate
 // instantiation so the untracked passes carry none of its registers.
 template <int MODE, bool TRACK = false>
-__global__ __launch_bounds__(256, MODE == PB_MID ? 5 : 4) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
+__global__ __launch_bounds__(256, 4) void k_pbmpm(SimParams P, Bufs B, int gin, int gout, int gclr, int it) {
     __shared__ Tile T;
     const uint32_t nchunks = B.counters[C_NCHUNKS];
     if (gclr >= 0) {
