@@ -1415,10 +1415,8 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 if (TRACK) resid = track_residual(P, B, q, dx, it);
                 m3 D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                 solve_constraints(P, Fe, relax, D);
-                // the stencil is recomputed (same values) rather than kept live across the solve
-                Stencil s2; stencil<POW2>(P, T, pos, s2);
-                float off2[3][3]; stencil_offsets(s2, off2);
-                p2g_pbmpm<false>(P, T, s2, off2, mass, dx, D);
+                // the G2P stencil stays live across the solve (4 waves/SIMD leave the registers)
+                p2g_pbmpm<false>(P, T, s, off, mass, dx, D);
                 }
             } else {
                 Part p;
