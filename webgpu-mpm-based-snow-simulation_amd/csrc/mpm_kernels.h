@@ -1437,7 +1437,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 }
                 cand = integrate_store(P, B, p, q);
             }
-            if (MODE == PB_LAST) queue_collider_wg(B, T, cand, q);
+            if (MODE == PB_LAST && P.has_collider) queue_collider_wg(B, T, cand, q);   // uniform: P is the chunk's
             if (MODE != PB_FIRST && TRACK && it > 0) publish_residual(B, resid, it);
         }
         __syncthreads();
