@@ -1352,8 +1352,8 @@ MPM_DEV void queue_collider_wg(const Bufs& B, Tile& T, bool cand, uint32_t q) {
 }
 
 // One work chunk of a PBMPM transfer pass (the body of k_pbmpm; the batched-scene kernel calls
-// it with the chunk's scene parameters and POW2 = false: no power-of-two cell products and no
-// cell-interleaved round order, which there cost more than they saved).
+// it with the chunk's scene parameters and POW2 = false: no power-of-two cell products, which
+// there cost more in register allocation than they saved).
 template <int MODE, bool TRACK = false, bool POW2 = true>
 MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk, const int4* __restrict__ gi,
                          int4* __restrict__ go, int it) {
@@ -1375,9 +1375,9 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                 const uint32_t old = mine ? B.perm[q] : 0u;
                 const v3 pos = mine ? ld3(B.cur, sd, PX, old) : mk(0, 0, 0);
                 // row of this particle in nxt: its slot in the cell-interleaved order of the round
-                // (single-scene kernels, allocated runs), else its sorted row
+                // (allocated runs), else its sorted row
                 uint32_t qd = q;
-                if (POW2 && ok) qd = T.start + r * blockDim.x + round_slot<POW2>(P, T, r, mine, pos);
+                if (ok) qd = T.start + r * blockDim.x + round_slot<POW2>(P, T, r, mine, pos);
                 if (!mine) continue;
                 const float mass = ld(B.cur, sd, PM, old);
                 const m3 Fe = ldm(B.cur, sd, FE, old);
