@@ -383,7 +383,9 @@ uint32_t grid_for(uint64_t work, uint32_t cap_wg) {
 // One substep (or a debug prefix of it) enqueued on the stream. Returns the stage reached.
 void enqueue_binning(mpm_ctx* c, const SimParams& P, const Bufs& B) {
     hipStream_t s = c->stream;
-    const uint32_t gk = grid_for(P.direct_blocks ? (uint64_t)c->n : std::max<uint64_t>(c->n, c->cap), 8192);
+    // all workgroups resident at once (8 per CU at 256 threads): measured faster than one particle
+    // per thread in two dispatch waves (1M particles: 30.3 -> 26.7 us)
+    const uint32_t gk = grid_for(P.direct_blocks ? (uint64_t)c->n : std::max<uint64_t>(c->n, c->cap), (uint32_t)c->n_cu * 8u);
     kt(c, -1);
     k_keys_count<<<gk, 256, 0, s>>>(P, B); kt(c, KID_KEYS);
     enqueue_scan(c, P, B, s);
