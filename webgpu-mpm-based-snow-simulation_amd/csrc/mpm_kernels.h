@@ -1365,6 +1365,8 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
         __syncthreads();
         const bool ok = T.center_ok != 0;
         const uint32_t rounds = (T.count + blockDim.x - 1) / blockDim.x;
+        // FIRST: each round's permutation entry is loaded a round ahead
+        uint32_t old_next = (MODE == PB_FIRST && threadIdx.x < T.count) ? B.perm[T.start + threadIdx.x] : 0u;
         for (uint32_t r = 0; r < rounds; ++r) {
             const uint32_t i = r * blockDim.x + threadIdx.x;
             const uint32_t q = T.start + i;
@@ -1372,7 +1374,8 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
             float resid = 0.0f;
             if (MODE == PB_FIRST) {
                 const bool mine = i < T.count;
-                const uint32_t old = mine ? B.perm[q] : 0u;
+                const uint32_t old = old_next;
+                old_next = i + blockDim.x < T.count ? B.perm[q + blockDim.x] : 0u;
                 const v3 pos = mine ? ld3(B.cur, sd, PX, old) : mk(0, 0, 0);
                 // row of this particle in nxt: its slot in the cell-interleaved order of the round
                 // (allocated runs), else its sorted row
