@@ -272,7 +272,7 @@ def main():
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on exactly one GPU; --gpus N > 1 is not supported "
+        print("bench.py: libmpm_hip steps a scene on exactly one GPU; --gpus N > 1 is not supported "
               "(the multi-GPU sharding of the step is design-only)", file=sys.stderr)
         sys.exit(2)
 

@@ -214,9 +214,9 @@ int  mpm_divisor_fast(mpm_ctx* ctx, float d, float scale, int* fast);
  * (GpuMpmPipelineManager.ts:445 nSolveConstraintIterations, pbmpm.wgsl:137 "/ 3.0"). */
 int  mpm_set_iterations(mpm_ctx* ctx, int k, float force_divisor);
 
-/* Learned PBMPM warm start (no reference counterpart; the north_star addition). A 15-64-64-9
- * GELU MLP with BF16 weights on the matrix cores maps a particle's state at the start of a
- * substep to the correction of D that k_ref iterations converge to.
+/* Learned PBMPM warm start (no reference counterpart: an addition to the reference's PBMPM
+ * solver). A 15-64-64-9 GELU MLP with BF16 weights on the matrix cores maps a particle's state at
+ * the start of a substep to the correction of D that k_ref iterations converge to.
  *  - mpm_learn_collect: n_substeps PBMPM substeps with k_ref iterations (force divisor k_ref),
  *    logging (features, target) pairs of every particle every `every` substeps, up to max_pairs.
  *  - mpm_learn_train: Adam on minibatches of `batch` pairs (multiple of 256) drawn with replacement;
