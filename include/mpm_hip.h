@@ -71,7 +71,8 @@ typedef struct mpm_config {
     float    collider_velocity_factor; /* colliderPrelude.wgsl:294 (0.2) */
     uint32_t wall_mode;            /* 0 = reference clamp+restitution; bit0 = also sticky grid border */
     uint32_t wall_cells;           /* sticky border thickness in cells (3) */
-    uint32_t chunk_particles;      /* particles per LDS-tiled work item (0 = default 512) */
+    uint32_t chunk_particles;      /* particles per LDS-tiled work item: 0 = default 512, else 256..65536
+                                      (MPM_E_RANGE otherwise); a batch takes scene 0's value */
     uint32_t grid_workgroups;      /* workgroups of the chunk kernels (0 = derived from CU count) */
     int32_t  device;               /* HIP device ordinal */
     uint32_t use_graph;            /* capture one substep in a hipGraph and replay it */

@@ -300,25 +300,27 @@ template <class R, class AP> struct Sim {
         std::fill(gz.begin(), gz.begin() + cells, A(0));
     }
     void dep(uint32_t ci, A* g, A v) { g[ci] = AP::add(g[ci], v); }
+    A conv_acc(R x, float S) const { return AP::conv(x, S); }
 
     // ------------------------------------------------ particleToGrid.cs.wgsl:27-79
+    // stressTensorOps.wgsl:39-56 (hardened Lame parameters) and :9-22 (corotated PK1 stress)
+    M3<R> pk1_explicit(const Particle<R>& pt) const {
+        R mu = (R)(float)(P.youngs_modulus / (2.0 * (1.0 + P.poissons_ratio)));
+        R la = (R)(float)(P.youngs_modulus * P.poissons_ratio / ((1.0 + P.poissons_ratio) * (1.0 - 2.0 * P.poissons_ratio)));
+        if (pt.own_lame) { mu = pt.mu0; la = pt.la0; }
+        R Jp = det3(pt.Fp);
+        R ef = r_exp((R)P.hardening_explicit * ((R)1 - Jp));
+        mu = mu * ef; la = la * ef;
+        R J = det3(pt.Fe);
+        M3<R> Rr = polar_rotation(pt.Fe);
+        return madd(mscale((R)2 * mu, msub(pt.Fe, Rr)), mscale((la * (J - (R)1)) * J, transpose(inverse3(pt.Fe))));
+    }
     void p2g_explicit(const Particle<R>& pt) {
         const R S = (R)U.S;
         int32_t c0[3]; cell_number(pt.pos, c0);
         V3<R> f = frac_pos(pt.pos, c0);
         V3<R> w[3], d[3]; weights(f, w); weight_derivs(f, d);
-        R mu = (R)(float)(P.youngs_modulus / (2.0 * (1.0 + P.poissons_ratio)));
-        R la = (R)(float)(P.youngs_modulus * P.poissons_ratio / ((1.0 + P.poissons_ratio) * (1.0 - 2.0 * P.poissons_ratio)));
-        if (pt.own_lame) { mu = pt.mu0; la = pt.la0; }
-        // stressTensorOps.wgsl:39-56
-        R Jp = det3(pt.Fp);
-        R ef = r_exp((R)P.hardening_explicit * ((R)1 - Jp));
-        mu = mu * ef; la = la * ef;
-        // stressTensorOps.wgsl:9-22
-        R J = det3(pt.Fe);
-        M3<R> Rr = polar_rotation(pt.Fe);
-        M3<R> Pk = madd(mscale((R)2 * mu, msub(pt.Fe, Rr)), mscale((la * (J - (R)1)) * J, transpose(inverse3(pt.Fe))));
-        M3<R> PT = transpose(Pk);
+        M3<R> PT = transpose(pk1_explicit(pt));
         R V = pt.mass * (R)(float)(1.0 / P.density);
         M3<R> negVPT = mscale(-V, PT);
         V3<R> pm = pt.mass * pt.vel;
@@ -908,6 +910,58 @@ void orc_closest_point_triangle(const float* p, const float* a, const float* b, 
 }
 float orc_intersect_ray_triangle(const float* o, const float* d, const float* a, const float* b, const float* c) {
     return Sim32::intersect_ray_triangle({o[0], o[1], o[2]}, {d[0], d[1], d[2]}, {a[0], a[1], a[2]}, {b[0], b[1], b[2]}, {c[0], c[1], c[2]});
+}
+
+// One particle through the PBMPM constraint solve (pbmpm.wgsl:16-39), FP32-faithful: D9 is
+// updated in place. Matrices column-major.
+void orc_solve_constraints1(const orc_params* prm, const float* Fe9, const float* Fp9, float* D9) {
+    Sim32 s; s.P = *prm;
+    Particle<float> pt{};
+    pt.Fe = ld3(Fe9); pt.Fp = ld3(Fp9); pt.D = ld3(D9);
+    s.solve_constraints(pt);
+    st3(pt.D, D9);
+}
+// The explicit method's first Piola-Kirchhoff stress of one particle (stressTensorOps.wgsl:9-22
+// with the hardening of :39-56), FP32-faithful.
+void orc_pk1_explicit1(const orc_params* prm, const float* Fe9, const float* Fp9, float* P9) {
+    Sim32 s; s.P = *prm;
+    Particle<float> pt{};
+    pt.Fe = ld3(Fe9); pt.Fp = ld3(Fp9);
+    st3(s.pk1_explicit(pt), P9);
+}
+// integrateParticles (integrateParticles.wgsl:12-72) on every particle, with no transfer before.
+void orc_integrate_only(void* hp) {
+    ((Handle*)hp)->visit([&](auto* s) { for (auto& pt : s->parts) s->integrate(pt); return 0; });
+}
+// Binning, then every allocated cell set to mass m and momentum m (a + M c) (c = the cell's
+// integer coordinates, M column-major), then the G2P of every binned particle (method 0
+// explicit, gridToParticle.cs.wgsl:28-80; 1 PBMPM, :82-130).
+void orc_g2p_affine(void* hp, int method, const float* a, const float* M9, float m) {
+    ((Handle*)hp)->visit([&](auto* s) {
+        typedef typename std::remove_reference<decltype(s->parts[0].pos.x)>::type R;
+        s->bin();
+        M3<R> M;
+        for (int c = 0; c < 3; ++c) M.c[c] = {(R)M9[3 * c], (R)M9[3 * c + 1], (R)M9[3 * c + 2]};
+        const V3<R> av = {(R)a[0], (R)a[1], (R)a[2]};
+        const float S = s->U.S;
+        for (uint32_t b = 0; b < s->n_alloc; ++b) {
+            uint32_t slot = s->mapped[b];
+            for (uint32_t l = 0; l < 64; ++l) {
+                V3<R> c = {(R)(s->hm_bx[slot] * 4 + (int32_t)(l & 3)), (R)(s->hm_by[slot] * 4 + (int32_t)((l >> 2) & 3)),
+                           (R)(s->hm_bz[slot] * 4 + (int32_t)(l >> 4))};
+                V3<R> v = av + mvmul(M, c);
+                size_t ci = (size_t)b * 64 + l;
+                s->gm[ci] = s->conv_acc((R)m, S);
+                s->gx[ci] = s->conv_acc((R)m * v.x, S);
+                s->gy[ci] = s->conv_acc((R)m * v.y, S);
+                s->gz[ci] = s->conv_acc((R)m * v.z, S);
+            }
+        }
+        for (uint32_t i = 0; i < s->n_sorted; ++i) {
+            if (method) s->g2p_pbmpm(s->parts[s->sorted[i]]); else s->g2p_explicit(s->parts[s->sorted[i]]);
+        }
+        return 0;
+    });
 }
 
 }  // extern "C"

@@ -4,13 +4,16 @@ ctypes front end of oracle/mpm_oracle.cpp: the CPU restatement of the reference'
 substep (FP32-faithful, and an FP64 physics variant). Never imported by the product package.
 """
 import ctypes
+import os
 import pathlib
 import subprocess
 
 import numpy as np
 
 ORACLE_DIR = pathlib.Path(__file__).resolve().parent
-LIB = ORACLE_DIR / "_build" / "liborc.so"
+# MPM_ORACLE_LIB points the checker at another build of these sources (the mutation test
+# compiles copies with planted mistakes and checks that the pins reject them).
+LIB = pathlib.Path(os.environ.get("MPM_ORACLE_LIB", str(ORACLE_DIR / "_build" / "liborc.so")))
 
 
 class OrcParams(ctypes.Structure):
@@ -89,6 +92,10 @@ def lib():
             "orc_bspline": (None, [_P, _P, _P]),
             "orc_closest_point_triangle": (None, [_P, _P, _P, _P, _P]),
             "orc_intersect_ray_triangle": (ctypes.c_float, [_P, _P, _P, _P, _P]),
+            "orc_solve_constraints1": (None, [ctypes.POINTER(OrcParams), _P, _P, _P]),
+            "orc_pk1_explicit1": (None, [ctypes.POINTER(OrcParams), _P, _P, _P]),
+            "orc_integrate_only": (None, [_P]),
+            "orc_g2p_affine": (None, [_P, ctypes.c_int, _P, _P, ctypes.c_float]),
             "orc_noise3": (ctypes.c_float, [ctypes.c_float] * 3),
             "orc_fbm": (ctypes.c_float, [ctypes.c_float] * 3),
             "orc_inside_mesh": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
@@ -178,6 +185,17 @@ class OracleSim:
         lib().orc_get_particles_f64(self.h, _ptr(out))
         return out
 
+    def integrate_only(self):
+        """integrateParticles on every particle with no transfer before it."""
+        lib().orc_integrate_only(self.h)
+
+    def g2p_affine(self, method, a, M, mass=1000.0):
+        """Binning, every allocated cell set to the affine velocity field a + M c (c = integer cell
+        coordinates, M a numpy row-major 3x3) with node mass `mass`, then G2P of every binned particle."""
+        av = np.ascontiguousarray(a, np.float32)
+        Mc = np.ascontiguousarray(np.asarray(M, np.float32).T.reshape(9))
+        lib().orc_g2p_affine(self.h, int(method), _ptr(av), _ptr(Mc), float(mass))
+
     def dump_grid(self):
         n = lib().orc_dump_grid(self.h, None, None, 0)
         xyz = np.zeros((n, 3), np.int32)
@@ -196,6 +214,31 @@ class OracleSim:
         s = OrcStats()
         lib().orc_get_stats(self.h, ctypes.byref(s))
         return {k: getattr(s, k) for k, _ in s._fields_}
+
+
+def _colmajor(m):
+    return np.ascontiguousarray(np.asarray(m, np.float32).T.reshape(9))
+
+
+def _rowmajor(c):
+    return c.reshape(3, 3).T.astype(np.float64)
+
+
+def solve_constraints(Fe, Fp, D, **param_kw):
+    """One particle through the PBMPM constraint solve (pbmpm.wgsl:16-39), FP32: the new D.
+    Matrices as numpy row-major 3x3."""
+    prm = params(**param_kw)
+    d = _colmajor(D)
+    lib().orc_solve_constraints1(ctypes.byref(prm), _ptr(_colmajor(Fe)), _ptr(_colmajor(Fp)), _ptr(d))
+    return _rowmajor(d)
+
+
+def pk1_explicit(Fe, Fp, **param_kw):
+    """The explicit method's hardened corotated first Piola-Kirchhoff stress of one particle."""
+    prm = params(**param_kw)
+    out = np.zeros(9, np.float32)
+    lib().orc_pk1_explicit1(ctypes.byref(prm), _ptr(_colmajor(Fe)), _ptr(_colmajor(Fp)), _ptr(out))
+    return _rowmajor(out)
 
 
 def f32_vec(fn, *arrays):

@@ -98,3 +98,35 @@ def particles_in_spheres(n, rng, vx=2.0):
     vel[: n // 2, 0] = vx
     vel[n // 2:, 0] = -vx
     return m.make_particles(np.concatenate(pts).astype(np.float32), vel=vel)
+
+
+def particles_from_state(mpm, s):
+    """192-byte records from 37 FP32 state words per particle (pos vel mass F_e F_p dx D,
+    matrices column-major), the layout of state_columns."""
+    s = np.asarray(s, np.float32)
+    n = s.shape[0]
+    p = mpm.make_particles(s[:, 0:3], vel=s[:, 3:6], mass=s[:, 6])
+    p["Fe"][:, :, :3] = s[:, 7:16].reshape(n, 3, 3)
+    p["Fp"][:, :, :3] = s[:, 16:25].reshape(n, 3, 3)
+    p["dx"][:, :3] = s[:, 25:28]
+    p["D"][:, :, :3] = s[:, 28:37].reshape(n, 3, 3)
+    return p
+
+
+def load_golden(name):
+    """A whole-scene fixture of tests/golden/ as a dict (meta decoded from JSON)."""
+    import json
+    import pathlib
+    z = np.load(pathlib.Path(__file__).resolve().parent / "golden" / f"{name}.npz")
+    d = {k: z[k] for k in z.files}
+    d["meta"] = json.loads(bytes(d["meta"]).decode())
+    return d
+
+
+def golden_digests():
+    import json
+    import pathlib
+    return json.loads((pathlib.Path(__file__).resolve().parent / "golden" / "digests.json").read_text())
+
+
+GOLDEN_WHOLE = ("explicit_cube_4096", "pbmpm_spheres_4096")

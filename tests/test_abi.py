@@ -6,6 +6,7 @@ import pathlib
 import re
 
 import numpy as np
+import pytest
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 
@@ -170,3 +171,19 @@ def test_deno_shim_config_offsets_match_the_c_struct(mpm):
     assert pairs.pop("size") == ctypes.sizeof(mpm.MpmConfig)
     for name, off in pairs.items():
         assert getattr(mpm.MpmConfig, name).offset == off, name
+
+
+@pytest.mark.parametrize("chunk", [1, 255, 65537, 2 ** 32 - 1])
+def test_create_rejects_chunk_sizes_out_of_range(mpm, chunk):
+    """chunk_particles outside 256..65536 is rejected before any allocation, for a single scene
+    and for a batch (whose chunk list is sized for chunks of at least 256 particles)."""
+    import ctypes
+    lib = mpm.load_library()
+    cfg = mpm.default_config(n_particles=1000, grid_res=[32, 32, 32], grid_min=[0, 0, 0], grid_max=[1, 1, 1], dt=1e-3,
+                             chunk_particles=chunk)
+    h = ctypes.c_void_p()
+    assert lib.mpm_create(ctypes.byref(cfg), ctypes.byref(h)) == -4 and not h.value
+    cfgs = (mpm.MpmConfig * 2)(cfg, cfg)
+    methods = (ctypes.c_int * 2)(0, 1)
+    assert lib.mpm_ensemble_create(ctypes.cast(cfgs, ctypes.c_void_p), ctypes.cast(methods, ctypes.c_void_p), 2,
+                                   ctypes.byref(h)) == -4 and not h.value

@@ -161,3 +161,74 @@ def test_glb_rejects_unsupported(mpm, bad):
         data = _glb(doc, binb)
     with pytest.raises(mpm.MpmError):
         mpm.load_glb(data)
+
+
+def _edit(data, fn):
+    jl = struct.unpack("<I", data[12:16])[0]
+    doc = json.loads(data[20:20 + jl])
+    binb = data[20 + jl + 8:]
+    fn(doc)
+    return _glb(doc, binb)
+
+
+# Counts, offsets and strides that would wrap a size_t range check or an allocation, indices that
+# are fractions / negative / huge, attributes shorter than POSITION, nodes that are not a tree.
+CRAFTED = {
+    "count 2^63+2048, stride 8": lambda d: (d["accessors"][2].update(count=2 ** 63 + 2048),
+                                            d["bufferViews"][d["accessors"][2]["bufferView"]].update(byteStride=8)),
+    "count 2^64": lambda d: d["accessors"][0].update(count=2.0 ** 64),
+    "count 1e300": lambda d: d["accessors"][0].update(count=1e300),
+    "negative count": lambda d: d["accessors"][0].update(count=-1),
+    "fractional count": lambda d: d["accessors"][0].update(count=2.5),
+    "count beyond view": lambda d: d["accessors"][0].update(count=d["accessors"][0]["count"] + 1),
+    "byteOffset -8": lambda d: d["accessors"][0].update(byteOffset=-8),
+    "byteOffset 2^64-8": lambda d: d["accessors"][0].update(byteOffset=2 ** 64 - 8),
+    "view byteOffset huge": lambda d: d["bufferViews"][0].update(byteOffset=2 ** 62),
+    "view byteLength huge": lambda d: d["bufferViews"][0].update(byteLength=2 ** 63),
+    "stride 2": lambda d: d["bufferViews"][0].update(byteStride=2),
+    "stride 6": lambda d: d["bufferViews"][0].update(byteStride=6),
+    "stride 256": lambda d: d["bufferViews"][0].update(byteStride=256),
+    "stride 4 < element": lambda d: d["bufferViews"][0].update(byteStride=4),
+    "accessor index -1": lambda d: d["meshes"][0]["primitives"][0]["attributes"].update(POSITION=-1),
+    "accessor index 1e12": lambda d: d["meshes"][0]["primitives"][0]["attributes"].update(POSITION=1e12),
+    "indices index 0.5": lambda d: d["meshes"][0]["primitives"][0].update(indices=0.5),
+    "bufferView 99": lambda d: d["accessors"][0].update(bufferView=99),
+    "mesh index -3": lambda d: d["nodes"][0].update(mesh=-3),
+    "child index 7": lambda d: d["nodes"][0].update(children=[7]),
+    "node listed twice": lambda d: d["nodes"][0].update(children=[1, 1]),
+    "node cycle": lambda d: d["nodes"][1].update(children=[0]),
+    "scene index 5": lambda d: d.update(scene=5),
+    "texture index 2^40": lambda d: d["materials"][0]["pbrMetallicRoughness"]["baseColorTexture"].update(index=2 ** 40),
+    "material 9": lambda d: d["meshes"][0]["primitives"][0].update(material=9),
+    "normals shorter than positions": lambda d: d["accessors"][d["meshes"][0]["primitives"][0]["attributes"]["NORMAL"]].update(count=3),
+    "uvs shorter than positions": lambda d: d["accessors"][d["meshes"][0]["primitives"][0]["attributes"]["TEXCOORD_0"]].update(count=2),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CRAFTED))
+def test_glb_rejects_crafted_sizes_and_indices(mpm, case):
+    data, _ = _scene()
+    bad = _edit(data, CRAFTED[case])
+    with pytest.raises(mpm.MpmError):
+        mpm.load_glb(bad)
+
+
+def test_glb_accepts_valid_stride(mpm):
+    """An interleaved-style byteStride (16 for a VEC3 float attribute over a copy of the data
+    spread to 16-byte rows) reads the same positions."""
+    data, meshes = _scene()
+    jl = struct.unpack("<I", data[12:16])[0]
+    doc = json.loads(data[20:20 + jl])
+    binb = bytearray(data[20 + jl + 8:])
+    acc = doc["accessors"][0]
+    view = doc["bufferViews"][acc["bufferView"]]
+    src = np.frombuffer(bytes(binb[view["byteOffset"]:view["byteOffset"] + view["byteLength"]]), np.float32).reshape(-1, 3)
+    rows = np.zeros((src.shape[0], 4), np.float32)
+    rows[:, :3] = src
+    off = len(binb)
+    binb += rows.tobytes()
+    doc["bufferViews"].append({"buffer": 0, "byteOffset": off, "byteLength": rows.nbytes, "byteStride": 16})
+    acc["bufferView"] = len(doc["bufferViews"]) - 1
+    got = mpm.load_glb(_glb(doc, bytes(binb)))
+    ref = mpm.load_glb(data)
+    assert np.array_equal(got["positions"], ref["positions"])

@@ -46,6 +46,23 @@ struct JVal {
     double number(const char* k, double dflt) const { const JVal* v = get(k); return (v && v->t == Num) ? v->num : dflt; }
 };
 
+// A JSON number that is a non-negative integer below `limit`: glTF indices, counts, offsets and
+// strides. Rejects fractions, negatives, NaN and values beyond the limit before any cast.
+inline bool as_uint(const JVal* v, double limit, size_t& out) {
+    if (!v || v->t != JVal::Num) return false;
+    const double d = v->num;
+    if (!(d >= 0.0) || !(d < limit) || d != std::floor(d)) return false;
+    out = (size_t)d;
+    return true;
+}
+// An optional such field of an object: absent -> dflt.
+inline bool opt_uint(const JVal* obj, const char* key, size_t dflt, double limit, size_t& out) {
+    const JVal* v = obj ? obj->get(key) : nullptr;
+    if (!v) { out = dflt; return true; }
+    return as_uint(v, limit, out);
+}
+inline size_t arr_len(const JVal* v) { return (v && v->t == JVal::Arr) ? v->arr.size() : 0; }
+
 struct JParser {
     const char* p;
     const char* e;
@@ -206,37 +223,52 @@ struct Loader {
     bool fail(const std::string& m) { if (err.empty()) err = m; return false; }
 
     // accessor elements as doubles; normalised integer attributes keep their raw integer values
-    // (loadScene.ts:190-193 copies the attribute array as it is)
-    bool accessor(int idx, const char* want_type, std::vector<double>& out, int& ncomp, bool allow_norm_int) {
+    // (loadScene.ts:190-193 copies the attribute array as it is). Every count, offset and stride
+    // is checked as an integer before use, and the byte range of the last element is checked
+    // against the buffer view without an expression that can wrap.
+    bool accessor(size_t idx, const char* want_type, std::vector<double>& out, int& ncomp, bool allow_norm_int) {
         const JVal* accs = doc.get("accessors");
-        const JVal* a = accs ? accs->at((size_t)idx) : nullptr;
+        const JVal* a = accs ? accs->at(idx) : nullptr;
         if (!a) return fail("accessor index out of range");
         if (a->get("sparse")) return fail("sparse accessors are not supported");
         const JVal* ty = a->get("type");
         const std::string type = ty && ty->t == JVal::Str ? ty->s : "";
         ncomp = type == "SCALAR" ? 1 : type == "VEC2" ? 2 : type == "VEC3" ? 3 : type == "VEC4" ? 4 : 0;
         if (!ncomp || (want_type && type != want_type)) return fail("unexpected accessor type " + type);
-        const int ct = (int)a->number("componentType", 0);
-        const size_t count = (size_t)a->number("count", 0);
-        const int csize = ct == 5126 || ct == 5125 ? 4 : ct == 5123 || ct == 5122 ? 2 : ct == 5121 || ct == 5120 ? 1 : 0;
+        size_t ct = 0, count = 0, acc_off = 0;
+        const double lim = (double)bin_len + 1.0;
+        if (!opt_uint(a, "componentType", 0, 65536.0, ct)) return fail("invalid componentType");
+        if (!opt_uint(a, "count", 0, lim, count)) return fail("invalid accessor count");
+        if (!opt_uint(a, "byteOffset", 0, lim, acc_off)) return fail("invalid accessor byteOffset");
+        const size_t csize = ct == 5126 || ct == 5125 ? 4 : ct == 5123 || ct == 5122 ? 2 : ct == 5121 || ct == 5120 ? 1 : 0;
         if (!csize) return fail("unsupported component type");
         if (ct != 5126 && ct != 5125 && ct != 5123 && ct != 5121) return fail("signed integer accessors are not supported");
-        const JVal* bvi = a->get("bufferView");
-        if (!bvi || bvi->t != JVal::Num) return fail("accessor without a buffer view");
         const JVal* bvs = doc.get("bufferViews");
-        const JVal* bv = bvs ? bvs->at((size_t)bvi->num) : nullptr;
-        if (!bv) return fail("buffer view index out of range");
-        if ((int)bv->number("buffer", 0) != 0 || !bin) return fail("only the GLB binary chunk is supported as a buffer");
-        const size_t off = (size_t)bv->number("byteOffset", 0) + (size_t)a->number("byteOffset", 0);
-        const size_t stride = bv->get("byteStride") ? (size_t)bv->number("byteStride", 0) : (size_t)(csize * ncomp);
-        const size_t view_len = (size_t)bv->number("byteLength", 0);
-        const size_t view_off = (size_t)bv->number("byteOffset", 0);
-        if (count && (off + (count - 1) * stride + (size_t)csize * ncomp > view_off + view_len || view_off + view_len > bin_len))
-            return fail("accessor reads beyond its buffer view");
+        size_t bvi = 0;
+        if (!as_uint(a->get("bufferView"), (double)arr_len(bvs), bvi)) return fail("accessor without a valid buffer view");
+        const JVal* bv = bvs->at(bvi);
+        size_t buf = 0, view_off = 0, view_len = 0;
+        if (!opt_uint(bv, "buffer", 0, 65536.0, buf) || buf != 0 || !bin) return fail("only the GLB binary chunk is supported as a buffer");
+        if (!opt_uint(bv, "byteOffset", 0, lim, view_off) || !opt_uint(bv, "byteLength", 0, lim, view_len))
+            return fail("invalid buffer view range");
+        if (view_off > bin_len || view_len > bin_len - view_off) return fail("buffer view beyond the BIN chunk");
+        const size_t elem = csize * (size_t)ncomp;
+        size_t stride = elem;
+        if (bv->get("byteStride")) {
+            // glTF: a defined byteStride is a multiple of 4 in [4, 252], at least the element size
+            if (!opt_uint(bv, "byteStride", 0, 253.0, stride) || stride % 4 || stride < 4 || stride < elem)
+                return fail("invalid byteStride");
+        }
+        if (count) {
+            if (acc_off > view_len || elem > view_len - acc_off) return fail("accessor reads beyond its buffer view");
+            if (count - 1 > (view_len - acc_off - elem) / stride) return fail("accessor reads beyond its buffer view");
+        }
+        const size_t off = view_off + acc_off;
         const JVal* nz = a->get("normalized");
         const bool norm = nz && nz->t == JVal::Bool && nz->b;
         if (norm && !allow_norm_int) return fail("normalised accessor where floats are expected");
-        out.resize(count * ncomp);
+        if (ct != 5126 && !norm && want_type && std::string(want_type) != "SCALAR") return fail("integer vertex attributes must be normalised");
+        out.resize(count * (size_t)ncomp);
         for (size_t j = 0; j < count; ++j)
             for (int c = 0; c < ncomp; ++c) {
                 const uint8_t* q = bin + off + j * stride + (size_t)c * csize;
@@ -247,23 +279,26 @@ struct Loader {
                 else { v = q[0]; }
                 out[j * ncomp + c] = v;
             }
-        if (ct != 5126 && !norm && want_type && std::string(want_type) != "SCALAR") return fail("integer vertex attributes must be normalised");
         return true;
     }
 
-    int texture_order(const JVal* prim) {
+    // first-appearance order of the primitive's base-colour texture, 0 without one
+    bool texture_order(const JVal* prim, int& order) {
+        order = 0;
         const JVal* mi = prim->get("material");
-        if (!mi || mi->t != JVal::Num) return 0;
+        if (!mi) return true;
         const JVal* mats = doc.get("materials");
-        const JVal* m = mats ? mats->at((size_t)mi->num) : nullptr;
-        const JVal* pbr = m ? m->get("pbrMetallicRoughness") : nullptr;
+        size_t m_i = 0;
+        if (!as_uint(mi, (double)arr_len(mats), m_i)) return fail("material index out of range");
+        const JVal* pbr = mats->at(m_i)->get("pbrMetallicRoughness");
         const JVal* bct = pbr ? pbr->get("baseColorTexture") : nullptr;
-        if (!bct) return 0;
-        const int ti = (int)bct->number("index", -1);
-        if (ti < 0) return 0;
-        if ((size_t)ti >= tex_of_texture_index.size()) tex_of_texture_index.resize(ti + 1, -1);
+        if (!bct) return true;
+        size_t ti = 0;
+        if (!as_uint(bct->get("index"), (double)arr_len(doc.get("textures")), ti)) return fail("texture index out of range");
+        if (ti >= tex_of_texture_index.size()) tex_of_texture_index.resize(ti + 1, -1);
         if (tex_of_texture_index[ti] < 0) tex_of_texture_index[ti] = n_textures++;
-        return tex_of_texture_index[ti];
+        order = tex_of_texture_index[ti];
+        return true;
     }
 
     bool primitive(const JVal* prim, const M4& world, Scene& sc) {
@@ -271,20 +306,28 @@ struct Loader {
         const JVal* pa = attrs ? attrs->get("POSITION") : nullptr;
         if (!pa) return fail("primitive without POSITION");
         const JVal* ia = prim->get("indices");
-        if (!ia || ia->t != JVal::Num) return fail("non-indexed primitives are not supported");
-        const int mode = (int)prim->number("mode", 4);
-        if (mode != 4) return fail("only triangle-list primitives are supported");
+        if (!ia) return fail("non-indexed primitives are not supported");
+        size_t mode = 0;
+        if (!opt_uint(prim, "mode", 4, 7.0, mode) || mode != 4) return fail("only triangle-list primitives are supported");
+        const double n_acc = (double)arr_len(doc.get("accessors"));
+        size_t a_pos = 0, a_nor = 0, a_uv = 0, a_idx = 0;
+        const JVal* na = attrs->get("NORMAL");
+        const JVal* ta = attrs->get("TEXCOORD_0");
+        if (!as_uint(pa, n_acc, a_pos) || !as_uint(ia, n_acc, a_idx) || (na && !as_uint(na, n_acc, a_nor)) ||
+            (ta && !as_uint(ta, n_acc, a_uv)))
+            return fail("accessor index out of range");
         std::vector<double> pos, nor, uv, idx;
         int nc;
-        if (!accessor((int)pa->num, "VEC3", pos, nc, false)) return false;
+        if (!accessor(a_pos, "VEC3", pos, nc, false)) return false;
         const size_t nv = pos.size() / 3;
-        const JVal* na = attrs->get("NORMAL");
-        if (na && !accessor((int)na->num, "VEC3", nor, nc, false)) return false;
-        const JVal* ta = attrs->get("TEXCOORD_0");
-        if (ta && !accessor((int)ta->num, "VEC2", uv, nc, true)) return false;
-        if (!accessor((int)ia->num, "SCALAR", idx, nc, false)) return false;
+        if (na && !accessor(a_nor, "VEC3", nor, nc, false)) return false;
+        if (ta && !accessor(a_uv, "VEC2", uv, nc, true)) return false;
+        if (!nor.empty() && nor.size() != pos.size()) return fail("NORMAL count differs from POSITION count");
+        if (!uv.empty() && uv.size() / 2 != nv) return fail("TEXCOORD_0 count differs from POSITION count");
+        if (!accessor(a_idx, "SCALAR", idx, nc, false)) return false;
         if (idx.size() % 3) return fail("index count is not a multiple of 3");
-        const int mat = texture_order(prim);
+        int mat = 0;
+        if (!texture_order(prim, mat)) return false;
         const uint32_t voff = (uint32_t)(sc.positions.size() / 3);
         Scene::Object ob;
         double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -324,11 +367,17 @@ struct Loader {
         return true;
     }
 
-    bool node(int ni, const M4& parent, Scene& sc, int depth) {
+    size_t node_visits = 0;
+
+    // glTF nodes form trees: a node is visited at most once, so more visits than nodes (a node
+    // listed as a child twice, a cycle) is malformed and stops the walk.
+    bool node(const JVal* ref, const M4& parent, Scene& sc, int depth) {
         if (depth > 256) return fail("node hierarchy too deep (cycle?)");
         const JVal* nodes = doc.get("nodes");
-        const JVal* n = nodes ? nodes->at((size_t)ni) : nullptr;
-        if (!n) return fail("node index out of range");
+        size_t ni = 0;
+        if (!as_uint(ref, (double)arr_len(nodes), ni)) return fail("node index out of range");
+        if (++node_visits > arr_len(nodes)) return fail("nodes do not form a tree");
+        const JVal* n = nodes->at(ni);
         M4 local = ident4();
         if (const JVal* m = n->get("matrix")) {
             if (m->t != JVal::Arr || m->arr.size() != 16) return fail("node matrix must have 16 numbers");
@@ -343,13 +392,14 @@ struct Loader {
         const M4 world = mul4(parent, local);
         if (const JVal* mi = n->get("mesh")) {
             const JVal* meshes = doc.get("meshes");
-            const JVal* mesh = meshes ? meshes->at((size_t)mi->num) : nullptr;
-            const JVal* prims = mesh ? mesh->get("primitives") : nullptr;
+            size_t m_i = 0;
+            if (!as_uint(mi, (double)arr_len(meshes), m_i)) return fail("mesh index out of range");
+            const JVal* prims = meshes->at(m_i)->get("primitives");
             if (!prims || prims->t != JVal::Arr) return fail("mesh without primitives");
             for (const JVal& p : prims->arr) if (!primitive(&p, world, sc)) return false;
         }
         if (const JVal* ch = n->get("children"))
-            for (const JVal& c : ch->arr) if (!node((int)c.num, world, sc, depth + 1)) return false;
+            for (const JVal& c : ch->arr) if (!node(&c, world, sc, depth + 1)) return false;
         return true;
     }
 
@@ -372,12 +422,12 @@ struct Loader {
         if (!jp.value(doc, 0) || doc.t != JVal::Obj) return fail("malformed JSON chunk" + (jp.err.empty() ? std::string() : ": " + jp.err));
         if (const JVal* bufs = doc.get("buffers"))
             for (const JVal& b : bufs->arr) if (b.get("uri")) return fail("external or data-URI buffers are not supported");
-        const int si = (int)doc.number("scene", 0);
         const JVal* scenes = doc.get("scenes");
-        const JVal* s = scenes ? scenes->at((size_t)si) : nullptr;
-        if (!s) return fail("no scene");
+        size_t si = 0;
+        if (!opt_uint(&doc, "scene", 0, 1e9, si) || si >= arr_len(scenes)) return fail("no scene");
+        const JVal* s = scenes->at(si);
         if (const JVal* roots = s->get("nodes"))
-            for (const JVal& r : roots->arr) if (!node((int)r.num, ident4(), sc, 0)) return false;
+            for (const JVal& r : roots->arr) if (!node(&r, ident4(), sc, 0)) return false;
         return true;
     }
 };

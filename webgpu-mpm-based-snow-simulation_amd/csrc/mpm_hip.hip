@@ -15,12 +15,15 @@
 #include <vector>
 #include <algorithm>
 #include <map>
+#include <new>
+#include <stdexcept>
 #include "mpm_kernels.h"
 #include "mpm_mlp.h"
 #include "mpm_ensemble.h"
 #include "mpm_scatter.h"
 #include "mpm_splat.h"
 #include "mpm_gltf.h"
+#include "mpm_bvh.h"
 #include "../../include/mpm_hip.h"
 
 using namespace mpm;
@@ -62,6 +65,8 @@ struct mpm_ctx {
     hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     SimParams gparams[2][2];
+    Bufs gbufs[2][2];                 // device addresses a captured graph's kernels were given
+    MlpBufs gmlp[2][2];
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     mpm_timings tm{};
@@ -197,12 +202,14 @@ template <class T> void wru(uint8_t* b, size_t off, T v) { std::memcpy(b + off, 
 
 MlpBufs mlp_bufs(mpm_ctx* c) {
     MlpBufs M;
+    std::memset(&M, 0, sizeof(M));   // padding too: graph keys compare the bytes
     M.params = c->mlp_params; M.wbf = c->mlp_wbf; M.grads = c->mlp_grads; M.norm = c->mlp_norm;
     return M;
 }
 
 Bufs make_bufs(mpm_ctx* c) {
     Bufs B;
+    std::memset(&B, 0, sizeof(B));   // padding too: graph keys compare the bytes
     B.cur = c->planes[c->cur]; B.nxt = c->planes[c->cur ^ 1];
     B.pid_cur = c->pid[c->cur]; B.pid_nxt = c->pid[c->cur ^ 1]; B.fpid = c->fpid;
     B.stride = c->stride;
@@ -465,9 +472,16 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
     return true;
 }
 
+// A captured substep holds its kernels' arguments by value: the parameters and every device
+// address. It is replayed only while both are unchanged; a buffer freed and reallocated since
+// (a new collider, residual tracking or network buffers switched on) forces a new capture.
 int ensure_graph(mpm_ctx* c, int method, const SimParams& P) {
     const int par = c->cur;
-    if (c->gexec[method][par] && std::memcmp(&c->gparams[method][par], &P, sizeof(SimParams)) == 0) return MPM_OK;
+    const Bufs B = make_bufs(c);
+    const MlpBufs M = mlp_bufs(c);
+    if (c->gexec[method][par] && std::memcmp(&c->gparams[method][par], &P, sizeof(SimParams)) == 0 &&
+        std::memcmp(&c->gbufs[method][par], &B, sizeof(Bufs)) == 0 && std::memcmp(&c->gmlp[method][par], &M, sizeof(MlpBufs)) == 0)
+        return MPM_OK;
     if (c->gexec[method][par]) { hipGraphExecDestroy(c->gexec[method][par]); c->gexec[method][par] = nullptr; }
     if (c->graph[method][par]) { hipGraphDestroy(c->graph[method][par]); c->graph[method][par] = nullptr; }
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -480,41 +494,9 @@ int ensure_graph(mpm_ctx* c, int method, const SimParams& P) {
     c->graph[method][par] = g;
     c->gexec[method][par] = e;
     c->gparams[method][par] = P;
+    c->gbufs[method][par] = B;
+    c->gmlp[method][par] = M;
     return MPM_OK;
-}
-
-// ------------------------------------------------------------------ BVH (GpuColliderBufferManager.ts:222-350)
-struct BTri { double cen[3], mn[3], mx[3]; };
-struct BNode { double mn[3], mx[3]; uint32_t left, right; bool leaf; };
-
-uint32_t bvh_rec(const std::vector<BTri>& tris, std::vector<uint32_t>& idx, uint32_t start, uint32_t end, std::vector<BNode>& nodes) {
-    const uint32_t ni = (uint32_t)nodes.size();
-    const uint32_t count = end - start;
-    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (uint32_t i = start; i < end; ++i) {
-        const BTri& t = tris[idx[i]];
-        for (int a = 0; a < 3; ++a) { mn[a] = std::min(mn[a], t.mn[a]); mx[a] = std::max(mx[a], t.mx[a]); }
-    }
-    BNode nd;
-    for (int a = 0; a < 3; ++a) { nd.mn[a] = mn[a]; nd.mx[a] = mx[a]; }
-    if (count <= 4) {
-        nd.left = start; nd.right = count; nd.leaf = true;
-        nodes.push_back(nd);
-        return ni;
-    }
-    const double ex = mx[0] - mn[0], ey = mx[1] - mn[1], ez = mx[2] - mn[2];
-    int axis = 0;
-    if (ey > ex && ey > ez) axis = 1;
-    else if (ez > ex && ez > ey) axis = 2;
-    std::stable_sort(idx.begin() + start, idx.begin() + end,
-                     [&](uint32_t a, uint32_t b) { return tris[a].cen[axis] < tris[b].cen[axis]; });
-    const uint32_t mid = start + count / 2;
-    nd.left = 0; nd.right = 0; nd.leaf = false;
-    nodes.push_back(nd);
-    const uint32_t l = bvh_rec(tris, idx, start, mid, nodes);
-    const uint32_t r = bvh_rec(tris, idx, mid, end, nodes);
-    nodes[ni].left = l; nodes[ni].right = r;
-    return ni;
 }
 
 // Device buffers of a context, sized by n, stride, K, cap, max_blocks, max_runs, max_chunks, ntiles.
@@ -695,6 +677,9 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     if (cfg->n_particles == 0) return MPM_E_INVALID;
     for (int a = 0; a < 3; ++a) if (cfg->grid_res[a] == 0 || cfg->grid_res[a] > 65536 || !(cfg->grid_max[a] > cfg->grid_min[a])) return MPM_E_INVALID;
     if (cfg->pbmpm_iterations < 1 || cfg->pbmpm_iterations > 64) return MPM_E_INVALID;
+    // chunk counts are formed as (count + chunk - 1) / chunk in u32: both stay far below 2^32
+    if (cfg->n_particles > 0x7FFFFFFFu) return MPM_E_RANGE;
+    if (cfg->chunk_particles && (cfg->chunk_particles < 256 || cfg->chunk_particles > 65536)) return MPM_E_RANGE;
     mpm_ctx* c = new mpm_ctx();
     c->cfg = *cfg;
     c->device = cfg->device;
@@ -826,41 +811,17 @@ int mpm_set_collider(mpm_ctx* c, const uint32_t* data, size_t n_words, uint32_t 
 int mpm_build_bvh(const float* pos, size_t n_vertices, const uint32_t* indices, size_t n_indices,
                   void* nodes_out, size_t nodes_cap, size_t* n_nodes, uint32_t* tri_order) {
     if (!n_nodes || (n_indices && (!pos || !indices)) || n_indices % 3) return MPM_E_INVALID;
-    const size_t nt = n_indices / 3;
-    std::vector<BNode> nodes;
-    std::vector<uint32_t> idx(nt);
-    if (nt == 0) {
-        BNode d{}; d.leaf = true; d.left = 0; d.right = 0;
-        nodes.push_back(d);
-    } else {
-        std::vector<BTri> tris(nt);
-        for (size_t t = 0; t < nt; ++t) {
-            double v[3][3];
-            for (int k = 0; k < 3; ++k) {
-                uint32_t vi = indices[3 * t + k];
-                if (vi >= n_vertices) return MPM_E_INVALID;
-                for (int a = 0; a < 3; ++a) v[k][a] = (double)pos[3 * (size_t)vi + a];
-            }
-            for (int a = 0; a < 3; ++a) {
-                tris[t].mn[a] = std::min(std::min(v[0][a], v[1][a]), v[2][a]);
-                tris[t].mx[a] = std::max(std::max(v[0][a], v[1][a]), v[2][a]);
-                tris[t].cen[a] = (v[0][a] + v[1][a] + v[2][a]) / 3.0;
-            }
-            idx[t] = (uint32_t)t;
-        }
-        bvh_rec(tris, idx, 0, (uint32_t)nt, nodes);
+    std::vector<mpm_bvh::BNode> nodes;
+    std::vector<uint32_t> order;
+    try {
+        if (!mpm_bvh::build(pos, n_vertices, indices, n_indices, nodes, order)) return MPM_E_INVALID;
+    } catch (const std::bad_alloc&) {
+        return MPM_E_NOMEM;
     }
     *n_nodes = nodes.size();
     if (!nodes_out || nodes_cap < nodes.size()) return MPM_E_RANGE;
-    uint8_t* o = (uint8_t*)nodes_out;
-    for (size_t i = 0; i < nodes.size(); ++i) {
-        const BNode& nd = nodes[i];
-        float f[6] = {(float)nd.mn[0], (float)nd.mn[1], (float)nd.mn[2], (float)nd.mx[0], (float)nd.mx[1], (float)nd.mx[2]};
-        uint32_t r = nd.leaf ? (nd.right | 0x80000000u) : nd.right;
-        std::memcpy(o + 32 * i, f, 12); std::memcpy(o + 32 * i + 12, &nd.left, 4);
-        std::memcpy(o + 32 * i + 16, f + 3, 12); std::memcpy(o + 32 * i + 28, &r, 4);
-    }
-    if (tri_order) for (size_t t = 0; t < nt; ++t) tri_order[t] = idx[t];
+    mpm_bvh::pack(nodes, (uint8_t*)nodes_out);
+    if (tri_order) std::memcpy(tri_order, order.data(), order.size() * 4);
     return MPM_OK;
 }
 
@@ -945,7 +906,16 @@ int mpm_load_glb(const void* glb, size_t len, mpm_scene_arrays* a, char* err, si
     if (!glb || !a) { seterr("null argument"); return MPM_E_INVALID; }
     mpm_gltf::Scene sc;
     mpm_gltf::Loader ld;
-    if (!ld.load((const uint8_t*)glb, len, sc)) { seterr(ld.err); return MPM_E_INVALID; }
+    // no exception crosses the C ABI: an allocation the file's sizes make impossible is an error
+    try {
+        if (!ld.load((const uint8_t*)glb, len, sc)) { seterr(ld.err); return MPM_E_INVALID; }
+    } catch (const std::bad_alloc&) {
+        seterr("out of host memory while reading the GLB");
+        return MPM_E_NOMEM;
+    } catch (const std::exception& ex) {
+        seterr(std::string("GLB reader: ") + ex.what());
+        return MPM_E_INVALID;
+    }
     const size_t nv = sc.positions.size() / 3, nn = sc.normals.size() / 3, ni = sc.indices.size(), no = sc.objects.size(), ns = sc.soup.size() / 4;
     const bool room = (!a->positions || a->n_vertices >= nv) && (!a->uvs || a->n_vertices >= nv) &&
                       (!a->material_indices || a->n_vertices >= nv) && (!a->normals || a->n_normals >= nn) &&
@@ -1436,6 +1406,8 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
         if (g.n_particles == 0 || (methods[s] != MPM_METHOD_EXPLICIT && methods[s] != MPM_METHOD_PBMPM)) return MPM_E_INVALID;
         for (int a = 0; a < 3; ++a) if (g.grid_res[a] == 0 || g.grid_res[a] > 65536 || !(g.grid_max[a] > g.grid_min[a])) return MPM_E_INVALID;
         if (g.pbmpm_iterations < 1 || g.pbmpm_iterations > 64) return MPM_E_INVALID;
+        // the chunk list is sized for chunks of at least 256 particles (and the tuning's range)
+        if (g.chunk_particles && (g.chunk_particles < 256 || g.chunk_particles > 65536)) return MPM_E_RANGE;
         n += g.n_particles;
         K += (uint64_t)((g.grid_res[0] + 3u) / 4u) * ((g.grid_res[1] + 3u) / 4u) * ((g.grid_res[2] + 3u) / 4u);
         R = std::max<uint32_t>(R, methods[s] == MPM_METHOD_PBMPM ? (uint32_t)g.pbmpm_iterations + 1u : 2u);
