@@ -93,15 +93,22 @@ typedef struct mpm_stats {
     uint64_t probe_failures;         /* hash insert exhausted max_probes */
     uint64_t substeps;
     uint64_t mlp_fallbacks;          /* particles whose warm-start prediction was dropped (last substep) */
+    uint64_t nonfinite_particles;    /* particles whose position was NaN or infinite at binning (binned at
+                                        cell 0 with zero grid contributions, as the reference's i32(NaN) = 0) */
 } mpm_stats;
 
-/* Per-phase device time of the last timed mpm_step (timing enabled with mpm_set_timing). */
+/* Device time of the last mpm_step / mpm_ensemble_step made with timing on (mpm_set_timing).
+ * compute_simulation_step_ns spans the whole call, all pass_substeps substeps, as the reference's
+ * computeSimulationStepNs spans the simulation compute pass of a frame (timestamp slots 0/1,
+ * GpuSnowPipelineRunner.svelte.ts:411-420,588-611). The three phases are those of the call's last
+ * substep, which is launched eagerly with events between its phases. */
 typedef struct mpm_timings {
     double compute_simulation_step_ns;   /* GpuSnowPipelineRunner.svelte.ts:520-524 computeSimulationStepNs */
-    double binning_ns;
-    double transfer_ns;                  /* P2G / G2P / constraint passes */
-    double integrate_ns;
-    uint64_t substeps;
+    double binning_ns;                   /* last substep: keys, scan, scatter (and table) */
+    double transfer_ns;                  /* last substep: P2G / grid update / G2P / constraint passes */
+    double integrate_ns;                 /* last substep: integrate, collider, unbinned particles */
+    uint64_t substeps;                   /* substeps stepped by the context so far */
+    uint64_t pass_substeps;              /* substeps inside compute_simulation_step_ns */
 } mpm_timings;
 
 void mpm_config_defaults(mpm_config* cfg);

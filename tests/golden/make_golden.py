@@ -10,6 +10,8 @@ Two kinds of fixture:
     cell-keyed grid of the first substep's last transfer (explicit: after the grid update; PBMPM:
     after the last iteration's P2G), as allocated block coordinates and every cell with a nonzero
     i32 accumulator.
+  * box_glb.npz, the reference's box collider asset (static/box.glb) as the library's GLB reader
+    returns it (positions, indices, normals), for GPU runs, where the reference tree is absent.
   * digests.json, larger seeded scenes (65,536 particles) stored as SHA-256 digests of the final
     particle state and of the cell-keyed grid, with the inputs regenerated from their seeds
     (the input digest is stored too, so a changed generator is detected).
@@ -115,7 +117,22 @@ def grid_digest(xyz, vals):
     return digest(np.concatenate([kx, kv], axis=1))
 
 
+def box_glb_fixture():
+    """The reference's static/box.glb (the box collider asset) read by the library's GLB reader:
+    positions, indices and normals, stored so GPU runs use the asset without the reference tree."""
+    src = pathlib.Path("/root/reference/static/box.glb")
+    if not src.exists():
+        print("box_glb: reference asset not present, fixture kept as is")
+        return
+    m = __graft_entry__.load_package()
+    d = m.load_glb(src.read_bytes())
+    np.savez_compressed(HERE / "box_glb.npz", positions=d["positions"], indices=d["indices"], normals=d["normals"],
+                        source_sha256=np.frombuffer(hashlib.sha256(src.read_bytes()).hexdigest().encode(), np.uint8))
+    print(f"box_glb: {d['positions'].shape[0]} vertices, {d['indices'].size // 3} triangles")
+
+
 def main():
+    box_glb_fixture()
     for name, spec in WHOLE.items():
         p, uni, words, nodes, okw, meta = setup(name, spec)
         meta["grid_stage"] = grid_stage(meta)

@@ -55,3 +55,31 @@ def test_golden_digest_reproduced_by_oracle(mpm, orc, name):
     out, xyz, vals, _ = mg.run_oracle(p, uni, words, nodes, okw, d["method"], d["substeps"], d["grid_stage"])
     assert mg.digest(mg.state37(out)) == d["out_state_sha256"]
     assert mg.grid_digest(xyz, vals) == d["grid_sha256"]
+
+
+def test_box_glb_fixture(mpm):
+    """The reference's box collider asset (static/box.glb) as the GLB reader returns it: the closed
+    unit cube, 12 triangles, total area 6, every edge shared by two triangles, signed volume -1: the
+    asset winds outward and the reader's (x, z, y) swap (loadScene.ts:27-30) is a reflection, which
+    reverses the winding. Where the reference tree is present the asset is read again and must give
+    the stored arrays bit for bit."""
+    import pathlib
+    z = np.load(pathlib.Path(__file__).resolve().parent / "golden" / "box_glb.npz")
+    v, idx = z["positions"].astype(np.float64), z["indices"].astype(np.int64)
+    tri = v[idx.reshape(-1, 3)]
+    cr = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    assert idx.size == 36 and np.allclose(0.5 * np.linalg.norm(cr, axis=1).sum(), 6.0)
+    assert np.isclose((tri[:, 0] * cr).sum() / 6.0, -1.0)
+    assert np.allclose(np.abs(v), 0.5)
+    key = lambda p: tuple(np.round(p, 6))
+    edges = {}
+    for t in tri:
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            e = tuple(sorted((key(t[a]), key(t[b]))))
+            edges[e] = edges.get(e, 0) + 1
+    assert set(edges.values()) == {2}
+    src = pathlib.Path("/root/reference/static/box.glb")
+    if src.exists():
+        d = mpm.load_glb(src.read_bytes())
+        assert np.array_equal(d["positions"], z["positions"]) and np.array_equal(d["indices"], z["indices"])
+        assert np.array_equal(d["normals"], z["normals"])

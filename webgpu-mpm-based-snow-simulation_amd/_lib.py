@@ -67,7 +67,7 @@ class MpmConfig(ctypes.Structure):
 class MpmStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "n_particles", "particles_binned", "particles_unbinned", "particle_blocks", "work_chunks",
-        "blocks_allocated", "blocks_dropped", "probe_failures", "substeps", "mlp_fallbacks")]
+        "blocks_allocated", "blocks_dropped", "probe_failures", "substeps", "mlp_fallbacks", "nonfinite_particles")]
 
 
 class MpmTimings(ctypes.Structure):
@@ -77,6 +77,7 @@ class MpmTimings(ctypes.Structure):
         ("transfer_ns", ctypes.c_double),
         ("integrate_ns", ctypes.c_double),
         ("substeps", ctypes.c_uint64),
+        ("pass_substeps", ctypes.c_uint64),
     ]
 
 

@@ -88,8 +88,10 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
         const uint32_t i = it * gsz + gi;
         bool valid = false;
         uint32_t key = EMPTY, mask = 0;
+        v3 pos = {0.0f, 0.0f, 0.0f};
+        if (i < n) pos = ld3(B.cur, B.stride, PX, i);
+        count_nonfinite(B, i < n, pos);
         if (i < n) {
-            const v3 pos = ld3(B.cur, B.stride, PX, i);
             with_scene(E, seg_find(pb, E.S, B.pid_cur[i]), [&](const SimParams& P) {
                 int c[3]; cell_number(P, pos, c);
                 mask = touched_mask(P, c);

@@ -68,7 +68,7 @@ struct mpm_ctx {
     Bufs gbufs[2][2];                 // device addresses a captured graph's kernels were given
     MlpBufs gmlp[2][2];
     bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // phases of a substep; [4], [5] the whole call
     mpm_timings tm{};
     uint64_t substeps = 0;
     std::vector<void*> allocs;
@@ -177,6 +177,27 @@ void kt_collect(mpm_ctx* c) {
 
 int fail(mpm_ctx* c, int code, const std::string& m) { if (c) c->err = m; return code; }
 #define HIPCHK(c, x) do { hipError_t _e = (x); if (_e != hipSuccess) return fail((c), MPM_E_HIP, std::string(#x) + ": " + hipGetErrorString(_e)); } while (0)
+
+// After a step with timing on: the whole call's device time between events recorded before its
+// first and after its last substep (the reference's compute-pass timestamps, slots 0 and 1,
+// GpuSnowPipelineRunner.svelte.ts:411-420,588-611), and the phases of its last substep, which
+// was launched eagerly with phase events (binning, transfer passes, integrate + collider).
+int finish_timing(mpm_ctx* c, int n_substeps) {
+    HIPCHK(c, hipEventRecord(c->ev[5], c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev[5]));
+    float pass = 0, a = 0, b = 0, d = 0;
+    HIPCHK(c, hipEventElapsedTime(&pass, c->ev[4], c->ev[5]));
+    if (n_substeps > 0) {
+        HIPCHK(c, hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+        HIPCHK(c, hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+        HIPCHK(c, hipEventElapsedTime(&d, c->ev[2], c->ev[3]));
+    }
+    c->tm.compute_simulation_step_ns = (double)pass * 1e6;
+    c->tm.binning_ns = (double)a * 1e6; c->tm.transfer_ns = (double)b * 1e6; c->tm.integrate_ns = (double)d * 1e6;
+    c->tm.substeps = c->substeps;
+    c->tm.pass_substeps = (uint64_t)n_substeps;
+    return MPM_OK;
+}
 
 template <class T> int dalloc(mpm_ctx* c, T** p, size_t count) {
     void* q = nullptr;
@@ -530,7 +551,7 @@ int alloc_buffers(mpm_ctx* c) {
     k_iota<<<grid_for(c->stride, 4096), 256, 0, c->stream>>>(c->pid[0], c->stride);
     k_fill_u32<<<1, 256, 0, c->stream>>>(c->counters, 0u, C_NCOUNTERS);
     if (hipStreamSynchronize(c->stream) != hipSuccess) { (void)hipGetLastError(); return MPM_E_HIP; }
-    for (int i = 0; i < 4; ++i) hipEventCreate(&c->ev[i]);
+    for (int i = 0; i < 6; ++i) hipEventCreate(&c->ev[i]);
     return MPM_OK;
 }
 
@@ -665,7 +686,7 @@ static void release(mpm_ctx* c) {
         if (c->gexec[m][p]) hipGraphExecDestroy(c->gexec[m][p]);
         if (c->graph[m][p]) hipGraphDestroy(c->graph[m][p]);
     }
-    for (int i = 0; i < 4; ++i) if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    for (int i = 0; i < 6; ++i) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     for (hipEvent_t e : c->kt_ev) hipEventDestroy(e);
     for (void* p : c->allocs) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -994,6 +1015,7 @@ int mpm_step(mpm_ctx* c, int method, int n_substeps) {
     int rc = make_params(c, P);
     if (rc) return rc;
     wru<uint32_t>(c->uni, 8, method == MPM_METHOD_PBMPM ? 1u : 0u);   // writeUsePbmpm
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     for (int s = 0; s < n_substeps; ++s) {
         const bool timed = c->timing && s == n_substeps - 1;
         if (c->cfg.use_graph && !timed && !c->ktiming) {
@@ -1006,17 +1028,8 @@ int mpm_step(mpm_ctx* c, int method, int n_substeps) {
         c->cur ^= 1;
         c->substeps++;
         if (c->ktiming && c->kt_n > 512) kt_collect(c);
-        if (timed) {
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            float a = 0, b = 0, d = 0;
-            hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-            hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
-            hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
-            c->tm.binning_ns = a * 1e6; c->tm.transfer_ns = b * 1e6; c->tm.integrate_ns = d * 1e6;
-            c->tm.compute_simulation_step_ns = (a + b + d) * 1e6;
-            c->tm.substeps = c->substeps;
-        }
     }
+    if (c->timing && (rc = finish_timing(c, n_substeps))) return rc;
     kt_collect(c);
     return MPM_OK;
 }
@@ -1132,9 +1145,19 @@ int mpm_get_stats(mpm_ctx* c, mpm_stats* out) {
     int rc = read_counters(c, h);
     if (rc) return rc;
     std::memset(out, 0, sizeof(*out));
+    // a particle block (run) whose own block was not allocated (dropped beyond max_blocks, or its
+    // table insert exhausted max_probes) is not transferred: its particles count as unbinned
+    uint64_t orphans = 0;
+    const uint32_t nr = h[C_NRUNS];
+    if (nr && nr <= c->max_runs) {
+        std::vector<uint32_t> cnt(nr), nbr(27 * (size_t)nr);
+        HIPCHK(c, hipMemcpy(cnt.data(), c->run_count, nr * 4, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(nbr.data(), c->nbr, nbr.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t r = 0; r < nr; ++r) if (nbr[27 * (size_t)r + 13] == EMPTY) orphans += cnt[r];
+    }
     out->n_particles = c->n;
-    out->particles_binned = h[C_NVALID];
-    out->particles_unbinned = c->n - h[C_NVALID];
+    out->particles_binned = h[C_NVALID] - orphans;
+    out->particles_unbinned = c->n - out->particles_binned;
     out->particle_blocks = h[C_NRUNS];
     out->work_chunks = h[C_NCHUNKS];
     out->blocks_allocated = std::min(h[C_NBLOCKS], c->max_blocks);
@@ -1142,6 +1165,7 @@ int mpm_get_stats(mpm_ctx* c, mpm_stats* out) {
     out->probe_failures = h[C_PROBEFAIL];
     out->substeps = c->substeps;
     out->mlp_fallbacks = h[C_FALLBACK];
+    out->nonfinite_particles = h[C_NONFIN_LAST];
     return MPM_OK;
 }
 
@@ -1549,6 +1573,7 @@ int mpm_ensemble_step(mpm_ctx* c, int n_substeps) {
     if (!c->has_particles) return fail(c, MPM_E_STATE, "every scene needs its particles uploaded");
     HIPCHK(c, hipSetDevice(c->device));
     int rc;
+    if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     for (int s = 0; s < n_substeps; ++s) {
         const bool timed = c->timing && s == n_substeps - 1;
         if (c->cfg.use_graph && !timed && !c->ktiming) {
@@ -1561,17 +1586,8 @@ int mpm_ensemble_step(mpm_ctx* c, int n_substeps) {
         c->cur ^= 1;
         c->substeps++;
         if (c->ktiming && c->kt_n > 512) kt_collect(c);
-        if (timed) {
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            float a = 0, b = 0, d = 0;
-            hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-            hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
-            hipEventElapsedTime(&d, c->ev[2], c->ev[3]);
-            c->tm.binning_ns = a * 1e6; c->tm.transfer_ns = b * 1e6; c->tm.integrate_ns = d * 1e6;
-            c->tm.compute_simulation_step_ns = (a + b + d) * 1e6;
-            c->tm.substeps = c->substeps;
-        }
     }
+    if (c->timing && (rc = finish_timing(c, n_substeps))) return rc;
     kt_collect(c);
     return MPM_OK;
 }

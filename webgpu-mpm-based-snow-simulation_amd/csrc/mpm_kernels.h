@@ -46,7 +46,10 @@ enum Plane : int {
 };
 
 enum Counter : int {
-    C_NVALID = 0, C_NRUNS, C_NCHUNKS, C_NBLOCKS, C_STRAG, C_DROPS, C_PROBEFAIL, C_SUBSTEPS, C_NCOLL, C_FALLBACK, C_NCOUNTERS = 16
+    C_NVALID = 0, C_NRUNS, C_NCHUNKS, C_NBLOCKS, C_STRAG, C_DROPS, C_PROBEFAIL, C_SUBSTEPS, C_NCOLL, C_FALLBACK,
+    C_NONFIN,        // particles with a non-finite position, counted by the keys pass of this substep
+    C_NONFIN_LAST,   // the same count, latched by the scatter pass (which also clears C_NONFIN)
+    C_NCOUNTERS = 16
 };
 
 struct SimParams {
@@ -348,6 +351,15 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
+// Counts the wave's lanes whose position is not finite into C_NONFIN (one atomic per wave that
+// has any). The reference bins such a particle at cell 0 (i32(NaN) = 0) and every one of its
+// grid contributions truncates to 0; the count makes the condition visible.
+MPM_DEV void count_nonfinite(const Bufs& B, bool active, v3 pos) {
+    const bool bad = active && !(isfinite(pos.x) && isfinite(pos.y) && isfinite(pos.z));
+    const uint64_t m = __ballot(bad);
+    if (m && lane_id() == 0) atomicAdd(&B.counters[C_NONFIN], (uint32_t)__popcll(m));
+}
+
 // Key of the block at offset j (0..26, x fastest, -1..1 per axis) from block (bx, by, bz).
 MPM_DEV uint32_t block_key_at(const SimParams& P, int bx, int by, int bz, int j) {
     bx += j % 3 - 1; by += (j / 3) % 3 - 1; bz += j / 9 - 1;
@@ -376,8 +388,10 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
         uint32_t i = it * gsz + gi;
         bool valid = false;
         uint32_t key = EMPTY, mask = 0;
+        v3 pos = {0.0f, 0.0f, 0.0f};
+        if (i < n) pos = ld3(B.cur, B.stride, PX, i);
+        count_nonfinite(B, i < n, pos);
         if (i < n) {
-            v3 pos = ld3(B.cur, B.stride, PX, i);
             int c[3]; cell_number(P, pos, c);
             mask = touched_mask(P, c);
             int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
@@ -519,6 +533,7 @@ __global__ __launch_bounds__(256) void k_scatter(SimParams P, Bufs B) {
     if (NBR) nbr_pass(P, B);
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nvalid = B.counters[C_NVALID];
+    if (i == 0) { B.counters[C_NONFIN_LAST] = B.counters[C_NONFIN]; B.counters[C_NONFIN] = 0u; }
     const uint32_t key = i < P.n ? B.key[i] : EMPTY;
     const bool strag = i < P.n && key == EMPTY;
     if (i < P.n && key != EMPTY) {
