@@ -32,16 +32,22 @@ def test_nonfinite_particles_match_oracle(mpm, orc, method):
     sc = dict(sc, particles=p)
     sim, o = make_pair(mpm, orc, sc)
     print(f"non-finite particles, method {method}", flush=True)
+    pos = p["pos"]
     for s in range(3):
+        # counted at binning: the positions the substep starts from (an infinite coordinate can be
+        # clamped to a wall by integrate and be finite afterwards)
+        expect = int((~np.isfinite(pos)).any(axis=1).sum())
         sim.step(method, 1)
         o.step(method, 1)
-        a = mpm.state_columns(sim.download()).astype(np.float32)
+        got = sim.download()
+        a = mpm.state_columns(got).astype(np.float32)
         b = mpm.state_columns(o.particles()).astype(np.float32)
         ok = _nan_equal(a, b)
         assert ok.all(), f"substep {s + 1}: {int((~ok).sum())} words differ, first {np.argwhere(~ok)[0]}"
-    assert sim.stats()["nonfinite_particles"] >= 3
+        assert sim.stats()["nonfinite_particles"] == expect, (s, expect)
+        pos = got["pos"]
     finite = np.isfinite(a).all(axis=1)
-    assert finite.sum() >= p.shape[0] - 5
+    assert finite.sum() >= p.shape[0] - 10
     sim.close()
     sim, o = make_pair(mpm, orc, sc)
     sim.debug_stage(method, 1)
