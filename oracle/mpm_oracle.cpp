@@ -269,7 +269,7 @@ template <class R, class AP> struct Sim {
         for (size_t p = 0; p < N; ++p) {
             int32_t c0[3]; cell_number(parts[p].pos, c0);
             for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
-                int32_t c[3] = {c0[0] + ox, c0[1] + oy, c0[2] + oz};
+                int32_t c[3] = {wadd(c0[0], ox), wadd(c0[1], oy), wadd(c0[2], oz)};
                 if (!in_range(c)) continue;
                 allocate_block(c[0] >> 2, c[1] >> 2, c[2] >> 2);
             }
@@ -325,7 +325,7 @@ template <class R, class AP> struct Sim {
         M3<R> negVPT = mscale(-V, PT);
         V3<R> pm = pt.mass * pt.vel;
         for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
-            int32_t c[3] = {c0[0] + ox, c0[1] + oy, c0[2] + oz};
+            int32_t c[3] = {wadd(c0[0], ox), wadd(c0[1], oy), wadd(c0[2], oz)};
             if (!in_range(c)) continue;
             uint32_t ci = cell_index(c);
             if (ci == EMPTY) continue;
@@ -394,7 +394,7 @@ template <class R, class AP> struct Sim {
         V3<R> nv = {0, 0, 0};
         M3<R> G = zero3<R>();
         for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
-            int32_t c[3] = {c0[0] + ox, c0[1] + oy, c0[2] + oz};
+            int32_t c[3] = {wadd(c0[0], ox), wadd(c0[1], oy), wadd(c0[2], oz)};
             V3<R> v;
             if (!node_velocity(c, v)) continue;
             R cw = (w[ox + 1].x * w[oy + 1].y) * w[oz + 1].z;
@@ -419,7 +419,7 @@ template <class R, class AP> struct Sim {
         V3<R> nv = {0, 0, 0};
         M3<R> B = zero3<R>();
         for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
-            int32_t c[3] = {c0[0] + ox, c0[1] + oy, c0[2] + oz};
+            int32_t c[3] = {wadd(c0[0], ox), wadd(c0[1], oy), wadd(c0[2], oz)};
             V3<R> v;
             if (!node_velocity(c, v)) continue;
             R cw = (w[ox + 1].x * w[oy + 1].y) * w[oz + 1].z;
@@ -455,7 +455,7 @@ template <class R, class AP> struct Sim {
         V3<R> w[3]; weights(f, w);
         V3<R> pcp = (V3<R>{(R)c0[0], (R)c0[1], (R)c0[2]} + f) - V3<R>{(R)0.5, (R)0.5, (R)0.5};
         for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
-            int32_t c[3] = {c0[0] + ox, c0[1] + oy, c0[2] + oz};
+            int32_t c[3] = {wadd(c0[0], ox), wadd(c0[1], oy), wadd(c0[2], oz)};
             if (!in_range(c)) continue;
             uint32_t ci = cell_index(c);
             if (ci == EMPTY) continue;
@@ -470,7 +470,7 @@ template <class R, class AP> struct Sim {
             dep(ci, gm.data(), AP::conv(wm, S));
         }
         for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
-            int32_t c[3] = {c0[0] + ox, c0[1] + oy, c0[2] + oz};
+            int32_t c[3] = {wadd(c0[0], ox), wadd(c0[1], oy), wadd(c0[2], oz)};
             if (!in_range(c)) continue;
             uint32_t ci = cell_index(c);
             if (ci == EMPTY) continue;

@@ -54,6 +54,10 @@ static inline uint32_t hash4(uint32_t px, uint32_t py, uint32_t pz, uint32_t pw)
     return h32 ^ (h32 >> 16);
 }
 
+// WGSL i32 addition wraps (two's complement); C++ signed overflow is undefined, so stencil cell
+// offsets (c + o, where c can be the saturated i32 of an infinite position) add through u32.
+static inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
 // ---------------------------------------------------------------- conversions
 static inline float bits_f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 static inline uint32_t f_bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }

@@ -210,6 +210,9 @@ MPM_DEV float comp(v3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
 
 // 27-bit set of neighbour-block offsets touched by the in-range 3x3x3 stencil of cell c
 // (bit (ox+1) + 3(oy+1) + 9(oz+1)); same block set mapAffectedBlocks.wgsl:75-84 allocates.
+// c + d wraps as WGSL's i32 addition does: the cell of an infinite position is the saturated
+// i32, and its wrapped neighbour is out of range (a signed add here would let the compiler
+// assume no overflow and test the two bounds inconsistently).
 MPM_DEV uint32_t touched_mask(const SimParams& P, const int c[3]) {
     uint32_t s[3];
 #pragma unroll
@@ -218,7 +221,7 @@ MPM_DEV uint32_t touched_mask(const SimParams& P, const int c[3]) {
         const int own = c[a] >> 2;
 #pragma unroll
         for (int d = -1; d <= 1; ++d) {
-            int cc = c[a] + d;
+            const int cc = (int)((uint32_t)c[a] + (uint32_t)d);
             if (cc >= 0 && cc < P.res[a]) m |= 1u << (((cc >> 2) - own) + 1);
         }
         s[a] = m;
