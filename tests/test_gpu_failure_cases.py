@@ -58,24 +58,24 @@ def test_nonfinite_particles_match_oracle(mpm, orc, method):
 
 
 def test_probe_exhaustion_is_counted_and_contained(mpm, orc):
-    """A block table of 61 slots for a scene touching ~150 blocks: inserts exhaust their probes
-    (max_probes 4). The failures are counted, no more than 61 blocks are allocated, every allocated
+    """A block table of 31 slots for a scene touching ~95 blocks: inserts exhaust their probes
+    (max_probes 4). The failures are counted, no more than 31 blocks are allocated, every allocated
     block is one the full table allocates too, particles whose own block is missing are counted
     unbinned (mpm_stats), and the state stays finite over substeps."""
     sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=4000, seed=42)
     sc = dict(sc, sim=dict(sc["sim"], grid_res=(32, 32, 32)))
-    sim, o = make_pair(mpm, orc, sc, hash_map_size=61, max_probes=4)
-    print("probe exhaustion: 61-slot table", flush=True)
+    sim, o = make_pair(mpm, orc, sc, hash_map_size=31, max_probes=4)
+    print("probe exhaustion: 31-slot table", flush=True)
     full, _ = make_pair(mpm, orc, sc, hash_map_size=65537)
     full.debug_stage(1, 0)
     full_blocks = set(map(tuple, full.dump_blocks().tolist()))
     full.close()
-    assert len(full_blocks) > 100
+    assert len(full_blocks) > 62
     sim.debug_stage(1, 0)
     st = sim.stats()
     blocks = set(map(tuple, sim.dump_blocks().tolist()))
     assert st["probe_failures"] > 0
-    assert 0 < len(blocks) <= 61 and blocks <= full_blocks
+    assert 0 < len(blocks) <= 31 and blocks <= full_blocks
     assert st["blocks_allocated"] == len(blocks)
     assert st["particles_binned"] + st["particles_unbinned"] == 4000 and st["particles_unbinned"] > 0
     # particle blocks (runs) whose own block is allocated hold exactly the binned particles
