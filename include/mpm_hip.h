@@ -190,6 +190,17 @@ int  mpm_load_glb(const void* glb, size_t len, mpm_scene_arrays* out, char* err,
 int  mpm_splat(mpm_ctx* ctx, int kind, const uint32_t* res, uint32_t* host_out, size_t cap);
 void* mpm_splat_grid(mpm_ctx* ctx);
 
+/* Allocation-free variant of mpm_splat (no counterpart in the reference). For callers that cannot
+ * allocate device memory: render loops that splat every frame, serving loops, code run under
+ * stream capture. The same splat kernels accumulate into a file-scope device array of
+ * MPM_SPLAT_NOALLOC_MAX_CELLS u32 cells (8 MiB, held on each GPU the library runs on until the
+ * process ends) instead of the context's grid, which mpm_splat grows with hipMalloc; the array is
+ * cleared by a kernel at every call and copied to host_out (required, cap >= cells). Grids of more
+ * cells return MPM_E_RANGE. mpm_splat_grid is not changed. Calls that overlap on one GPU (from
+ * several contexts or threads) share the array and race; the last call's values remain in it. */
+#define MPM_SPLAT_NOALLOC_MAX_CELLS (1u << 21)
+int  mpm_splat_noalloc(mpm_ctx* ctx, int kind, const uint32_t* res, uint32_t* host_out, size_t cap);
+
 /* addExplicitMpmDispatches / addPbmpmDispatches x n_substeps, then queue.submit
  * (GpuSnowPipelineRunner.svelte.ts:392-445, GpuMpmPipelineManager.ts:327-521). Enqueues on the
  * context's stream and returns; mpm_sync waits. */

@@ -136,6 +136,7 @@ _SIGS = [
     ("mpm_load_glb", ctypes.c_int, [_P, ctypes.c_size_t, ctypes.POINTER(MpmSceneArrays), ctypes.c_char_p, ctypes.c_size_t]),
     ("mpm_splat", ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),
     ("mpm_splat_grid", _P, [_P]),
+    ("mpm_splat_noalloc", ctypes.c_int, [_P, ctypes.c_int, _P, _P, ctypes.c_size_t]),
     ("mpm_ensemble_create", ctypes.c_int, [_P, _P, ctypes.c_int, ctypes.POINTER(_P)]),
     ("mpm_ensemble_upload", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t]),
     ("mpm_ensemble_download", ctypes.c_int, [_P, ctypes.c_int, _P, ctypes.c_size_t]),

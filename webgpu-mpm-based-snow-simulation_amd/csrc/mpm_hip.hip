@@ -15,6 +15,7 @@
 #include <vector>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include "mpm_kernels.h"
@@ -27,6 +28,37 @@
 #include "../../include/mpm_hip.h"
 
 using namespace mpm;
+
+// Accumulation grid of mpm_splat_noalloc: a file-scope device array, so that entry point calls no
+// allocator. Every GPU the library runs on holds its own copy until the process ends. In host
+// code `scratch` names a host-side stand-in, never the device memory: host code uses the
+// per-device address from hipGetSymbolAddress (scratch_ptr below) for kernel arguments and copies.
+static __device__ __align__(256) uint32_t scratch[MPM_SPLAT_NOALLOC_MAX_CELLS];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+hipError_t g_scratch_err[SCRATCH_MAX_DEVICES] = {};
+bool g_scratch_done[SCRATCH_MAX_DEVICES] = {};
+
+// Once per device, with that device current (context set-up): the device address of `scratch`.
+void scratch_resolve(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void* p = nullptr;
+    g_scratch_err[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (g_scratch_err[device] == hipSuccess) g_scratch_ptr[device] = (uint32_t*)p;
+    else (void)hipGetLastError();
+    g_scratch_done[device] = true;
+}
+uint32_t* scratch_ptr(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+}  // namespace
 
 struct mpm_ctx {
     mpm_config cfg;
@@ -708,6 +740,7 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { (void)hipGetLastError(); delete c; return MPM_E_HIP; }
+    scratch_resolve(c->device);
     c->n = cfg->n_particles;
     c->stride = (c->n + 63u) & ~63u;
     for (int a = 0; a < 3; ++a) c->nb[a] = (cfg->grid_res[a] + 3u) / 4u;
@@ -921,6 +954,43 @@ int mpm_splat(mpm_ctx* c, int kind, const uint32_t* res_in, uint32_t* host_out, 
     return MPM_OK;
 }
 void* mpm_splat_grid(mpm_ctx* c) { return c ? (void*)c->splat : nullptr; }
+
+// mpm_splat with the accumulation grid in the file-scope `scratch` array instead of the context's
+// lazily grown grid: no hipMalloc / hipFree, the grid cleared by a fill kernel (no memset node
+// for a capturing caller), copied to host_out. Calls that overlap on one GPU share the array.
+int mpm_splat_noalloc(mpm_ctx* c, int kind, const uint32_t* res_in, uint32_t* host_out, size_t cap) {
+    if (!c) return MPM_E_INVALID;
+    if (c->ens) return fail(c, MPM_E_INVALID, "batched context: splat a single-scene context");
+    if (kind != MPM_SPLAT_DENSITY && kind != MPM_SPLAT_MASS) return fail(c, MPM_E_INVALID, "unknown splat kind");
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
+    if (!host_out) return fail(c, MPM_E_INVALID, "the allocation-free splat needs host_out");
+    uint32_t res[3];
+    for (int a = 0; a < 3; ++a) res[a] = kind == MPM_SPLAT_MASS ? rdu<uint32_t>(c->uni, 208 + 4 * a) : (res_in ? res_in[a] : 0u);
+    const uint64_t cells = (uint64_t)res[0] * res[1] * res[2];
+    if (cells == 0) return fail(c, MPM_E_RANGE, "splat grid resolution must be positive");
+    if (cells > MPM_SPLAT_NOALLOC_MAX_CELLS)
+        return fail(c, MPM_E_RANGE, "allocation-free splat grid holds at most 2^21 cells (" + std::to_string(cells) + " asked); use mpm_splat");
+    if (cap < cells) return fail(c, MPM_E_RANGE, "splat output capacity too small");
+    HIPCHK(c, hipSetDevice(c->device));
+    uint32_t* grid = scratch_ptr(c->device);
+    if (!grid) {
+        const hipError_t e = (c->device >= 0 && c->device < SCRATCH_MAX_DEVICES) ? g_scratch_err[c->device] : hipErrorInvalidDevice;
+        return fail(c, MPM_E_HIP, std::string("no device address for the splat scratch array: ") + hipGetErrorString(e));
+    }
+    const float S = rdu<float>(c->uni, 4);
+    float g0[3], g1[3];
+    for (int a = 0; a < 3; ++a) { g0[a] = rdu<float>(c->uni, 16 + 4 * a); g1[a] = rdu<float>(c->uni, 32 + 4 * a); }
+    k_fill_u32<<<grid_for(cells, 4096), 256, 0, c->stream>>>(grid, 0u, (uint32_t)cells);
+    const uint32_t G = grid_for(c->n, 1u << 30);
+    if (kind == MPM_SPLAT_DENSITY)
+        k_splat<SPLAT_DENSITY><<<G, 256, 0, c->stream>>>(c->planes[c->cur], c->n, c->stride, g0[0], g0[1], g0[2], g1[0], g1[1], g1[2], S, res[0], res[1], res[2], grid);
+    else
+        k_splat<SPLAT_MASS><<<G, 256, 0, c->stream>>>(c->planes[c->cur], c->n, c->stride, g0[0], g0[1], g0[2], g1[0], g1[1], g1[2], S, res[0], res[1], res[2], grid);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(host_out, grid, cells * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
 
 int mpm_load_glb(const void* glb, size_t len, mpm_scene_arrays* a, char* err, size_t err_cap) {
     auto seterr = [&](const std::string& m) { if (err && err_cap) { std::strncpy(err, m.c_str(), err_cap - 1); err[err_cap - 1] = 0; } };
