@@ -5,11 +5,15 @@ solve + P2G + G2P, integrate with the box collider), inputs resident in HBM. Pri
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 
-roofline: the kernel with the largest device time per substep, measured with HIP events on
-the library's stream over a separate instrumented pass (eager launches, one event after every
-kernel); achieved = that kernel's compulsory particle-state bytes per launch / its average
-duration. cpu_baseline: the oracle (FP32-faithful CPU restatement of the reference substep,
-one thread) on two substeps of the same 1M-particle scene.
+roofline: the kernel with the largest device time per substep, measured with HIP events on the
+library's stream over a separate instrumented pass (eager launches, one event after every kernel).
+achieved = the reference formulation's algorithmic bytes for the work of one launch / its average
+duration: a PBMPM iteration re-streams {x, m, F_e, F_p, D} and writes {D, v}, 172 B per particle
+(the SURVEY's per-unit figure; 268 + 172 k B per particle-substep in all), so one k_pbmpm<MID>
+launch is 172 B x binned particles. The implemented layout's own compulsory bytes for the kernel
+are reported beside it. Fields read from committed rocprofv3 summaries carry their source file.
+cpu_baseline: the oracle (FP32-faithful CPU restatement of the reference substep) with OpenMP
+over the host's cores, 10 warm-up substeps, then the median of 5 timed substeps of the same 1M scene.
 """
 import argparse
 import importlib.util
@@ -26,20 +30,28 @@ ROOT = pathlib.Path(__file__).resolve().parent
 PKG_DIR = ROOT / "webgpu-mpm-based-snow-simulation_amd"
 
 HBM_PEAK_GBS = 8000.0
-# Compulsory particle-state bytes per particle for each transfer kernel of the implemented layout
-# (FP32 SoA planes; reads + writes; F_p lives per particle id and is touched only on a plastic
-# clamp, so the passes carry det F_p instead):
+# Reference formulation, bytes per particle for the work of one launch of each transfer kernel:
+#   PBMPM iteration: read x, m, F_e, F_p, D (124 B), write D, v (48 B) = 172 B (the solve, P2G and
+#   G2P of one iteration); FIRST also carries the substep's first read of the state and LAST its
+#   final write: per substep 268 + 172 k B (136 B read + 132 B written beyond the iterations).
+#   Explicit substep: 268 B (P2G reads x, v, m, F_e, F_p; G2P + integrate rewrite x, v, F_e, F_p, D, dx).
+REFERENCE_BYTES_PER_PARTICLE = {
+    "k_pbmpm<FIRST>": 172, "k_pbmpm<MID>": 172, "k_pbmpm<LAST>": 172,
+    "k_explicit_p2g": 136, "k_explicit_g2p": 132,
+}
+STEP_ALGO_BYTES = lambda k: 268 + 172 * k
+# Compulsory particle-state bytes per particle of the implemented layout (FP32 SoA planes; F_p lives
+# per particle id and is touched only on a plastic clamp, so the passes carry det F_p instead):
 #   k_pbmpm<FIRST>:    read perm, id, x, m, F_e, det F_p, D, dx (112 B); write id, x, m, F_e, det F_p, relax (64 B)
 #   k_pbmpm<MID>:      read x, m, F_e, relax (56 B)
 #   k_pbmpm<LAST>:     read x, m, F_e (52 B); write x, v, F_e, dx, D (108 B)
 #   k_explicit_p2g:    read perm, id, x, m, v, F_e, det F_p (76 B); write id, x, m, F_e, det F_p (60 B)
 #   k_explicit_g2p:    as k_pbmpm<LAST>
-KERNEL_BYTES_PER_PARTICLE = {
+OWN_LAYOUT_BYTES_PER_PARTICLE = {
     "k_pbmpm<FIRST>": 176, "k_pbmpm<MID>": 56, "k_pbmpm<LAST>": 160,
     "k_explicit_p2g": 136, "k_explicit_g2p": 160,
 }
-# Reference formulation (one substep of PBMPM with k iterations): 268 + 172 k bytes per particle.
-STEP_ALGO_BYTES = lambda k: 268 + 172 * k
+SIMDS, CLOCK_HZ = 1024, 2.4e9       # MI355X: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock
 
 
 def load_package():
@@ -61,8 +73,18 @@ def cpu_model():
     return platform.processor()
 
 
-def cpu_baseline(mpm, sc, sim, substeps=2):
-    """The oracle timed on the host: FP32-faithful restatement, single thread."""
+def host_threads():
+    """Threads for the CPU baseline: the host's cores, bounded by OMP_NUM_THREADS when set."""
+    n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
+def cpu_baseline(mpm, sc, sim, warmup=10, runs=5):
+    """The oracle on the host: the FP32-faithful restatement of the reference substep, OpenMP over
+    host_threads() threads, `warmup` substeps, then the median of `runs` single timed substeps."""
     sys.path.insert(0, str(ROOT))
     from oracle import oracle as orc
     p = sc["particles"]
@@ -70,13 +92,21 @@ def cpu_baseline(mpm, sc, sim, substeps=2):
     o = orc.OracleSim(p, sim.uniform_bytes(), pbmpm_iterations=kw["pbmpm_iterations"],
                       pbmpm_force_divisor=kw["pbmpm_force_divisor"], gravity=kw["gravity"])
     o.set_collider(sim.colliderManager.colliderData, sim.colliderManager.bvhNodes)
+    threads = o.set_threads(host_threads())
     t0 = time.perf_counter()
-    o.step(sc["method"], substeps)
-    dt = time.perf_counter() - t0
+    o.step(sc["method"], warmup)
+    t_warm = time.perf_counter() - t0
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        o.step(sc["method"], 1)
+        times.append(time.perf_counter() - t0)
     o.close()
-    return {"value": p.shape[0] * substeps / dt, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{substeps} substeps of the full {p.shape[0]}-particle scene ({dt:.1f} s), "
-                      f"oracle FP32 restatement on 1 thread of {cpu_model()} ({os.cpu_count()} logical CPUs)"}
+    med = float(np.median(times))
+    return {"value": p.shape[0] / med, "unit": "particle-substeps/s", "cores": threads, "kind": "port",
+            "sample": f"full {p.shape[0]}-particle scene: {warmup} warm-up substeps ({t_warm:.1f} s), then the median of "
+                      f"{runs} timed substeps ({', '.join(f'{t:.2f}' for t in times)} s); oracle FP32 restatement, "
+                      f"OpenMP {threads} threads on {cpu_model()} ({os.cpu_count()} logical CPUs)"}
 
 
 def learned_vs_plain(mpm, n, k_ref=16, substeps=600):
@@ -196,7 +226,7 @@ def terrain_bench(mpm, steps=100, warmup=10):
         sim.sync()
         kt = sim.kernel_times()
         st = sim.stats()
-    return {"workload": "explicit_slab_terrain", "particles": n, "grid": "256^3 hashed", "collider_triangles": len(sc["collider"][1]) // 3,
+    return {"workload": "explicit_slab_terrain", "particles": n, "grid": "256^3 sparse, direct block keys", "collider_triangles": len(sc["collider"][1]) // 3,
             "value": n * steps / dt, "unit": "particle-substeps/s", "ms_per_step": 1e3 * dt / steps,
             "blocks_allocated": st["blocks_allocated"],
             "kernel_ms_per_substep": {k: round(v[0] / 10, 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0])}}
@@ -214,47 +244,69 @@ def copy_calibration(mpm, nbytes):
     assert rc == 0 and out[-1] == 1.0
 
 
-def measured_valu(kernel):
-    """VALU wave-instructions per launch from the committed rocprofv3 --pmc summary, if one exists."""
-    f = ROOT / "profiles" / "valu.json"
+def profile_entry(name, kernel):
+    """(entry, source) of a kernel in a committed rocprofv3 summary under profiles/ (None if absent):
+    these numbers come from separate counter runs, not from this run."""
+    f = ROOT / "profiles" / name
     if not f.exists():
-        return None
-    try:
-        return json.loads(f.read_text()).get(kernel, {}).get("valu_wave_instructions_per_launch")
-    except (ValueError, OSError):
-        return None
-
-
-def measured_traffic(kernel):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if one exists."""
-    f = ROOT / "profiles" / "traffic.json"
-    if not f.exists():
-        return None
+        return None, None
     try:
         d = json.loads(f.read_text())
-        return d.get(kernel, {}).get("bytes_per_launch")
     except (ValueError, OSError):
-        return None
+        return None, None
+    return d.get(kernel), d.get("_source", f"profiles/{name}")
 
 
-def measured_atomics(kernel, seconds):
-    """Global / LDS atomic rates of one launch from the committed rocprofv3 --pmc summary."""
-    f = ROOT / "profiles" / "atomics.json"
-    if not f.exists():
-        return None
-    try:
-        d = json.loads(f.read_text())
-        k, pk = d.get(kernel), d.get("_peaks", {})
-        if not k or seconds <= 0:
-            return None
-        g, l = k["tcc_atomic_requests"] / seconds, k["lds_atomic_wave_instr"] / seconds
-        return {"global_requests_per_launch": k["tcc_atomic_requests"], "global_per_s": g,
-                "global_frac": g / pk["global_atomic_requests_per_s"],
-                "lds_wave_instr_per_launch": k["lds_atomic_wave_instr"], "lds_per_s": l,
-                "lds_frac": l / pk["lds_atomic_wave_instr_per_s"],
-                "peaks_assumed": pk}
-    except (ValueError, OSError, KeyError, ZeroDivisionError):
-        return None
+def profiled_fields(kernel, avg_s):
+    """Counter-derived figures of the dominant kernel, each with the committed summary it is read from."""
+    out = {}
+    tr, src = profile_entry("traffic.json", kernel)
+    if tr:
+        out["traffic_source"] = src
+        out["traffic_fetch_bytes"] = tr["fetch_bytes"]
+        out["traffic_write_bytes"] = tr["write_bytes"]
+    va, src = profile_entry("valu.json", kernel)
+    if va:
+        # VALU issue utilisation = wave-instructions x 4 cycles / (1,024 SIMDs x kernel cycles), both
+        # from the counter run; the same instruction count over this run's launch time beside it
+        out["valu"] = {"source": src, "wave_instructions_per_launch": va["valu_wave_instructions_per_launch"],
+                       "issue_utilisation_profiled": va.get("issue_utilisation"),
+                       "issue_utilisation_at_this_run_duration": va["valu_wave_instructions_per_launch"] * 4 / (SIMDS * CLOCK_HZ * avg_s)}
+    at, src = profile_entry("atomics.json", kernel)
+    if at:
+        pk = profile_entry("atomics.json", "_peaks")[0] or {}
+        g, l = at["tcc_atomic_requests"] / avg_s, at["lds_atomic_wave_instr"] / avg_s
+        out["atomics"] = {"source": src, "global_requests_per_launch": at["tcc_atomic_requests"], "global_per_s": g,
+                          "global_frac": g / pk["global_atomic_requests_per_s"] if pk else None,
+                          "lds_wave_instr_per_launch": at["lds_atomic_wave_instr"], "lds_per_s": l,
+                          "lds_frac": l / pk["lds_atomic_wave_instr_per_s"] if pk else None, "peaks_assumed": pk}
+    return out
+
+
+def hash_table_bench(mpm, steps=100, warmup=10):
+    """The same 1M-particle scene with the reference's open-addressing block table (hash_map_size
+    200,003 = sparseGridPrelude.wgsl:6, atomicCAS insert + compaction, mapAffectedBlocks.wgsl:7-62)
+    instead of direct block keys: particle-substeps/s and the table kernels' device time."""
+    sc = mpm.scenes.scene("pbmpm_two_spheres")
+    n = sc["particles"].shape[0]
+    with mpm.MpmSimulation(n, **dict(sc["sim"], hash_map_size=200003)) as sim:
+        sim.set_collider(*sc["collider"])
+        sim.upload(sc["particles"])
+        sim.step(1, warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(1, steps)
+        sim.sync()
+        dt = time.perf_counter() - t0
+        sim.set_kernel_timing(True)
+        sim.step(1, 10)
+        sim.sync()
+        kt = sim.kernel_times()
+        st = sim.stats()
+    return {"workload": "pbmpm_two_spheres, open-addressing block table", "particles": n, "hash_map_size": 200003,
+            "value": n * steps / dt, "unit": "particle-substeps/s", "ms_per_step": 1e3 * dt / steps,
+            "blocks_allocated": st["blocks_allocated"], "probe_failures": st["probe_failures"],
+            "kernel_ms_per_substep": {k: round(v[0] / 10, 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0])}}
 
 
 def main():
@@ -267,6 +319,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start comparison")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 64-scene workload")
     ap.add_argument("--no-terrain", action="store_true", help="skip the 4M-particle explicit terrain workload")
+    ap.add_argument("--no-hash-table", action="store_true", help="skip the open-addressing block table run")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
@@ -304,13 +357,17 @@ def main():
     kt = sim.kernel_times()
     sim.set_kernel_timing(False)
     per_substep = {name: ms / args.kernel_steps for name, (ms, cnt) in kt.items()}
-    dom = max((nm for nm in per_substep if nm in KERNEL_BYTES_PER_PARTICLE), key=lambda nm: per_substep[nm])
+    dom = max((nm for nm in per_substep if nm in REFERENCE_BYTES_PER_PARTICLE), key=lambda nm: per_substep[nm])
     dom_ms, dom_cnt = kt[dom]
     avg_s = dom_ms / dom_cnt * 1e-3
     stats = sim.stats()
-    algo_bytes = KERNEL_BYTES_PER_PARTICLE[dom] * stats["particles_binned"]
+    binned = stats["particles_binned"]
+    algo_bytes = REFERENCE_BYTES_PER_PARTICLE[dom] * binned
     achieved = algo_bytes / avg_s / 1e9
+    own_bytes = OWN_LAYOUT_BYTES_PER_PARTICLE[dom] * binned
     eager_ms = sum(per_substep.values())
+    prof = profiled_fields(dom, avg_s)
+    traffic = (prof["traffic_fetch_bytes"] + prof["traffic_write_bytes"]) if "traffic_fetch_bytes" in prof else None
 
     line = {
         "metric": "particle-substeps/s, 1M-particle PBMPM",
@@ -325,18 +382,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded two-sphere scene, 1 mass unit per particle)",
-        "config": {"workload": "pbmpm_two_spheres", "particles": n, "grid": "128^3 hashed (4^3 blocks)",
+        "config": {"workload": "pbmpm_two_spheres", "particles": n, "grid": "128^3 sparse, 4^3 blocks addressed directly by block key",
                    "pbmpm_iterations": k, "dt": sc["sim"]["dt"], "collider": "12-triangle box, BVH",
                    "parallelism": "single GPU"},
+        # priced on HBM bandwidth; the counters show the kernel's limiter is VALU issue (see "limiter")
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
-                     "kernel": dom, "avg_launch_us": avg_s * 1e6,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": dom, "avg_launch_us": avg_s * 1e6, "launches_timed": dom_cnt,
+                     "algorithmic_bytes_per_particle": REFERENCE_BYTES_PER_PARTICLE[dom], "particles_per_launch": binned,
                      "algorithmic_bytes_per_launch": algo_bytes,
-                     # the kernel is VALU-bound: issue utilisation = wave-instructions x 4 cycles /
-                     # (1,024 SIMDs x 2.4 GHz x launch time), instructions from the committed counters
-                     "valu_issue_utilisation": (measured_valu(dom) * 4 / (1024 * 2.4e9 * avg_s)) if measured_valu(dom) else None,
+                     "own_layout_bytes_per_launch": own_bytes, "own_layout_achieved": own_bytes / avg_s / 1e9,
+                     "own_layout_frac": own_bytes / avg_s / 1e9 / HBM_PEAK_GBS,
+                     "limiter": "valu" if "valu" in prof else None,
                      "step_frac_reference_bytes": (STEP_ALGO_BYTES(k) * value / 1e9) / HBM_PEAK_GBS,
-                     "atomics": measured_atomics(dom, avg_s)},
+                     **prof},
         "kernel_ms_per_substep": {nm: round(v, 5) for nm, v in sorted(per_substep.items(), key=lambda x: -x[1])},
         "eager_ms_per_substep": eager_ms,
     }
@@ -345,6 +404,9 @@ def main():
     if not args.no_terrain:
         print("bench: 4M explicit terrain workload", file=sys.stderr, flush=True)
         line["explicit_terrain"] = terrain_bench(mpm)
+    if not args.no_hash_table:
+        print("bench: open-addressing block table", file=sys.stderr, flush=True)
+        line["hash_table"] = hash_table_bench(mpm)
     if not args.no_ensemble:
         print("bench: batched 64-scene workload", file=sys.stderr, flush=True)
         line["ensemble"] = ensemble_bench(mpm)
@@ -352,7 +414,7 @@ def main():
         print("bench: learned warm start vs plain PBMPM", file=sys.stderr, flush=True)
         line["learned_vs_plain"] = learned_vs_plain(mpm, n)
     if not args.no_cpu_baseline:
-        print("bench: cpu baseline (oracle, 2 substeps)", file=sys.stderr, flush=True)
+        print("bench: cpu baseline (oracle, OpenMP)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(mpm, sc, sim)
     else:
         line["cpu_baseline"] = None

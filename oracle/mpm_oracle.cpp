@@ -26,6 +26,11 @@
 //   mpm/colliderPrelude.wgsl:37-102,153-356  closest point, ray/triangle, BVH traversal, response
 //   shaderPrelude/stressTensorOps.wgsl:9-22,39-56, plasticityOps.wgsl:7-48
 //   mpm/GpuMpmPipelineManager.ts:327-521     dispatch order of one substep
+// Threads (mode 0 only, orc_set_threads): the passes run over particles / blocks with OpenMP, the
+// grid sums as relaxed i32 atomic adds (integer sums are order-free, so every value equals the
+// serial run), the block table with a CAS claim of an EMPTY slot (RESERVED while the claimant
+// publishes, as mapAffectedBlocks.wgsl:7-62 does) and the binning cursors as atomic counters.
+// Block numbering and the order within a block then vary; no result depends on either.
 // Deliberate, documented departures (they change no value the reference defines):
 //   * particles whose own cell is outside the grid are not binned and take no part in
 //     P2G/G2P (the reference then re-processes a stale sortedParticleIndices entry);
@@ -83,6 +88,12 @@ typedef struct orc_stats {
 namespace {
 
 static const uint32_t EMPTY = 0xFFFFFFFFu;
+static const uint32_t RESERVED = 0xFFFFFFFEu;
+
+static inline uint32_t ld_acq(const uint32_t* p) { return __atomic_load_n(p, __ATOMIC_ACQUIRE); }
+static inline void st_rel(uint32_t* p, uint32_t v) { __atomic_store_n(p, v, __ATOMIC_RELEASE); }
+static inline uint32_t fetch_add_u32(uint32_t* p, uint32_t v) { return __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
+static inline void add_u64(uint64_t* p, uint64_t v) { __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
 static const int UNIFORM_BYTES = 37376;
 
 struct Uniforms {   // byte offsets: GpuUniformsBufferManager.ts:54-182
@@ -182,6 +193,8 @@ template <class R, class AP> struct Sim {
     std::vector<uint32_t> coll_data;
     std::vector<uint32_t> bvh;   // 8 words per node
     orc_stats st{};
+    int threads = 1;             // > 1: OpenMP passes with atomic grid sums (i32 grid only)
+    bool par() const { return threads > 1; }
 
     R dt() const { return (R)U.dt; }
     V3<R> gmin() const { return {(R)U.grid_min[0], (R)U.grid_min[1], (R)U.grid_min[2]}; }
@@ -239,7 +252,7 @@ template <class R, class AP> struct Sim {
     }
     uint32_t cell_index(const int32_t c[3]) {
         uint32_t bi = lookup_block(c[0] >> 2, c[1] >> 2, c[2] >> 2);
-        if (bi == EMPTY) { st.lookups_missed++; return EMPTY; }
+        if (bi == EMPTY) { if (par()) add_u64(&st.lookups_missed, 1); else st.lookups_missed++; return EMPTY; }
         uint32_t local = (uint32_t)((c[0] & 3) + (c[1] & 3) * 4 + (c[2] & 3) * 16);
         return bi * 64u + local;
     }
@@ -262,27 +275,59 @@ template <class R, class AP> struct Sim {
             cand = (cand + 1 >= P.hash_map_size) ? 0 : cand + 1;
         }
     }
+    // The threaded insert: claim an EMPTY slot by CAS to RESERVED, take a block number, publish it
+    // (a claim beyond max_blocks is undone to EMPTY, mapAffectedBlocks.wgsl:27-31); a RESERVED
+    // slot is waited on until its claimant publishes; an undone slot is probed again.
+    void allocate_block_par(int32_t bx, int32_t by, int32_t bz) {
+        uint32_t h = hash3((uint32_t)bx, (uint32_t)by, (uint32_t)bz);
+        uint32_t cand = h % P.hash_map_size;
+        for (uint32_t i = 0; i < P.max_probes;) {
+            uint32_t cur = ld_acq(&hm_index[cand]);
+            if (cur == EMPTY) {
+                uint32_t expect = EMPTY;
+                if (__atomic_compare_exchange_n(&hm_index[cand], &expect, RESERVED, false, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE)) {
+                    const uint32_t nb = fetch_add_u32(&n_alloc, 1u);
+                    if (nb >= P.max_blocks) { add_u64(&st.blocks_dropped, 1); st_rel(&hm_index[cand], EMPTY); return; }
+                    hm_bx[cand] = bx; hm_by[cand] = by; hm_bz[cand] = bz;
+                    mapped[nb] = cand;
+                    st_rel(&hm_index[cand], nb);
+                    return;
+                }
+                cur = expect;
+            }
+            while (cur == RESERVED) cur = ld_acq(&hm_index[cand]);
+            if (cur == EMPTY) continue;                       // undone claim: the same slot again
+            if (hm_bx[cand] == bx && hm_by[cand] == by && hm_bz[cand] == bz) return;
+            cand = (cand + 1 >= P.hash_map_size) ? 0 : cand + 1;
+            ++i;
+        }
+    }
     void bin() {
         std::fill(hm_index.begin(), hm_index.end(), EMPTY);
         n_alloc = 0;
         const size_t N = parts.size();
+        const bool pr = par();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(threads) if (pr)
         for (size_t p = 0; p < N; ++p) {
             int32_t c0[3]; cell_number(parts[p].pos, c0);
             for (int oz = -1; oz <= 1; ++oz) for (int oy = -1; oy <= 1; ++oy) for (int ox = -1; ox <= 1; ++ox) {
                 int32_t c[3] = {wadd(c0[0], ox), wadd(c0[1], oy), wadd(c0[2], oz)};
                 if (!in_range(c)) continue;
-                allocate_block(c[0] >> 2, c[1] >> 2, c[2] >> 2);
+                if (pr) allocate_block_par(c[0] >> 2, c[1] >> 2, c[2] >> 2);
+                else allocate_block(c[0] >> 2, c[1] >> 2, c[2] >> 2);
             }
         }
+        if (n_alloc > P.max_blocks) n_alloc = P.max_blocks;
         st.blocks_allocated = n_alloc;
         std::fill(counts.begin(), counts.end(), 0u);
         std::vector<uint32_t> own(N, EMPTY);
+#pragma omp parallel for schedule(static) num_threads(threads) if (pr)
         for (size_t p = 0; p < N; ++p) {
             int32_t c[3]; cell_number(parts[p].pos, c);
             // countParticlesPerBlock.wgsl:16-18: own block looked up with no range check
             uint32_t bi = lookup_block(c[0] >> 2, c[1] >> 2, c[2] >> 2);
             own[p] = bi;
-            if (bi != EMPTY) counts[bi]++;
+            if (bi != EMPTY) { if (pr) fetch_add_u32(&counts[bi], 1u); else counts[bi]++; }
         }
         uint32_t off = 0;
         for (uint32_t b = 0; b < n_alloc; ++b) { offsets[b] = off; off += counts[b]; }
@@ -290,7 +335,9 @@ template <class R, class AP> struct Sim {
         n_sorted = off;
         st.particles_unbinned = N - off;
         std::vector<uint32_t> cur(offsets.begin(), offsets.begin() + n_alloc);
-        for (size_t p = 0; p < N; ++p) if (own[p] != EMPTY) sorted[cur[own[p]]++] = (uint32_t)p;
+#pragma omp parallel for schedule(static) num_threads(threads) if (pr)
+        for (size_t p = 0; p < N; ++p)
+            if (own[p] != EMPTY) sorted[pr ? fetch_add_u32(&cur[own[p]], 1u) : cur[own[p]]++] = (uint32_t)p;
     }
     void clear_grid() {
         size_t cells = (size_t)n_alloc * 64;
@@ -299,7 +346,12 @@ template <class R, class AP> struct Sim {
         std::fill(gy.begin(), gy.begin() + cells, A(0));
         std::fill(gz.begin(), gz.begin() + cells, A(0));
     }
-    void dep(uint32_t ci, A* g, A v) { g[ci] = AP::add(g[ci], v); }
+    void dep(uint32_t ci, A* g, A v) {
+        if constexpr (std::is_same<A, int32_t>::value) {
+            if (par()) { fetch_add_u32((uint32_t*)&g[ci], (uint32_t)v); return; }   // wrapping, like the i32 atomics
+        }
+        g[ci] = AP::add(g[ci], v);
+    }
     A conv_acc(R x, float S) const { return AP::conv(x, S); }
 
     // ------------------------------------------------ particleToGrid.cs.wgsl:27-79
@@ -355,6 +407,7 @@ template <class R, class AP> struct Sim {
     void grid_update_explicit() {
         const R S = (R)U.S;
         const V3<R> g = {(R)P.gravity[0], (R)P.gravity[1], (R)P.gravity[2]};
+#pragma omp parallel for schedule(static) num_threads(threads) if (par())
         for (uint32_t b = 0; b < n_alloc; ++b) {
             uint32_t slot = mapped[b];
             for (uint32_t l = 0; l < 64; ++l) {
@@ -681,16 +734,23 @@ template <class R, class AP> struct Sim {
 
     // ------------------------------------------------ GpuMpmPipelineManager.ts:327-428
     // stage: 0 binning, 1 +P2G, 2 +grid update, >=3 full substep
+    void integrate_all() {
+        const size_t N = parts.size();
+#pragma omp parallel for schedule(dynamic, 1024) num_threads(threads) if (par())
+        for (size_t p = 0; p < N; ++p) integrate(parts[p]);
+    }
     void explicit_substep(int stage) {
         bin();
         if (stage < 1) return;
         clear_grid();
+#pragma omp parallel for schedule(dynamic, 512) num_threads(threads) if (par())
         for (uint32_t i = 0; i < n_sorted; ++i) p2g_explicit(parts[sorted[i]]);
         if (stage < 2) return;
         grid_update_explicit();
         if (stage < 3) return;
+#pragma omp parallel for schedule(dynamic, 512) num_threads(threads) if (par())
         for (uint32_t i = 0; i < n_sorted; ++i) g2p_explicit(parts[sorted[i]]);
-        for (auto& pt : parts) integrate(pt);
+        integrate_all();
     }
     // ------------------------------------------------ GpuMpmPipelineManager.ts:431-521
     // stage: 0 binning, s in 1..k: grid after the pbmpm dispatch of iteration s-1, > k full
@@ -700,15 +760,17 @@ template <class R, class AP> struct Sim {
         for (int it = 0; it < k; ++it) {
             if (stage < it + 1) return;
             clear_grid();
+#pragma omp parallel for schedule(dynamic, 512) num_threads(threads) if (par())
             for (uint32_t i = 0; i < n_sorted; ++i) {
                 Particle<R>& pt = parts[sorted[i]];
                 solve_constraints(pt);
                 p2g_pbmpm(pt);
             }
             if (stage < it + 2 && stage <= k) return;
+#pragma omp parallel for schedule(dynamic, 512) num_threads(threads) if (par())
             for (uint32_t i = 0; i < n_sorted; ++i) g2p_pbmpm(parts[sorted[i]]);
         }
-        for (auto& pt : parts) integrate(pt);
+        integrate_all();
     }
 };
 
@@ -931,7 +993,15 @@ void orc_pk1_explicit1(const orc_params* prm, const float* Fe9, const float* Fp9
 }
 // integrateParticles (integrateParticles.wgsl:12-72) on every particle, with no transfer before.
 void orc_integrate_only(void* hp) {
-    ((Handle*)hp)->visit([&](auto* s) { for (auto& pt : s->parts) s->integrate(pt); return 0; });
+    ((Handle*)hp)->visit([&](auto* s) { s->integrate_all(); return 0; });
+}
+// OpenMP threads of the passes (FP32-faithful mode only; the FP64 modes stay serial). Returns the
+// count in effect.
+int orc_set_threads(void* hp, int n) {
+    Handle* h = (Handle*)hp;
+    if (!h->s32) return 1;
+    h->s32->threads = n > 1 ? n : 1;
+    return h->s32->threads;
 }
 // Binning, then every allocated cell set to mass m and momentum m (a + M c) (c = the cell's
 // integer coordinates, M column-major), then the G2P of every binned particle (method 0

@@ -96,6 +96,7 @@ def lib():
             "orc_pk1_explicit1": (None, [ctypes.POINTER(OrcParams), _P, _P, _P]),
             "orc_integrate_only": (None, [_P]),
             "orc_g2p_affine": (None, [_P, ctypes.c_int, _P, _P, ctypes.c_float]),
+            "orc_set_threads": (ctypes.c_int, [_P, ctypes.c_int]),
             "orc_noise3": (ctypes.c_float, [ctypes.c_float] * 3),
             "orc_fbm": (ctypes.c_float, [ctypes.c_float] * 3),
             "orc_inside_mesh": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
@@ -184,6 +185,10 @@ class OracleSim:
         out = np.zeros((self.n, 37), np.float64)
         lib().orc_get_particles_f64(self.h, _ptr(out))
         return out
+
+    def set_threads(self, n):
+        """OpenMP threads of the passes (FP32-faithful mode only); returns the count in effect."""
+        return lib().orc_set_threads(self.h, int(n))
 
     def integrate_only(self):
         """integrateParticles on every particle with no transfer before it."""
