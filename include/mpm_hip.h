@@ -24,7 +24,7 @@ extern "C" {
 #define MPM_UNIFORMS_BYTES 37376   /* GpuUniformsBufferManager.ts:17 */
 #define MPM_PARTICLE_BYTES 192     /* shaderPrelude/mpmOps.wgsl:1-12, GpuMpmBufferManager.ts:24 */
 #define MPM_BVH_NODE_BYTES 32      /* mpm/colliderPrelude.wgsl:6-11 */
-#define MPM_KERNEL_COUNT 24        /* kernels of one substep, see mpm_kernel_name() */
+#define MPM_KERNEL_COUNT 25        /* kernels of one substep, see mpm_kernel_name() */
 
 enum {
     MPM_OK = 0,
@@ -76,7 +76,14 @@ typedef struct mpm_config {
     uint32_t grid_workgroups;      /* workgroups of the chunk kernels (0 = derived from CU count) */
     int32_t  device;               /* HIP device ordinal */
     uint32_t use_graph;            /* capture one substep in a hipGraph and replay it */
+    uint32_t plasticity_mode;      /* MPM_PLASTICITY_*: 0 = the reference's check of diag(R^T F_e) after the
+                                      4-step polar decomposition (plasticityOps.wgsl:7-48); 1 = 3x3 SVD
+                                      F_e = U S V^T, S clamped to [1 - theta_c, 1 + theta_s] (no reference
+                                      counterpart; Stomakhin et al. 2013); 2 = SVD with an FP64 SVD on the
+                                      GPU for near-degenerate F_e (sigma_min < 1e-2 sigma_max) */
 } mpm_config;
+
+enum { MPM_PLASTICITY_POLAR = 0, MPM_PLASTICITY_SVD = 1, MPM_PLASTICITY_SVD_FP64 = 2 };
 
 typedef struct mpm_ctx mpm_ctx;
 
@@ -95,6 +102,7 @@ typedef struct mpm_stats {
     uint64_t mlp_fallbacks;          /* particles whose warm-start prediction was dropped (last substep) */
     uint64_t nonfinite_particles;    /* particles whose position was NaN or infinite at binning (binned at
                                         cell 0 with zero grid contributions, as the reference's i32(NaN) = 0) */
+    uint64_t svd_fp64_particles;     /* MPM_PLASTICITY_SVD_FP64: particles decomposed in FP64 (last substep) */
 } mpm_stats;
 
 /* Device time of the last mpm_step / mpm_ensemble_step made with timing on (mpm_set_timing).

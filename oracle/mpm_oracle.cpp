@@ -45,6 +45,7 @@
 #include <numeric>
 #include <type_traits>
 #include "orc_math.h"
+#include "orc_svd.h"
 
 using namespace orc;
 
@@ -74,6 +75,7 @@ typedef struct orc_params {
     uint32_t hash_map_size;         // sparseGridPrelude.wgsl:6
     uint32_t max_blocks;            // sparseGridPrelude.wgsl:7
     uint32_t max_probes;            // sparseGridPrelude.wgsl:8 (100)
+    uint32_t plasticity_mode;       // 0 reference polar-diag check, 1 SVD clamp, 2 SVD + FP64 fallback (orc_svd.h)
 } orc_params;
 
 typedef struct orc_stats {
@@ -81,6 +83,7 @@ typedef struct orc_stats {
     uint64_t blocks_dropped;
     uint64_t particles_unbinned;
     uint64_t lookups_missed;
+    uint64_t svd_fp64;              // SVD plasticity decompositions taken in FP64 (fallback)
 } orc_stats;
 
 }  // extern "C"
@@ -558,6 +561,7 @@ template <class R, class AP> struct Sim {
 
     // ------------------------------------------------ plasticityOps.wgsl:7-48
     void plasticity(Particle<R>& pt) {
+        if (P.plasticity_mode != 0u) { plasticity_svd_option(pt); return; }
         M3<R> dE = pt.Fe, dP = pt.Fp;
         M3<R> rot = polar_rotation(dE);
         M3<R> sc = mmul(transpose(rot), dE);
@@ -572,6 +576,22 @@ template <class R, class AP> struct Sim {
         M3<R> F = mmul(dE, dP);
         pt.Fe = nFe;
         pt.Fp = mmul(nFeInv, F);
+    }
+
+    // The SVD plasticity option (orc_svd.h). The FP64 modes of the oracle decompose in double.
+    void plasticity_svd_option(Particle<R>& pt) {
+        const R lo = (R)(float)(1.0 - P.theta_c), hi = (R)(float)(1.0 + P.theta_s);
+        if constexpr (std::is_same<R, float>::value) {
+            bool fp64 = false;
+            plasticity_svd(pt.Fe, pt.Fp, lo, hi, P.plasticity_mode == 2u, &fp64);
+            if (fp64) { if (par()) add_u64(&st.svd_fp64, 1); else st.svd_fp64++; }
+        } else {
+            const Svd3<R> d = svd3(pt.Fe);
+            if (lo <= d.sig[0] && lo <= d.sig[1] && lo <= d.sig[2] && d.sig[0] <= hi && d.sig[1] <= hi && d.sig[2] <= hi) return;
+            const M3<R> nFe = svd_clamped(d, lo, hi);
+            pt.Fp = mmul(inverse3(nFe), mmul(pt.Fe, pt.Fp));
+            pt.Fe = nFe;
+        }
     }
 
     // ------------------------------------------------ colliderPrelude.wgsl:37-102
@@ -990,6 +1010,34 @@ void orc_pk1_explicit1(const orc_params* prm, const float* Fe9, const float* Fp9
     Particle<float> pt{};
     pt.Fe = ld3(Fe9); pt.Fp = ld3(Fp9);
     st3(s.pk1_explicit(pt), P9);
+}
+// The SVD of the plasticity option for one FP32 matrix (column-major): sig[3], V (9, column-major),
+// B = A V (9); use_fp64 decomposes the matrix in double (results rounded to float).
+void orc_svd3(const float* A9, int use_fp64, float* sig, float* V9, float* B9) {
+    if (use_fp64) {
+        M3<double> A;
+        for (int c = 0; c < 3; ++c) A.c[c] = {(double)A9[3 * c], (double)A9[3 * c + 1], (double)A9[3 * c + 2]};
+        const Svd3<double> d = svd3(A);
+        for (int c = 0; c < 3; ++c) {
+            sig[c] = (float)d.sig[c];
+            V9[3 * c] = (float)d.V.c[c].x; V9[3 * c + 1] = (float)d.V.c[c].y; V9[3 * c + 2] = (float)d.V.c[c].z;
+            B9[3 * c] = (float)d.B.c[c].x; B9[3 * c + 1] = (float)d.B.c[c].y; B9[3 * c + 2] = (float)d.B.c[c].z;
+        }
+        return;
+    }
+    const Svd3<float> d = svd3(ld3(A9));
+    for (int c = 0; c < 3; ++c) sig[c] = d.sig[c];
+    st3(d.V, V9); st3(d.B, B9);
+}
+// The SVD plasticity update of one particle (mode 1 or 2 of plasticity_mode): returns 1 when F_e
+// changed; *fp64 = whether the FP64 path was taken.
+int orc_plasticity_svd1(const float* Fe9, const float* Fp9, float lo, float hi, int fallback, float* Fe_out, float* Fp_out, int* fp64) {
+    M3<float> Fe = ld3(Fe9), Fp = ld3(Fp9);
+    bool f = false;
+    const bool ch = plasticity_svd(Fe, Fp, lo, hi, fallback != 0, &f);
+    st3(Fe, Fe_out); st3(Fp, Fp_out);
+    *fp64 = f ? 1 : 0;
+    return ch ? 1 : 0;
 }
 // integrateParticles (integrateParticles.wgsl:12-72) on every particle, with no transfer before.
 void orc_integrate_only(void* hp) {

@@ -40,11 +40,12 @@ class OrcParams(ctypes.Structure):
         ("hash_map_size", ctypes.c_uint32),
         ("max_blocks", ctypes.c_uint32),
         ("max_probes", ctypes.c_uint32),
+        ("plasticity_mode", ctypes.c_uint32),
     ]
 
 
 class OrcStats(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_uint64) for n in ("blocks_allocated", "blocks_dropped", "particles_unbinned", "lookups_missed")]
+    _fields_ = [(n, ctypes.c_uint64) for n in ("blocks_allocated", "blocks_dropped", "particles_unbinned", "lookups_missed", "svd_fp64")]
 
 
 _lib = None
@@ -97,6 +98,8 @@ def lib():
             "orc_integrate_only": (None, [_P]),
             "orc_g2p_affine": (None, [_P, ctypes.c_int, _P, _P, ctypes.c_float]),
             "orc_set_threads": (ctypes.c_int, [_P, ctypes.c_int]),
+            "orc_svd3": (None, [_P, ctypes.c_int, _P, _P, _P]),
+            "orc_plasticity_svd1": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_int, _P, _P, _P]),
             "orc_noise3": (ctypes.c_float, [ctypes.c_float] * 3),
             "orc_fbm": (ctypes.c_float, [ctypes.c_float] * 3),
             "orc_inside_mesh": (ctypes.c_int, [_P, _P, ctypes.c_uint32]),
@@ -244,6 +247,25 @@ def pk1_explicit(Fe, Fp, **param_kw):
     out = np.zeros(9, np.float32)
     lib().orc_pk1_explicit1(ctypes.byref(prm), _ptr(_colmajor(Fe)), _ptr(_colmajor(Fp)), _ptr(out))
     return _rowmajor(out)
+
+
+def svd3(A, fp64=False):
+    """The SVD plasticity option's 3x3 SVD of one matrix (numpy row-major): (sigma[3], V, B = A V),
+    V and B row-major with the singular vectors / sigma_j u_j as columns."""
+    sig = np.zeros(3, np.float32)
+    V = np.zeros(9, np.float32)
+    B = np.zeros(9, np.float32)
+    lib().orc_svd3(_ptr(_colmajor(A)), 1 if fp64 else 0, _ptr(sig), _ptr(V), _ptr(B))
+    return sig.astype(np.float64), _rowmajor(V), _rowmajor(B)
+
+
+def plasticity_svd(Fe, Fp, lo, hi, fallback=False):
+    """The option's plasticity update of one particle: (changed, F_e, F_p, took_fp64), row-major."""
+    fe, fp = np.zeros(9, np.float32), np.zeros(9, np.float32)
+    flag = ctypes.c_int(0)
+    ch = lib().orc_plasticity_svd1(_ptr(_colmajor(Fe)), _ptr(_colmajor(Fp)), float(lo), float(hi), 1 if fallback else 0,
+                                   _ptr(fe), _ptr(fp), ctypes.byref(flag))
+    return bool(ch), _rowmajor(fe), _rowmajor(fp), bool(flag.value)
 
 
 def f32_vec(fn, *arrays):

@@ -6,7 +6,7 @@ import numpy as np
 SHARED_KEYS = ("pbmpm_iterations", "pbmpm_force_divisor", "gravity", "youngs_modulus", "poissons_ratio",
                "hardening_explicit", "hardening_pbmpm", "pbmpm_blend", "pbmpm_volume_exponent", "theta_c",
                "theta_s", "density", "grad_scale", "collider_margin", "friction_ccd", "friction_proximity",
-               "collider_velocity_factor", "wall_mode", "wall_cells")
+               "collider_velocity_factor", "wall_mode", "wall_cells", "plasticity_mode")
 
 
 def make_pair(mpm, orc, sc, fp64=False, with_collider=True, **sim_over):

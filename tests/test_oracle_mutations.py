@@ -1,5 +1,5 @@
 """The oracle's pins reject plausible mistakes: each case below plants one slip in a copy of
-oracle/mpm_oracle.cpp (a swapped blend weight, a flipped exponent sign, a transposed product, a
+oracle/mpm_oracle.cpp or oracle/orc_svd.h (a swapped blend weight, a flipped exponent sign, a transposed product, a
 dropped factor ...), builds it with the oracle's own flags in a temporary directory, and runs the
 pin tests against that build. Every planted mistake must make at least one pin fail; the unmodified
 sources must pass. CPU only; the copies never leave the temporary directory."""
@@ -13,7 +13,7 @@ import pytest
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fno-finite-math-only", "-shared"]
-PINS = ["tests/test_oracle_pins.py", "tests/test_oracle.py"]
+PINS = ["tests/test_oracle_pins.py", "tests/test_oracle.py", "tests/test_oracle_svd.py"]
 
 # (what the slip is, text in mpm_oracle.cpp, replacement); reference lines in the pins' docstrings
 MUTATIONS = [
@@ -45,10 +45,24 @@ MUTATIONS = [
      "pt.D = mscale_r(mscale_r(mscale_r(B, dt()), (R)2), (R)P.grad_scale);"),
     ("PBMPM G2P B transposed", "B = madd(B, outer(cw * v, dist));", "B = madd(B, outer(dist, cw * v));"),
 ]
+# slips in the SVD plasticity option (oracle/orc_svd.h)
+SVD_MUTATIONS = [
+    ("Jacobi V update sign", "out.V.c[p] = c * vp - s * vq;", "out.V.c[p] = c * vp + s * vq;"),
+    ("Jacobi tau sign", "const R tau = (S[q][q] - S[p][p]) / ((R)2 * apq);", "const R tau = (S[p][p] - S[q][q]) / ((R)2 * apq);"),
+    ("Jacobi diagonal update sign", "const R spp = S[p][p] - t * apq, sqq = S[q][q] + t * apq;",
+     "const R spp = S[p][p] + t * apq, sqq = S[q][q] - t * apq;"),
+    ("clamp ratio dropped", "f[j] = t / d.sig[j];", "f[j] = t;"),
+    ("clamp with V transposed", "((f[0] * mat_el(d.V, k, 0)) * mat_el(d.B, row, 0) + (f[1] * mat_el(d.V, k, 1)) * mat_el(d.B, row, 1)) +\n                                  (f[2] * mat_el(d.V, k, 2)) * mat_el(d.B, row, 2)",
+     "((f[0] * mat_el(d.V, 0, k)) * mat_el(d.B, row, 0) + (f[1] * mat_el(d.V, 1, k)) * mat_el(d.B, row, 1)) +\n                                  (f[2] * mat_el(d.V, 2, k)) * mat_el(d.B, row, 2)"),
+    ("F_p update order", "const M3<float> nFp = mmul(inverse3(nFe), mmul(Fe, Fp));", "const M3<float> nFp = mmul(mmul(Fe, Fp), inverse3(nFe));"),
+    ("degenerate threshold 1e-2 -> 1e-4", "return mn < (R)1e-2 * mx;", "return mn < (R)1e-4 * mx;"),
+    ("one sweep instead of six", "for (int sweep = 0; sweep < 6; ++sweep) {", "for (int sweep = 0; sweep < 1; ++sweep) {"),
+]
 
 
-def _build(src_dir, text, out):
-    (src_dir / "mpm_oracle.cpp").write_text(text)
+def _build(src_dir, texts, out):
+    for name, text in texts.items():
+        (src_dir / name).write_text(text)
     r = subprocess.run(["g++"] + FLAGS + ["-o", str(out), str(src_dir / "mpm_oracle.cpp"), str(src_dir / "orc_scatter.cpp")],
                        capture_output=True, text=True)
     return r.returncode, r.stderr
@@ -62,26 +76,26 @@ def _pins_pass(lib):
 
 
 def test_pins_reject_planted_mistakes(tmp_path):
-    src = (ROOT / "oracle" / "mpm_oracle.cpp").read_text()
-    cases = [("unmodified", src)]
-    for name, a, b in MUTATIONS:
-        assert src.count(a) == 1, f"mutation site not unique or missing: {name}"
-        cases.append((name, src.replace(a, b)))
+    files = ("mpm_oracle.cpp", "orc_math.h", "orc_svd.h", "orc_scatter.cpp")
+    base = {f: (ROOT / "oracle" / f).read_text() for f in files}
+    cases = [("unmodified", dict(base))]
+    for target, muts in (("mpm_oracle.cpp", MUTATIONS), ("orc_svd.h", SVD_MUTATIONS)):
+        for name, a, b in muts:
+            assert base[target].count(a) == 1, f"mutation site not unique or missing: {name}"
+            cases.append((name, dict(base, **{target: base[target].replace(a, b)})))
 
     def run(i_case):
-        i, (name, text) = i_case
+        i, (name, texts) = i_case
         d = tmp_path / f"m{i}"
         d.mkdir()
-        for h in ("orc_math.h", "orc_scatter.cpp"):
-            (d / h).write_text((ROOT / "oracle" / h).read_text())
-        rc, err = _build(d, text, d / "liborc.so")
+        rc, err = _build(d, texts, d / "liborc.so")
         assert rc == 0, f"{name}: build failed {err[-300:]}"
         ok, tail = _pins_pass(d / "liborc.so")
         return name, ok, tail
 
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         results = list(ex.map(run, enumerate(cases)))
-    base = results[0]
-    assert base[1], f"unmodified oracle fails its pins: {base[2]}"
+    first = results[0]
+    assert first[1], f"unmodified oracle fails its pins: {first[2]}"
     survivors = [name for name, ok, _ in results[1:] if ok]
     assert not survivors, f"planted mistakes not caught by any pin: {survivors}"

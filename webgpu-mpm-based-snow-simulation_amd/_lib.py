@@ -13,7 +13,7 @@ LIB_PATH = PKG_DIR / "libmpm_hip.so"
 UNIFORMS_BYTES = 37376
 PARTICLE_BYTES = 192
 BVH_NODE_BYTES = 32
-KERNEL_COUNT = 24
+KERNEL_COUNT = 25
 MLP_NPARAM = 6288
 
 MPM_OK = 0
@@ -61,13 +61,14 @@ class MpmConfig(ctypes.Structure):
         ("grid_workgroups", ctypes.c_uint32),
         ("device", ctypes.c_int32),
         ("use_graph", ctypes.c_uint32),
+        ("plasticity_mode", ctypes.c_uint32),
     ]
 
 
 class MpmStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "n_particles", "particles_binned", "particles_unbinned", "particle_blocks", "work_chunks",
-        "blocks_allocated", "blocks_dropped", "probe_failures", "substeps", "mlp_fallbacks", "nonfinite_particles")]
+        "blocks_allocated", "blocks_dropped", "probe_failures", "substeps", "mlp_fallbacks", "nonfinite_particles", "svd_fp64_particles")]
 
 
 class MpmTimings(ctypes.Structure):

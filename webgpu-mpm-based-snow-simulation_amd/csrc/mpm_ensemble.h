@@ -80,7 +80,7 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
     if (!P0.direct_blocks)
         for (uint32_t s = gi; s < P0.cap; s += gsz) B.table[s] = EMPTY;
     for (uint32_t s = gi; s < E.S; s += gsz) E.nblk[s] = 0;
-    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; }
+    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; B.counters[C_SVD64] = 0; }
     __syncthreads();
     const uint32_t n = P0.n;
     const uint32_t nloop = (n + gsz - 1) / gsz;

@@ -164,10 +164,11 @@ const char* k_kernel_names[MPM_KERNEL_COUNT] = {
     "k_keys_count", "k_scan_tiles", "k_scan_top", "k_scan_apply", "k_scatter", "k_insert", "k_compact", "k_nbr",
     "k_pbmpm<FIRST>", "k_pbmpm<MID>", "k_pbmpm<LAST>", "k_explicit_p2g", "k_grid_update", "k_explicit_g2p", "k_straggle",
     "k_collide", "k_mlp_warmstart", "k_ens_keys_count", "k_ens_insert", "k_ens_compact", "k_ens_nbr", "k_ens_tasks",
-    "k_ens_transfer", "k_ens_straggle"};
+    "k_ens_transfer", "k_ens_straggle", "k_plastic_svd"};
 enum { KID_KEYS = 0, KID_SCAN_TILES, KID_SCAN_TOP, KID_SCAN_APPLY, KID_SCATTER, KID_INSERT, KID_COMPACT, KID_NBR,
        KID_PB_FIRST, KID_PB_MID, KID_PB_LAST, KID_EX_P2G, KID_EX_GRID, KID_EX_G2P, KID_STRAGGLE, KID_COLLIDE, KID_MLP,
-       KID_ENS_KEYS, KID_ENS_INSERT, KID_ENS_COMPACT, KID_ENS_NBR, KID_ENS_TASKS, KID_ENS_TRANSFER, KID_ENS_STRAGGLE };
+       KID_ENS_KEYS, KID_ENS_INSERT, KID_ENS_COMPACT, KID_ENS_NBR, KID_ENS_TASKS, KID_ENS_TRANSFER, KID_ENS_STRAGGLE,
+       KID_PLASTIC_SVD };
 
 // Records an event after a kernel launch when per-kernel timing is on (the first event of a
 // step is the start mark, id -1).
@@ -349,6 +350,7 @@ int params_from(mpm_ctx* c, const uint8_t* u, const mpm_config& g, int k_iter, f
     P.vol_exp = g.pbmpm_volume_exponent;
     P.plast_lo = (float)(1.0 - g.theta_c);
     P.plast_hi = (float)(1.0 + g.theta_s);
+    P.plast_mode = g.plasticity_mode;
     {
         // squared inner band of band_clear, rounded toward its inside
         const double blo = (double)P.plast_lo + BAND_MARGIN, bhi = (double)P.plast_hi - BAND_MARGIN;
@@ -521,6 +523,7 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
     } else {
         k_straggle<<<gs, 256, 0, s>>>(P, B); kt(c, KID_STRAGGLE);
     }
+    if (P.plast_mode != 0u) { k_plastic_svd<<<grid_for(c->n, 1u << 30), 256, 0, s>>>(P, B); kt(c, KID_PLASTIC_SVD); }
     if (full) hipEventRecord(c->ev[3], s);
     return true;
 }
@@ -733,6 +736,7 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     // chunk counts are formed as (count + chunk - 1) / chunk in u32: both stay far below 2^32
     if (cfg->n_particles > 0x7FFFFFFFu) return MPM_E_RANGE;
     if (cfg->chunk_particles && (cfg->chunk_particles < 256 || cfg->chunk_particles > 65536)) return MPM_E_RANGE;
+    if (cfg->plasticity_mode > 2) return MPM_E_INVALID;
     mpm_ctx* c = new mpm_ctx();
     c->cfg = *cfg;
     c->device = cfg->device;
@@ -1236,6 +1240,7 @@ int mpm_get_stats(mpm_ctx* c, mpm_stats* out) {
     out->substeps = c->substeps;
     out->mlp_fallbacks = h[C_FALLBACK];
     out->nonfinite_particles = h[C_NONFIN_LAST];
+    out->svd_fp64_particles = h[C_SVD64];
     return MPM_OK;
 }
 
@@ -1502,6 +1507,7 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
         if (g.pbmpm_iterations < 1 || g.pbmpm_iterations > 64) return MPM_E_INVALID;
         // the chunk list is sized for chunks of at least 256 particles (and the tuning's range)
         if (g.chunk_particles && (g.chunk_particles < 256 || g.chunk_particles > 65536)) return MPM_E_RANGE;
+        if (g.plasticity_mode != 0) return MPM_E_INVALID;   // the SVD pass is for single-scene contexts
         n += g.n_particles;
         K += (uint64_t)((g.grid_res[0] + 3u) / 4u) * ((g.grid_res[1] + 3u) / 4u) * ((g.grid_res[2] + 3u) / 4u);
         R = std::max<uint32_t>(R, methods[s] == MPM_METHOD_PBMPM ? (uint32_t)g.pbmpm_iterations + 1u : 2u);

@@ -30,6 +30,7 @@
 //                                                      unbinned particles in k_collide / k_straggle
 #pragma once
 #include "mpm_math.h"
+#include "mpm_svd.h"
 
 namespace mpm {
 
@@ -49,6 +50,7 @@ enum Counter : int {
     C_NVALID = 0, C_NRUNS, C_NCHUNKS, C_NBLOCKS, C_STRAG, C_DROPS, C_PROBEFAIL, C_SUBSTEPS, C_NCOLL, C_FALLBACK,
     C_NONFIN,        // particles with a non-finite position, counted by the keys pass of this substep
     C_NONFIN_LAST,   // the same count, latched by the scatter pass (which also clears C_NONFIN)
+    C_SVD64,         // SVD plasticity: particles whose F_e was decomposed in FP64 this substep
     C_NCOUNTERS = 16
 };
 
@@ -83,6 +85,7 @@ struct SimParams {
     uint32_t p2g_fold;         // PBMPM P2G may fold S into the mass (S = 2^k <= 2^16, quot_i32 verified
                                // for dt, no pointer force, gravity along y only with |grav_y / force_div|
                                // <= 2^40, 2^-20 <= dt <= 2^20)
+    uint32_t plast_mode;       // 0 reference polar-diag check, 1 SVD clamp, 2 SVD clamp + FP64 fallback
     uint32_t band_skip;        // plasticity check skipped for F_e well inside the band (band_clear)
     float band_lo2, band_hi2;  // (lo + BAND_MARGIN)^2 rounded up, (hi - BAND_MARGIN)^2 rounded down
     uint32_t ex_fold;          // explicit P2G may fold S into momentum, mass and dt (S = 2^k <= 2^16,
@@ -380,7 +383,7 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (!P.direct_blocks)
         for (uint32_t s = gi; s < P.cap; s += gsz) B.table[s] = EMPTY;
-    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; }
+    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; B.counters[C_SVD64] = 0; }
     const uint32_t n = P.n;
     const uint32_t nloop = (n + gsz - 1) / gsz;
     // key offset of the neighbour block this lane flags (lane j < 27: offset j, x fastest); a
@@ -678,7 +681,32 @@ MPM_DEV bool band_clear(const SimParams& P, const m3& F) {
     return c00 + r0 <= P.band_hi2 && c11 + r1 <= P.band_hi2 && c22 + r2 <= P.band_hi2 &&
            c00 - r0 >= P.band_lo2 && c11 - r1 >= P.band_lo2 && c22 - r2 >= P.band_lo2;
 }
+// The SVD plasticity option (mpm_svd.h) for the particle at row q of nxt: F_e = U S V^T with S
+// clamped to [lo, hi]; a near-degenerate F_e in FP64 when plast_mode is 2 (counted).
+MPM_DEV void plasticity_svd_row(const SimParams& P, const Bufs& B, uint32_t q, m3& Fe) {
+    const uint32_t id = B.pid_nxt[q];
+    const m3 Fp = ld_fp(B.fpid, id);
+    float fe[9], fp[9], nfe[9], nfp[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        fe[3 * c] = Fe.c[c].x; fe[3 * c + 1] = Fe.c[c].y; fe[3 * c + 2] = Fe.c[c].z;
+        fp[3 * c] = Fp.c[c].x; fp[3 * c + 1] = Fp.c[c].y; fp[3 * c + 2] = Fp.c[c].z;
+    }
+    bool fp64 = false;
+    const bool changed = plasticity_svd(fe, fp, P.plast_lo, P.plast_hi, P.plast_mode == 2u, nfe, nfp, &fp64);
+    if (fp64) atomicAdd(&B.counters[C_SVD64], 1u);
+    if (!changed) return;
+    m3 nFp;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Fe.c[c] = mk(nfe[3 * c], nfe[3 * c + 1], nfe[3 * c + 2]);
+        nFp.c[c] = mk(nfp[3 * c], nfp[3 * c + 1], nfp[3 * c + 2]);
+    }
+    st_fp(B.fpid, id, nFp);
+    st(B.nxt, B.stride, JP, q, det3(nFp));
+}
 MPM_DEV void plasticity_row(const SimParams& P, const Bufs& B, uint32_t q, m3& Fe) {
+    if (P.plast_mode != 0u) return;       // the SVD option runs as its own pass, k_plastic_svd
     if (P.band_skip && band_clear(P, Fe)) return;
     const m3 dE = Fe;
     const m3 rot = polar_rotation(dE);
@@ -698,6 +726,26 @@ MPM_DEV void plasticity_row(const SimParams& P, const Bufs& B, uint32_t q, m3& F
     const m3 nFp = mmul(nFeInv, F);
     st_fp(B.fpid, id, nFp);
     st(B.nxt, B.stride, JP, q, det3(nFp));
+}
+
+// The SVD plasticity option as a pass of its own after the substep's integrate (LAST or the
+// explicit G2P, the collider and the unbinned particles): there F_e = (I + D) F_e has been stored
+// unclamped, and nothing after plasticity in integrateParticles.wgsl:33-72 (walls, collider) reads
+// or writes F_e or F_p, so clamping afterwards gives the values clamping in place would. A pass of
+// its own keeps the SVD's registers out of the transfer kernels.
+__global__ __launch_bounds__(256) void k_plastic_svd(SimParams P, Bufs B) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= P.n) return;
+    m3 Fe = ldm(B.nxt, B.stride, FE, q);
+    if (P.band_skip && band_clear(P, Fe)) return;
+    const m3 before = Fe;
+    plasticity_svd_row(P, B, q, Fe);
+    bool same = true;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        same = same && as_u(Fe.c[c].x) == as_u(before.c[c].x) && as_u(Fe.c[c].y) == as_u(before.c[c].y) &&
+               as_u(Fe.c[c].z) == as_u(before.c[c].z);
+    if (!same) stm(B.nxt, B.stride, FE, q, Fe);
 }
 
 MPM_DEV v3 xform(const float* T, v3 p) {

@@ -28,7 +28,7 @@ const SYMBOLS = {
 const CFG = {
   n_particles: 0, grid_res: 4, grid_min: 16, grid_max: 28, fixed_point_scale: 40, dt: 44,
   max_blocks: 48, hash_map_size: 52, max_probes: 56, pbmpm_iterations: 60, pbmpm_force_divisor: 64,
-  gravity: 68, youngs_modulus: 80, poissons_ratio: 88, use_graph: 176, size: 184,
+  gravity: 68, youngs_modulus: 80, poissons_ratio: 88, use_graph: 176, plasticity_mode: 180, size: 184,
 } as const;
 
 export enum GpuSimulationMethodType { ExplicitMpm = 0, Pbmpm = 1 }
