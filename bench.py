@@ -309,6 +309,33 @@ def hash_table_bench(mpm, steps=100, warmup=10):
             "kernel_ms_per_substep": {k: round(v[0] / 10, 4) for k, v in sorted(kt.items(), key=lambda x: -x[1][0])}}
 
 
+def svd_plasticity_bench(mpm, steps=100, warmup=10):
+    """The same 1M-particle scene with the SVD plasticity option (mode 1 FP32, mode 2 with the FP64
+    fallback): particle-substeps/s and the SVD pass's device time per substep."""
+    out = {}
+    sc = mpm.scenes.scene("pbmpm_two_spheres")
+    n = sc["particles"].shape[0]
+    for mode in (1, 2):
+        with mpm.MpmSimulation(n, **dict(sc["sim"], plasticity_mode=mode)) as sim:
+            sim.set_collider(*sc["collider"])
+            sim.upload(sc["particles"])
+            sim.step(1, warmup)
+            sim.sync()
+            t0 = time.perf_counter()
+            sim.step(1, steps)
+            sim.sync()
+            dt = time.perf_counter() - t0
+            sim.set_kernel_timing(True)
+            sim.step(1, 10)
+            sim.sync()
+            kt = sim.kernel_times()
+            st = sim.stats()
+        out[f"mode_{mode}"] = {"value": n * steps / dt, "unit": "particle-substeps/s", "ms_per_step": 1e3 * dt / steps,
+                               "k_plastic_svd_ms_per_substep": round(kt.get("k_plastic_svd", (0.0, 1))[0] / 10, 4),
+                               "svd_fp64_particles_last_substep": st["svd_fp64_particles"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -320,6 +347,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 64-scene workload")
     ap.add_argument("--no-terrain", action="store_true", help="skip the 4M-particle explicit terrain workload")
     ap.add_argument("--no-hash-table", action="store_true", help="skip the open-addressing block table run")
+    ap.add_argument("--no-svd", action="store_true", help="skip the SVD plasticity option runs")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
@@ -407,6 +435,9 @@ def main():
     if not args.no_hash_table:
         print("bench: open-addressing block table", file=sys.stderr, flush=True)
         line["hash_table"] = hash_table_bench(mpm)
+    if not args.no_svd:
+        print("bench: SVD plasticity option", file=sys.stderr, flush=True)
+        line["svd_plasticity"] = svd_plasticity_bench(mpm)
     if not args.no_ensemble:
         print("bench: batched 64-scene workload", file=sys.stderr, flush=True)
         line["ensemble"] = ensemble_bench(mpm)
