@@ -134,3 +134,25 @@ def test_ensemble_workload_runs(mpm):
         for i in (0, 1, 63):
             assert np.isfinite(mpm.state_columns(ens.download(i))).all()
         assert ens.stats()["blocks_dropped"] == 0
+
+
+def test_batch_scenes_match_oracle_directly(mpm, orc, batch):
+    """Three scenes of the batch (explicit with sticky walls, PBMPM k = 3 and k = 5, one of them
+    partly below its grid) against the oracle run on that scene alone, bit for bit, in both
+    schedules: not only transitively through the single-scene path."""
+    from helpers import uniforms_blob, SHARED_KEYS, assert_particles_equal
+    scenes, _ = batch
+    for persistent in (False, True):
+        with mpm.MpmEnsemble([sc["sim"] for sc in scenes]) as ens:
+            ens.set_tuning(512, 2, persistent)
+            for i, sc in enumerate(scenes):
+                ens.upload(i, sc["particles"])
+            ens.step(4)
+            ens.sync()
+            for i in (0, 1, 4, 5):
+                kw = scenes[i]["sim"]
+                uni = uniforms_blob(np.float32(kw["dt"]), kw["grid_min"], kw["grid_max"], kw["grid_res"])
+                o = orc.OracleSim(scenes[i]["particles"], uni, hash_map_size=200003, max_blocks=70000,
+                                  **{k: v for k, v in kw.items() if k in SHARED_KEYS})
+                o.step(kw["method"], 4)
+                assert_particles_equal(mpm, ens.download(i), o.particles(), f"batch scene {i} (persistent={persistent})")

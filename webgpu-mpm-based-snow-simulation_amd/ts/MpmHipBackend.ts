@@ -1,8 +1,13 @@
 // Deno FFI shim: the simulation managers of the reference, backed by libmpm_hip.so.
 //
-// Drop-in for src/lib/gpu/mpm/GpuMpmBufferManager.ts, GpuMpmPipelineManager.ts and
-// src/lib/gpu/uniforms/GpuUniformsBufferManager.ts on the simulation path. The symbols and their
-// C signatures are declared in include/mpm_hip.h. Needs `deno run --allow-ffi`.
+// Drop-in for src/lib/gpu/mpm/GpuMpmBufferManager.ts, GpuMpmPipelineManager.ts,
+// src/lib/gpu/uniforms/GpuUniformsBufferManager.ts and src/lib/gpu/collider/GpuColliderBufferManager.ts
+// on the simulation path, with the same class and method names; MpmHipBackend holds the context and
+// the simulation half of GpuSnowPipelineRunner.svelte.ts (addSimulationStepsComputePass,
+// updateColliderTransformMat / Vel, scatterParticlesInMeshVolume, the GPU-time callback data).
+// WebGPU arguments (devices, encoders) have no counterpart: work is enqueued on the library's HIP
+// stream. The symbols and their C signatures are declared in include/mpm_hip.h.
+// Needs `deno run --allow-ffi`.
 
 const SYMBOLS = {
   mpm_config_defaults: { parameters: ["buffer"], result: "void" },
@@ -22,6 +27,7 @@ const SYMBOLS = {
   mpm_scatter_particles: { parameters: ["pointer", "buffer", "usize"], result: "i32" },
   mpm_splat: { parameters: ["pointer", "i32", "buffer", "buffer", "usize"], result: "i32" },
   mpm_load_glb: { parameters: ["buffer", "usize", "buffer", "buffer", "usize"], result: "i32" },
+  mpm_get_stats: { parameters: ["pointer", "buffer"], result: "i32" },
 } as const;
 
 // mpm_config layout (include/mpm_hip.h), 184 bytes, natural alignment.
@@ -31,16 +37,142 @@ const CFG = {
   gravity: 68, youngs_modulus: 80, poissons_ratio: 88, use_graph: 176, plasticity_mode: 180, size: 184,
 } as const;
 
-export enum GpuSimulationMethodType { ExplicitMpm = 0, Pbmpm = 1 }
+export enum GpuSimulationMethodType { ExplicitMpm = 0, Pbmpm = 1 }   // GpuSimulationMethod.ts:1-4
+
+type Lib = Deno.DynamicLibrary<typeof SYMBOLS>;
+type Vec3 = [number, number, number];
+
+function check(lib: Lib, rc: number, ctx: Deno.PointerValue | null) {
+  if (rc === 0) return;
+  let msg = "";
+  if (ctx) {
+    const p = lib.symbols.mpm_last_error(ctx);
+    if (p) msg = new Deno.UnsafePointerView(p).getCString();
+  }
+  throw new Error(`libmpm_hip error ${rc}: ${msg}`);
+}
+
+// GpuUniformsBufferManager.ts:54-182 — the same writers and byte offsets; every writer funnels
+// through mpm_write_uniforms (the library parses the blob at every step).
+export class GpuUniformsBufferManager {
+  constructor(private lib: Lib, private ctx: Deno.PointerValue) {}
+  private write(offset: number, data: ArrayBufferView) {
+    const bytes = new Uint8Array(data.buffer, data.byteOffset, data.byteLength);
+    check(this.lib, this.lib.symbols.mpm_write_uniforms(this.ctx, BigInt(offset), bytes, BigInt(bytes.byteLength)), this.ctx);
+  }
+  writeSimulationTimestepS(t: number) { this.write(0, new Float32Array([t])); }
+  writeFixedPointScale(s: number) { this.write(4, new Float32Array([s])); }
+  writeUsePbmpm(b: boolean) { this.write(8, new Uint32Array([b ? 1 : 0])); }
+  writeTime(t: number) { this.write(12, new Uint32Array([t >>> 0])); }
+  writeGridMinCoords(v: Vec3) { this.write(16, new Float32Array(v)); }
+  writeGridMaxCoords(v: Vec3) { this.write(32, new Float32Array(v)); }
+  writeViewProjMat(m: Float32Array) { this.write(48, m); }
+  writeViewProjInvMat(m: Float32Array) { this.write(112, m); }
+  writeMeshMinCoords(v: Vec3) { this.write(176, new Float32Array(v)); }
+  writeMeshMaxCoords(v: Vec3) { this.write(192, new Float32Array(v)); }
+  writeGridResolution(v: Vec3) { this.write(208, new Uint32Array(v)); }
+  writeColliderMinCoords(v: Vec3) { this.write(224, new Float32Array(v)); }
+  writeColliderMaxCoords(v: Vec3) { this.write(240, new Float32Array(v)); }
+  writeColliderTransformMat(m: Float32Array) { this.write(256, m); }
+  writeColliderVel(v: Vec3) { this.write(320, new Float32Array(v)); }
+  writeCameraPos(v: Vec3) { this.write(336, new Float32Array(v)); }
+  writeColliderNumIndices(n: number) { this.write(348, new Uint32Array([n])); }
+  writeGridCellDims(v: Vec3) { this.write(352, new Float32Array(v)); }
+  writeColliderNumObjects(n: number) { this.write(364, new Uint32Array([n])); }
+  writeLightViewProjMat(m: Float32Array) { this.write(368, m); }
+  writeColliderTransformInv(m: Float32Array) { this.write(432, m); }
+  writeInteractionPos(v: Vec3) { this.write(496, new Float32Array(v)); }
+  writeInteractionStrength(s: number) { this.write(508, new Float32Array([s])); }
+  writeInteractionRadius(r: number) { this.write(512, new Float32Array([r])); }
+  writeIsInteracting(b: boolean) { this.write(516, new Uint32Array([b ? 1 : 0])); }
+  writeInteractionMode(m: number) { this.write(520, new Uint32Array([m])); }
+  writeInteractionDir(v: Vec3) { this.write(528, new Float32Array(v)); }
+  writeColliderObjects(objects: { min: Vec3, max: Vec3, startIndex: number, countIndices: number }[]) {
+    const n = Math.min(objects.length, 1024);
+    const buf = new ArrayBuffer(32 * n);
+    const f = new Float32Array(buf), u = new Uint32Array(buf);
+    objects.slice(0, n).forEach((o, i) => {
+      f.set(o.min, 8 * i); u[8 * i + 3] = o.startIndex; f.set(o.max, 8 * i + 4); u[8 * i + 7] = o.countIndices;
+    });
+    if (n) this.write(544, u);
+  }
+}
+
+// GpuMpmBufferManager.ts:15-80: the particle buffer (192-byte ParticleData records) lives in the
+// library as SoA planes; these two calls are its writeBuffer / readback.
+export class GpuMpmBufferManager {
+  constructor(private lib: Lib, private ctx: Deno.PointerValue, readonly nParticles: number) {}
+  writeParticles(aos: ArrayBuffer) {
+    check(this.lib, this.lib.symbols.mpm_upload_particles(this.ctx, new Uint8Array(aos), BigInt(this.nParticles)), this.ctx);
+  }
+  readParticles(): ArrayBuffer {
+    const out = new ArrayBuffer(this.nParticles * 192);
+    check(this.lib, this.lib.symbols.mpm_download_particles(this.ctx, new Uint8Array(out), BigInt(this.nParticles)), this.ctx);
+    return out;
+  }
+}
+
+// GpuColliderBufferManager.ts:61-350: BVH built on the host by mpm_build_bvh (median split,
+// leaf <= 4, 32-byte nodes), indices reordered to leaf order, data packed as
+// [indices][xyz vertices][normals][uvs][material ids], uploaded with mpm_set_collider.
+export class GpuColliderBufferManager {
+  readonly colliderData: Uint32Array;
+  readonly bvhNodes: Uint32Array;
+  readonly numIndices: number;
+  readonly minCoords: Vec3;
+  readonly maxCoords: Vec3;
+  constructor(private lib: Lib, { positions, indices, normals = new Float32Array(), uvs = new Float32Array(), materialIndices = new Uint32Array() }: {
+    positions: Float32Array, indices: Uint32Array, normals?: Float32Array, uvs?: Float32Array, materialIndices?: Uint32Array,
+  }) {
+    const nTris = indices.length / 3;
+    const nodes = new Uint32Array(8 * Math.max(2 * nTris, 1));
+    const nNodes = new BigUint64Array(1);
+    const order = new Uint32Array(Math.max(nTris, 1));
+    check(lib, lib.symbols.mpm_build_bvh(new Uint8Array(positions.buffer), BigInt(positions.length / 3), new Uint8Array(indices.buffer),
+      BigInt(indices.length), new Uint8Array(nodes.buffer), BigInt(nodes.length / 8), new Uint8Array(nNodes.buffer),
+      new Uint8Array(order.buffer)), null);
+    this.bvhNodes = nodes.slice(0, 8 * Number(nNodes[0]));
+    const reordered = new Uint32Array(indices.length);
+    for (let t = 0; t < nTris; t++) reordered.set(indices.subarray(3 * order[t], 3 * order[t] + 3), 3 * t);
+    const parts = [reordered, new Uint32Array(positions.buffer, positions.byteOffset, positions.length),
+      new Uint32Array(normals.buffer, normals.byteOffset, normals.length), new Uint32Array(uvs.buffer, uvs.byteOffset, uvs.length),
+      materialIndices];
+    this.colliderData = new Uint32Array(parts.reduce((a, p) => a + p.length, 0));
+    let o = 0;
+    for (const p of parts) { this.colliderData.set(p, o); o += p.length; }
+    this.numIndices = indices.length;
+    const mn: Vec3 = [Infinity, Infinity, Infinity], mx: Vec3 = [-Infinity, -Infinity, -Infinity];
+    for (let i = 0; i < positions.length; i++) { mn[i % 3] = Math.min(mn[i % 3], positions[i]); mx[i % 3] = Math.max(mx[i % 3], positions[i]); }
+    this.minCoords = mn; this.maxCoords = mx;
+  }
+  upload(ctx: Deno.PointerValue) {
+    check(this.lib, this.lib.symbols.mpm_set_collider(ctx, new Uint8Array(this.colliderData.buffer), BigInt(this.colliderData.length),
+      this.numIndices, new Uint8Array(this.bvhNodes.buffer), BigInt(this.bvhNodes.length / 8)), ctx);
+  }
+}
+
+// GpuMpmPipelineManager.ts:327-521: one call enqueues one substep of the method's schedule on the
+// library's stream (the reference records its dispatches into the given compute pass).
+export class GpuMpmPipelineManager {
+  constructor(private lib: Lib, private ctx: Deno.PointerValue) {}
+  addExplicitMpmDispatches(_opts: { nParticles?: number, nBlocksInHashMap?: number, hashMapSize?: number } = {}) {
+    check(this.lib, this.lib.symbols.mpm_step(this.ctx, GpuSimulationMethodType.ExplicitMpm, 1), this.ctx);
+  }
+  addPbmpmDispatches(_opts: { nParticles?: number, nBlocksInHashMap?: number, hashMapSize?: number } = {}) {
+    check(this.lib, this.lib.symbols.mpm_step(this.ctx, GpuSimulationMethodType.Pbmpm, 1), this.ctx);
+  }
+}
 
 export class MpmHipBackend {
-  readonly lib: Deno.DynamicLibrary<typeof SYMBOLS>;
+  readonly lib: Lib;
   readonly ctx: Deno.PointerValue;
-  readonly nParticles: number;
+  readonly uniformsManager: GpuUniformsBufferManager;
+  readonly mpmManager: GpuMpmBufferManager;
+  readonly mpmPipelineManager: GpuMpmPipelineManager;
+  colliderManager: GpuColliderBufferManager | null = null;
 
   constructor(libPath: string, opts: {
-    nParticles: number, gridResolution: [number, number, number],
-    gridMinCoords: [number, number, number], gridMaxCoords: [number, number, number],
+    nParticles: number, gridResolution: Vec3, gridMinCoords: Vec3, gridMaxCoords: Vec3,
     simulationTimestepS: number, pbmpmIterations?: number,
   }) {
     this.lib = Deno.dlopen(libPath, SYMBOLS);
@@ -57,89 +189,66 @@ export class MpmHipBackend {
       dv.setFloat32(CFG.pbmpm_force_divisor, opts.pbmpmIterations, true);
     }
     const out = new BigUint64Array(1);
-    this.check(this.lib.symbols.mpm_create(new Uint8Array(cfg), new Uint8Array(out.buffer)), null);
+    check(this.lib, this.lib.symbols.mpm_create(new Uint8Array(cfg), new Uint8Array(out.buffer)), null);
     this.ctx = Deno.UnsafePointer.create(out[0]);
-    this.nParticles = opts.nParticles;
+    this.uniformsManager = new GpuUniformsBufferManager(this.lib, this.ctx);
+    this.mpmManager = new GpuMpmBufferManager(this.lib, this.ctx, opts.nParticles);
+    this.mpmPipelineManager = new GpuMpmPipelineManager(this.lib, this.ctx);
   }
 
-  private check(rc: number, ctx: Deno.PointerValue | null) {
-    if (rc === 0) return;
-    let msg = "";
-    if (ctx) {
-      const p = this.lib.symbols.mpm_last_error(ctx);
-      if (p) msg = new Deno.UnsafePointerView(p).getCString();
-    }
-    throw new Error(`libmpm_hip error ${rc}: ${msg}`);
+  // GpuSnowPipelineRunner.svelte.ts:162-177 (collider set-up) and :383-390
+  setCollider(mesh: { positions: Float32Array, indices: Uint32Array, normals?: Float32Array, uvs?: Float32Array, materialIndices?: Uint32Array }) {
+    const cm = new GpuColliderBufferManager(this.lib, mesh);
+    cm.upload(this.ctx);
+    const um = this.uniformsManager;
+    um.writeColliderMinCoords(cm.minCoords);
+    um.writeColliderMaxCoords(cm.maxCoords);
+    um.writeColliderNumObjects(1);
+    um.writeColliderObjects([{ min: cm.minCoords, max: cm.maxCoords, startIndex: 0, countIndices: cm.numIndices }]);
+    this.colliderManager = cm;
   }
-
-  // GpuUniformsBufferManager.ts:54-182 — same byte offsets; every writer funnels through here.
-  writeBuffer(offset: number, data: ArrayBufferView) {
-    const bytes = new Uint8Array(data.buffer, data.byteOffset, data.byteLength);
-    this.check(this.lib.symbols.mpm_write_uniforms(this.ctx, BigInt(offset), bytes, BigInt(bytes.byteLength)), this.ctx);
+  // column-major 4x4 (wgpu-matrix layout) and its inverse
+  updateColliderTransformMat(transformMat: Float32Array, transformInv: Float32Array) {
+    this.uniformsManager.writeColliderTransformMat(transformMat);
+    this.uniformsManager.writeColliderTransformInv(transformInv);
   }
-  writeSimulationTimestepS(t: number) { this.writeBuffer(0, new Float32Array([t])); }
-  writeFixedPointScale(s: number) { this.writeBuffer(4, new Float32Array([s])); }
-  writeUsePbmpm(b: boolean) { this.writeBuffer(8, new Uint32Array([b ? 1 : 0])); }
-  writeGridMinCoords(v: [number, number, number]) { this.writeBuffer(16, new Float32Array(v)); }
-  writeGridMaxCoords(v: [number, number, number]) { this.writeBuffer(32, new Float32Array(v)); }
-  writeGridResolution(v: [number, number, number]) { this.writeBuffer(208, new Uint32Array(v)); }
-  writeMeshMinCoords(v: [number, number, number]) { this.writeBuffer(176, new Float32Array(v)); }
-  writeMeshMaxCoords(v: [number, number, number]) { this.writeBuffer(192, new Float32Array(v)); }
-  writeColliderMinCoords(v: [number, number, number]) { this.writeBuffer(224, new Float32Array(v)); }
-  writeColliderMaxCoords(v: [number, number, number]) { this.writeBuffer(240, new Float32Array(v)); }
-  writeColliderTransformMat(m: Float32Array) { this.writeBuffer(256, m); }
-  writeColliderVel(v: [number, number, number]) { this.writeBuffer(320, new Float32Array(v)); }
-  writeColliderNumIndices(n: number) { this.writeBuffer(348, new Uint32Array([n])); }
-  writeGridCellDims(v: [number, number, number]) { this.writeBuffer(352, new Float32Array(v)); }
-  writeColliderNumObjects(n: number) { this.writeBuffer(364, new Uint32Array([n])); }
-  writeColliderTransformInv(m: Float32Array) { this.writeBuffer(432, m); }
-  writeInteractionPos(v: [number, number, number]) { this.writeBuffer(496, new Float32Array(v)); }
-  writeInteractionStrength(s: number) { this.writeBuffer(508, new Float32Array([s])); }
-  writeInteractionRadius(r: number) { this.writeBuffer(512, new Float32Array([r])); }
-  writeIsInteracting(b: boolean) { this.writeBuffer(516, new Uint32Array([b ? 1 : 0])); }
-  writeInteractionMode(m: number) { this.writeBuffer(520, new Uint32Array([m])); }
+  updateColliderVel(v: Vec3) { this.uniformsManager.writeColliderVel(v); }
 
-  // GpuColliderBufferManager: packed data (indices in BVH leaf order, then vertices) + BVH nodes.
-  setCollider(colliderData: Uint32Array, numIndices: number, bvhNodes: Uint32Array) {
-    this.check(this.lib.symbols.mpm_set_collider(this.ctx, new Uint8Array(colliderData.buffer), BigInt(colliderData.length),
-      numIndices, new Uint8Array(bvhNodes.buffer), BigInt(bvhNodes.length / 8)), this.ctx);
-  }
-
-  // scatterParticlesInMeshVolume (GpuSnowPipelineRunner.svelte.ts:368-381): the mesh is the
-  // GpuMeshBufferManager vertex list (consecutive triangles, 4 floats per vertex); its AABB must
-  // already be in the uniforms (writeMeshMinCoords / writeMeshMaxCoords).
+  // scatterParticlesInMeshVolume (:368-381): the GpuMeshBufferManager vertex list (consecutive
+  // triangles, 4 floats per vertex); its AABB must already be in the uniforms (writeMeshMin/MaxCoords).
   scatterParticlesInMeshVolume(meshXyzw: Float32Array) {
-    this.check(this.lib.symbols.mpm_scatter_particles(this.ctx, new Uint8Array(meshXyzw.buffer), BigInt(meshXyzw.length / 4)), this.ctx);
+    check(this.lib, this.lib.symbols.mpm_scatter_particles(this.ctx, new Uint8Array(meshXyzw.buffer), BigInt(meshXyzw.length / 4)), this.ctx);
+  }
+
+  // addSimulationStepsComputePass (:392-445): writeUsePbmpm, then nSimulationSteps substeps of the
+  // method in one call (enqueue-only, like queue.submit); the async form runs on a Deno worker thread.
+  addSimulationStepsComputePass({ nSimulationSteps, simulationMethodType }: { nSimulationSteps: number, simulationMethodType: GpuSimulationMethodType }) {
+    this.uniformsManager.writeUsePbmpm(simulationMethodType === GpuSimulationMethodType.Pbmpm);
+    check(this.lib, this.lib.symbols.mpm_step(this.ctx, simulationMethodType, nSimulationSteps), this.ctx);
+  }
+  async addSimulationStepsComputePassAsync({ nSimulationSteps, simulationMethodType }: { nSimulationSteps: number, simulationMethodType: GpuSimulationMethodType }) {
+    this.uniformsManager.writeUsePbmpm(simulationMethodType === GpuSimulationMethodType.Pbmpm);
+    check(this.lib, await this.lib.symbols.mpm_step_async(this.ctx, simulationMethodType, nSimulationSteps), this.ctx);
+  }
+  sync() { check(this.lib, this.lib.symbols.mpm_sync(this.ctx), this.ctx); }
+
+  // the timestamp query behind onGpuTimeUpdate (:588-611): computeSimulationStepNs of the last
+  // addSimulationStepsComputePass made with timing on
+  setGpuTiming(on: boolean) { check(this.lib, this.lib.symbols.mpm_set_timing(this.ctx, on ? 1 : 0), this.ctx); }
+  gpuTimes(): { computeSimulationStepNs: bigint } {
+    const t = new Float64Array(6);
+    check(this.lib, this.lib.symbols.mpm_get_timings(this.ctx, new Uint8Array(t.buffer)), this.ctx);
+    return { computeSimulationStepNs: BigInt(Math.round(t[0])) };
   }
 
   // marching-cubes density (kind 0, res = the density grid) / volumetric mass grid (kind 1)
-  splat(kind: 0 | 1, res: [number, number, number]): Uint32Array {
+  splat(kind: 0 | 1, res: Vec3): Uint32Array {
     const cells = res[0] * res[1] * res[2];
     const out = new Uint32Array(cells);
-    this.check(this.lib.symbols.mpm_splat(this.ctx, kind, new Uint8Array(new Uint32Array(res).buffer),
+    check(this.lib, this.lib.symbols.mpm_splat(this.ctx, kind, new Uint8Array(new Uint32Array(res).buffer),
       new Uint8Array(out.buffer), BigInt(cells)), this.ctx);
     return out;
   }
-
-  // particleDataBuffer (192-byte ParticleData records)
-  uploadParticles(aos: ArrayBuffer) {
-    this.check(this.lib.symbols.mpm_upload_particles(this.ctx, new Uint8Array(aos), BigInt(this.nParticles)), this.ctx);
-  }
-  downloadParticles(): ArrayBuffer {
-    const out = new ArrayBuffer(this.nParticles * 192);
-    this.check(this.lib.symbols.mpm_download_particles(this.ctx, new Uint8Array(out), BigInt(this.nParticles)), this.ctx);
-    return out;
-  }
-
-  // GpuMpmPipelineManager.addExplicitMpmDispatches / addPbmpmDispatches, batched like
-  // addSimulationStepsComputePass (GpuSnowPipelineRunner.svelte.ts:392-445).
-  addSimulationSteps(method: GpuSimulationMethodType, nSimulationSteps: number) {
-    this.check(this.lib.symbols.mpm_step(this.ctx, method, nSimulationSteps), this.ctx);
-  }
-  async addSimulationStepsAsync(method: GpuSimulationMethodType, nSimulationSteps: number) {
-    this.check(await this.lib.symbols.mpm_step_async(this.ctx, method, nSimulationSteps), this.ctx);
-  }
-  sync() { this.check(this.lib.symbols.mpm_sync(this.ctx), this.ctx); }
 
   destroy() { this.lib.symbols.mpm_destroy(this.ctx); this.lib.close(); }
 }
