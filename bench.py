@@ -109,18 +109,23 @@ def cpu_baseline(mpm, sc, sim, warmup=10, runs=5):
                       f"OpenMP {threads} threads on {cpu_model()} ({os.cpu_count()} logical CPUs)"}
 
 
-def learned_vs_plain(mpm, n, k_ref=16, substeps=600):
+def learned_vs_plain(mpm, n, k=16, k_ref=64, substeps=600):
     """The learned warm start against plain PBMPM on the bench scene.
 
-    Training pairs come from another sampling of the scene (seed 7): every 10th of `substeps`
-    substeps run with k_ref iterations, every particle logged. The network is trained on the GPU,
-    then both methods run the bench scene (seed 1) for `substeps` substeps with k_ref iterations
-    and per-iteration residual tracking, r_i = mean over substeps of
-    max_particles |dx_i - dx_{i-1} - g dt^2 / k|_2 / domain (gravity's per-iteration share
-    removed). Reported: the iterations each method needs to
-    reach the residual plain PBMPM has after k_ref / 2 iterations, and 1e-6; the final position
-    error after `substeps` substeps against the k_ref-iteration run, for plain and learned at a
-    few iteration counts (the scene breaks up chaotically, so this drifts for any k != k_ref)."""
+    PBMPM's k iterations relax the particle constraints towards a fixed point (the force
+    divisor splits the external forces over them), so few iterations leave the material too
+    soft; the reference for accuracy is the most-iterated run the library allows, k_ref = 64.
+    Training pairs come from another sampling of the scene (seed 7): a trajectory at k
+    iterations, every 10th substep logged with the k_ref-iteration result from the same start
+    state as the target. The network is trained on the GPU (Adam, 6,000 steps).
+
+    Measured on the bench scene (seed 1), `substeps` substeps each:
+      * the rms position error after the run against plain PBMPM at k_ref, for the warm start at
+        k and plain PBMPM at a sweep of iteration counts; iteration_ratio = (the fewest plain
+        iterations whose error is at most the warm start's) / k;
+      * the per-iteration residual r_i = mean over substeps of max_particles
+        |dx_i - dx_{i-1} - g dt^2 / k|_2 / domain along each method's own run at k, and the
+        iterations each needs to reach plain's r at k / 2."""
     def make(seed):
         sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=n, seed=seed)
         sim = mpm.MpmSimulation(n, **sc["sim"])
@@ -130,52 +135,71 @@ def learned_vs_plain(mpm, n, k_ref=16, substeps=600):
 
     t0 = time.perf_counter()
     tr = make(7)
-    pairs = tr.learn_collect(substeps, k_ref=k_ref, every=10)
+    pairs = tr.learn_collect(substeps, k_ref=k_ref, every=10, k_run=k)
     loss_a = tr.learn_train(4000, batch=65536, lr=1e-3, seed=1)
     loss_b = tr.learn_train(2000, batch=65536, lr=2e-4, seed=2)
     net = tr.learn_params()
     tr.close()
     t_train = time.perf_counter() - t0
 
-    def run(k, learned, track=False):
+    def run(kk, learned, track=False):
+        """Final positions, residuals (track) and the stepping time per substep: one substep
+        first (graph capture), then the rest timed between synchronisations."""
         sim = make(1)
-        sim.set_iterations(k)
+        sim.set_iterations(kk)
         if learned:
             sim.set_learn_params(*net)
             sim.learn_enable(True)
         if track:
             sim.set_residual_tracking(True)
-        sim.step(1, substeps)
-        pos = sim.download()["pos"].astype(np.float64)
-        res = sim.residuals()[0][:k].astype(np.float64) if track else None
+        sim.step(1, 1)
+        sim.sync()
+        t = time.perf_counter()
+        sim.step(1, substeps - 1)
+        sim.sync()
+        ms = (time.perf_counter() - t) / (substeps - 1) * 1e3
+        pos = sim.download()["pos"][:, :3].astype(np.float64)
+        res = sim.residuals()[0][:kk].astype(np.float64) if track else None
         st = sim.stats()
         sim.close()
-        return pos, res, st
+        return pos, res, st, ms
 
-    ref, r_plain, _ = run(k_ref, False, True)
-    _, r_learn, st = run(k_ref, True, True)
+    ref, _, _, ms_ref = run(k_ref, False)
+
+    def err(pos):
+        e = np.linalg.norm(pos - ref, axis=1)
+        return float(np.sqrt((e * e).mean()))
+
+    pos_l, _, st, ms_learn = run(k, True)
+    pos_p, _, _, ms_plain = run(k, False)
+    _, r_learn, _, _ = run(k, True, True)
+    _, r_plain, _, _ = run(k, False, True)
+    e_learn = err(pos_l)
+    sweep = {k: err(pos_p)}
+    for kk in (k_ref // 2, 3 * k_ref // 4, k_ref - 8, k_ref - 4, k_ref - 2, k_ref - 1):
+        if kk > k and kk not in sweep:
+            sweep[kk] = err(run(kk, False)[0])
+    match = [kk for kk in sorted(sweep) if sweep[kk] <= e_learn]
 
     def iters_to(r, thr):
         hit = [i for i in range(1, len(r)) if r[i] <= thr]
         return hit[0] if hit else None
 
-    thr = float(r_plain[k_ref // 2])
+    thr = float(r_plain[k // 2])
     it_p, it_l = iters_to(r_plain, thr), iters_to(r_learn, thr)
-    drift = {}
-    for k in (2, 4, 8):
-        for learned in (False, True):
-            pos, _, _ = run(k, learned)
-            e = np.linalg.norm(pos - ref, axis=1)
-            drift[f"{'learned' if learned else 'plain'}_k{k}"] = {"rms": float(np.sqrt((e * e).mean())), "max": float(e.max())}
     return {
-        "k_ref": k_ref, "substeps": substeps, "training_pairs": pairs,
+        "k": k, "k_ref": k_ref, "substeps": substeps, "training_pairs": pairs,
         "train_loss": [loss_a, loss_b], "train_and_collect_s": round(t_train, 2),
+        "rms_error_vs_k_ref": {"learned": e_learn, **{f"plain_k{kk}": v for kk, v in sorted(sweep.items())}},
+        "plain_iterations_to_match": match[0] if match else None,
+        "iteration_ratio": (match[0] / k) if match else k_ref / k,
+        "iteration_ratio_is_bound": not match,
+        "ms_per_substep": {"learned": round(ms_learn, 4), "plain": round(ms_plain, 4), "plain_k_ref": round(ms_ref, 4)},
         "residual_plain": [float(x) for x in r_plain[1:]], "residual_learned": [float(x) for x in r_learn[1:]],
-        "threshold": thr, "iters_plain": it_p, "iters_learned": it_l,
-        "iteration_ratio": (it_p / it_l) if (it_p and it_l) else None,
+        "residual_threshold": thr, "residual_iters_plain": it_p, "residual_iters_learned": it_l,
+        "residual_iteration_ratio": (it_p / it_l) if (it_p and it_l) else None,
         "iters_to_1e-6": {"plain": iters_to(r_plain, 1e-6), "learned": iters_to(r_learn, 1e-6)},
         "fallbacks_last_substep": st["mlp_fallbacks"],
-        "position_error_after_600_vs_k_ref": drift,
     }
 
 

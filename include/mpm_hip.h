@@ -242,21 +242,29 @@ int  mpm_divisor_fast(mpm_ctx* ctx, float d, float scale, int* fast);
 int  mpm_set_iterations(mpm_ctx* ctx, int k, float force_divisor);
 
 /* Learned PBMPM warm start (no reference counterpart: an addition to the reference's PBMPM
- * solver). A 15-64-64-9 GELU MLP with BF16 weights on the matrix cores maps a particle's state at
- * the start of a substep to the correction of D that k_ref iterations converge to.
- *  - mpm_learn_collect: n_substeps PBMPM substeps with k_ref iterations (force divisor k_ref),
+ * solver). A 16-64-64-12 GELU MLP with BF16 weights on the matrix cores maps a particle's state at
+ * the start of a substep (singular values of F_e, det F_p, D, and dprev: the previous substep's last
+ * per-iteration change of dx) to the change of D and of dx that k_ref iterations make.
+ *  - mpm_learn_collect: n_substeps PBMPM substeps with k_run iterations (force divisor k_run),
  *    logging (features, target) pairs of every particle every `every` substeps, up to max_pairs.
+ *    Features come from that trajectory; targets from a k_ref-iteration substep (force divisor
+ *    k_ref) run from the same start state, which the trajectory then discards (k_ref > k_run
+ *    teaches the network where more iterations than the run uses would take the substep).
  *  - mpm_learn_train: Adam on minibatches of `batch` pairs (multiple of 256) drawn with replacement;
  *    mean_loss = mean squared error of the standardised targets over the last 50 steps.
  *  - mpm_learn_get/set_params: 6,288 FP32 parameters (W1[16][64] b1 W2[64][64] b2 W3[64][16] b3)
  *    and 64 standardisation values (feature mean, std, target mean, std; 16 each).
  *  - mpm_learn_enable: apply the warm start before every PBMPM substep; a particle whose
- *    standardised prediction exceeds `limit` (or is not finite) keeps the plain D (fallback). */
-int  mpm_learn_collect(mpm_ctx* ctx, int n_substeps, int k_ref, int every, uint64_t max_pairs, uint64_t* n_pairs);
+ *    standardised prediction exceeds `limit` (or is not finite), or that no iteration reached in the
+ *    previous substep, keeps the plain D and dx (fallback). */
+int  mpm_learn_collect(mpm_ctx* ctx, int n_substeps, int k_run, int k_ref, int every, uint64_t max_pairs, uint64_t* n_pairs);
 int  mpm_learn_train(mpm_ctx* ctx, int steps, int batch, float lr, uint32_t seed, float* mean_loss);
 int  mpm_learn_get_params(mpm_ctx* ctx, float* params, float* norm);
 int  mpm_learn_set_params(mpm_ctx* ctx, const float* params, const float* norm);
 int  mpm_learn_enable(mpm_ctx* ctx, int on, float limit);
+/* The warm start's input dprev by particle id (3 floats each, n = n_particles): set from in_xyz and/or
+ * read into out_xyz (either may be NULL). Enabling the warm start clears it; LAST writes it. */
+int  mpm_learn_dprev(mpm_ctx* ctx, const float* in_xyz, float* out_xyz, size_t n);
 /* Gradient of the MSE loss over n samples (n multiple of 16) computed by the MFMA forward and
  * backward kernels, for checking against a host computation (test hook). */
 int  mpm_mlp_grad(const float* params, const float* norm, const float* feats, const float* targs, uint32_t n,

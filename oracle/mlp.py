@@ -4,11 +4,13 @@ The warm-start network has no counterpart in the reference (it is an addition on
 PBMPM solver of src/lib/gpu/mpm/pbmpm.wgsl), so this module restates the network from its
 definition, in numpy:
 
-  x  = standardise(features)            features = [diag(R^T Fe) (3), det Fp, D (9, column-major), |v|, m]
-  z1 = x  W1 + b1,  h1 = gelu(z1)       W1 [16][64]  (feature 15 is padding, always 0)
+  x  = standardise(features)            features = [singular values of Fe, largest first (3), det Fp,
+                                                    D (9, column-major), dprev (3)]
+  z1 = x  W1 + b1,  h1 = gelu(z1)       W1 [16][64]
   z2 = h1 W2 + b2,  h2 = gelu(z2)       W2 [64][64]
-  y  = h2 W3 + b3                        W3 [64][16]  (outputs 9..15 are padding)
-  loss = sum over samples and outputs 0..8 of (y - standardise(target))^2
+  y  = h2 W3 + b3                        W3 [64][16]  (outputs 12..15 are padding)
+  loss = sum over samples and outputs 0..11 of (y - standardise(target))^2
+  targets = [D change (9, column-major), dx change (3)]
 
 gelu is the tanh form 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))).
 
@@ -22,7 +24,7 @@ gelu / gelu' and of the bfloat16 rounding (tests/test_mlp_oracle.py).
 """
 import numpy as np
 
-NIN, NOUT, H = 15, 9, 64
+NIN, NOUT, H = 16, 12, 64
 W1, B1 = 0, 16 * 64
 W2 = B1 + 64
 B2 = W2 + 64 * 64
@@ -125,15 +127,12 @@ def loss64(params, norm, feats, targs):
     return float(np.sum(e * e) / x.shape[0])
 
 
-def features(fe, fp, d, vel, mass):
-    """Network input of one particle (column-major 3x3 matrices as numpy [3][3] row = row)."""
-    u, s, vt = np.linalg.svd(np.asarray(fe, np.float64))
-    r = u @ vt
-    sym = r.T @ np.asarray(fe, np.float64)
+def features(fe, fp, d, dprev):
+    """Network input of one particle (3x3 matrices as numpy [row][col]; stored column-major)."""
+    s = np.linalg.svd(np.asarray(fe, np.float64), compute_uv=False)      # largest first
     f = np.zeros(16)
-    f[0:3] = np.diag(sym)
+    f[0:3] = s
     f[3] = np.linalg.det(np.asarray(fp, np.float64))
     f[4:13] = np.asarray(d, np.float64).T.ravel()     # column-major
-    f[13] = np.linalg.norm(vel)
-    f[14] = mass
+    f[13:16] = dprev
     return f

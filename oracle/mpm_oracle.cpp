@@ -774,6 +774,7 @@ template <class R, class AP> struct Sim {
     }
     // ------------------------------------------------ GpuMpmPipelineManager.ts:431-521
     // stage: 0 binning, s in 1..k: grid after the pbmpm dispatch of iteration s-1, > k full
+    std::vector<float>* trace = nullptr;   // when set: dx of every particle (by id) after each G2P
     void pbmpm_substep(int stage) {
         bin();
         const int k = P.pbmpm_iterations;
@@ -789,6 +790,14 @@ template <class R, class AP> struct Sim {
             if (stage < it + 2 && stage <= k) return;
 #pragma omp parallel for schedule(dynamic, 512) num_threads(threads) if (par())
             for (uint32_t i = 0; i < n_sorted; ++i) g2p_pbmpm(parts[sorted[i]]);
+            if (trace) {
+                const size_t N = parts.size();
+                for (size_t p = 0; p < N; ++p) {
+                    float* o = trace->data() + ((size_t)it * N + p) * 12;
+                    o[0] = (float)parts[p].dx.x; o[1] = (float)parts[p].dx.y; o[2] = (float)parts[p].dx.z;
+                    for (int c = 0; c < 3; ++c) { o[3 + 3 * c] = (float)parts[p].D.c[c].x; o[4 + 3 * c] = (float)parts[p].D.c[c].y; o[5 + 3 * c] = (float)parts[p].D.c[c].z; }
+                }
+            }
         }
         integrate_all();
     }
@@ -1042,6 +1051,19 @@ int orc_plasticity_svd1(const float* Fe9, const float* Fp9, float lo, float hi, 
 // integrateParticles (integrateParticles.wgsl:12-72) on every particle, with no transfer before.
 void orc_integrate_only(void* hp) {
     ((Handle*)hp)->visit([&](auto* s) { s->integrate_all(); return 0; });
+}
+// One full PBMPM substep recording every particle's dx and D (by particle id) after the G2P of each
+// of the P.pbmpm_iterations iterations: out[it][n][12] = dx (3), D (9, column-major); unbinned
+// particles keep theirs.
+void orc_pbmpm_trace(void* hp, float* out) {
+    ((Handle*)hp)->visit([&](auto* s) {
+        std::vector<float> tr((size_t)s->P.pbmpm_iterations * s->parts.size() * 12, 0.0f);
+        s->trace = &tr;
+        s->pbmpm_substep(1 << 30);
+        s->trace = nullptr;
+        std::memcpy(out, tr.data(), tr.size() * sizeof(float));
+        return 0;
+    });
 }
 // OpenMP threads of the passes (FP32-faithful mode only; the FP64 modes stay serial). Returns the
 // count in effect.

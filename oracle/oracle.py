@@ -98,6 +98,7 @@ def lib():
             "orc_integrate_only": (None, [_P]),
             "orc_g2p_affine": (None, [_P, ctypes.c_int, _P, _P, ctypes.c_float]),
             "orc_set_threads": (ctypes.c_int, [_P, ctypes.c_int]),
+            "orc_pbmpm_trace": (None, [_P, _P]),
             "orc_svd3": (None, [_P, ctypes.c_int, _P, _P, _P]),
             "orc_plasticity_svd1": (ctypes.c_int, [_P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_int, _P, _P, _P]),
             "orc_noise3": (ctypes.c_float, [ctypes.c_float] * 3),
@@ -192,6 +193,13 @@ class OracleSim:
     def set_threads(self, n):
         """OpenMP threads of the passes (FP32-faithful mode only); returns the count in effect."""
         return lib().orc_set_threads(self.h, int(n))
+
+    def pbmpm_trace(self, k):
+        """One PBMPM substep (k iterations, as created); dx and D (column-major) of every particle after
+        each G2P: [k, n, 12]."""
+        out = np.zeros((k, self.n, 12), np.float32)
+        lib().orc_pbmpm_trace(self.h, _ptr(out))
+        return out
 
     def integrate_only(self):
         """integrateParticles on every particle with no transfer before it."""

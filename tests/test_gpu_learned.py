@@ -12,12 +12,13 @@ pytestmark = pytest.mark.gpu
 def _net(seed, out_scale=1.0):
     p = mlp.init_params(seed, out_scale=out_scale)
     norm = np.zeros((4, 16), np.float32)
-    norm[0, :15] = np.linspace(-0.5, 0.5, 15)
-    norm[1, :15] = np.linspace(0.5, 2.0, 15)
-    norm[1, 15] = 1.0
-    norm[2, :9] = np.linspace(-1e-3, 1e-3, 9)
-    norm[3, :9] = np.linspace(1e-3, 4e-3, 9)
-    norm[3, 9:] = 1.0
+    norm[0, :16] = np.linspace(-0.5, 0.5, 16)
+    norm[1, :16] = np.linspace(0.5, 2.0, 16)
+    norm[0, 13:16] = 0.0                     # dprev: centred, ~1e-5 scale
+    norm[1, 13:16] = 2e-5
+    norm[2, :12] = np.linspace(-1e-3, 1e-3, 12)
+    norm[3, :12] = np.linspace(1e-3, 4e-3, 12)
+    norm[3, 12:] = 1.0
     return p, norm.ravel()
 
 
@@ -31,9 +32,8 @@ def test_mlp_gradient_matches_oracle(mpm):
     p, norm = _net(3)
     nm = norm.reshape(4, 16)
     feats = (nm[0] + nm[1] * rng.standard_normal((n, 16))).astype(np.float32)
-    feats[:, 15] = 0
     targs = (nm[2] + nm[3] * rng.standard_normal((n, 16))).astype(np.float32)
-    targs[:, 9:] = 0
+    targs[:, 12:] = 0
     g, y, loss = mpm.mlp_grad(p, norm, feats, targs)
     go, yo, lo = mlp.forward_backward(p, norm, feats, targs, emulate=True)
     assert np.abs(y - yo).max() <= 0.02 * np.abs(yo).max()
@@ -43,7 +43,6 @@ def test_mlp_gradient_matches_oracle(mpm):
         ga, gb = g[a:b], go[a:b]
         assert np.linalg.norm(ga - gb) <= 0.02 * np.linalg.norm(gb) + 1e-12, name
         assert np.abs(ga - gb).max() <= 0.05 * np.abs(gb).max() + 1e-12, name
-    assert np.all(g[mlp.W1 + 15 * 64:mlp.B1] == 0)
     assert np.all(g[mlp.W3:mlp.B3].reshape(64, 16)[:, mlp.NOUT:] == 0)
 
 
@@ -66,29 +65,41 @@ def _records(mpm, n, rng):
 
 
 def test_warmstart_inference_matches_oracle(mpm):
-    """One warm start over 4,000 particles: D + destandardised prediction per particle, against
-    the oracle network on features from an exact polar decomposition (SVD)."""
+    """One warm start over 4,000 particles: D and dx plus the destandardised predictions per
+    particle, against the oracle network on features from numpy's SVD; a particle whose dprev is
+    0 (no iteration reached it) keeps plain PBMPM and counts as a fallback."""
     rng = np.random.default_rng(12)
     n = 4000
     p, fe, fp, d = _records(mpm, n, rng)
+    p["dx"][:, :3] = 1e-3 * rng.standard_normal((n, 3))
+    dprev = (2e-5 * rng.standard_normal((n, 3))).astype(np.float32)
+    dprev[::50] = 0.0
     params, norm = _net(4, out_scale=0.5)
     with mpm.MpmSimulation(n, grid_res=(64, 64, 64), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1 / 600,
                            pbmpm_iterations=2, pbmpm_force_divisor=2.0) as sim:
         sim.upload(p)
         sim.set_learn_params(params, norm)
         sim.learn_enable(True, limit=1e6)
+        sim.learn_dprev(dprev)
+        assert np.array_equal(sim.learn_dprev(), dprev)
         sim.debug_stage(1, -2)                           # MPM_STAGE_WARM_START
         out = sim.download()
-        assert sim.stats()["mlp_fallbacks"] == 0
-    feats = np.stack([mlp.features(fe[i], fp[i], d[i], p["vel"][i], p["mass"][i]) for i in range(n)]).astype(np.float32)
+        assert sim.stats()["mlp_fallbacks"] == n // 50
+    feats = np.stack([mlp.features(fe[i], fp[i], d[i], dprev[i]) for i in range(n)]).astype(np.float32)
     _, y, _ = mlp.forward_backward(params, norm, feats, np.zeros((n, 16), np.float32), emulate=True)
     nm = norm.reshape(4, 16)
-    want = d + (y[:, :9] * nm[3, :9] + nm[2, :9]).reshape(n, 3, 3).transpose(0, 2, 1)   # column-major outputs
+    pred = y[:, :12] * nm[3, :12] + nm[2, :12]
+    used = np.any(dprev != 0, axis=1)
+    want_d = d + np.where(used[:, None, None], pred[:, :9].reshape(n, 3, 3).transpose(0, 2, 1), 0.0)  # column-major outputs
+    want_dx = p["dx"][:, :3].astype(np.float64) + np.where(used[:, None], pred[:, 9:12], 0.0)
     got = mpm.matrix(out["D"])
-    err = np.abs(got - want).reshape(n, 9) / nm[3, :9]
+    err = np.abs(got - want_d).reshape(n, 9) / nm[3, :9]
     assert np.quantile(err, 0.99) < 0.02 and err.max() < 0.1
-    # everything but D is untouched
-    assert np.array_equal(out["pos"], p["pos"]) and np.array_equal(out["Fe"], p["Fe"])
+    errx = np.abs(out["dx"][:, :3] - want_dx) / nm[3, 9:12]
+    assert np.quantile(errx, 0.99) < 0.02 and errx.max() < 0.1
+    assert np.array_equal(out["dx"][~used, :3], p["dx"][~used, :3]) and np.array_equal(out["D"][~used], p["D"][~used])
+    # everything but D and dx is untouched
+    assert np.array_equal(out["pos"], p["pos"]) and np.array_equal(out["Fe"], p["Fe"]) and np.array_equal(out["vel"], p["vel"])
 
 
 def test_fallback_keeps_plain_pbmpm_bitexact(mpm):
@@ -113,15 +124,14 @@ def test_fallback_keeps_plain_pbmpm_bitexact(mpm):
 
 def test_collect_and_train_reduces_loss(mpm):
     """Pairs logged from k = 16 substeps, then Adam on the matrix cores: the standardised loss
-    (about 1 per output for an untrained network) falls; the part of the target a per-particle
-    network can explain on this scene is about a third of its variance."""
+    (about 1 per output for an untrained network) falls."""
     sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=32768, seed=32)
     with mpm.MpmSimulation(32768, **sc["sim"]) as sim:
         sim.set_collider(*sc["collider"])
         sim.upload(sc["particles"])
         sim.step(1, 20)
         npairs = sim.learn_collect(40, k_ref=16, every=2)
-        assert npairs == 20 * 32768
+        assert npairs == 19 * 32768          # substeps 2, 4, .., 38 (substep 0 only produces dprev)
         first = sim.learn_train(50, batch=4096, lr=1e-3, seed=1)
         later = sim.learn_train(1500, batch=4096, lr=1e-3, seed=2)
         assert np.isfinite(first) and np.isfinite(later)
@@ -149,11 +159,13 @@ def test_residual_tracking(mpm):
 
 
 def test_trained_warm_start_gain_over_plain(mpm):
-    """What the trained network gains: per-iteration residuals of PBMPM with and without the warm
-    start on the two-sphere scene (65,536 particles, k = 8, 120 substeps), the network trained on
-    pairs from another sampling of the scene. The warm start must not make any iteration's residual
-    worse by more than 3%; the measured gain is printed (about 1-2% per iteration at this scale)."""
-    n = 65536
+    """What the trained network gains. PBMPM's iterations relax the constraints towards a fixed
+    point, so accuracy is measured against the most-iterated run (k = 64): the warm start at
+    k = 16, trained on another sampling of the scene (trajectory at 16 iterations, targets from
+    64-iteration substeps from the same states), must end closer to the 64-iteration run than
+    plain PBMPM at 48 iterations does (3x fewer iterations; about 62 measured). Two-sphere scene,
+    262,144 particles, 200 substeps; rms position error over the particles."""
+    n, k, k_ref, sub = 262144, 16, 64, 200
 
     def make(seed):
         sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=n, seed=seed)
@@ -163,20 +175,27 @@ def test_trained_warm_start_gain_over_plain(mpm):
         return sim
 
     with make(41) as tr:
-        tr.learn_collect(120, k_ref=8, every=4)
-        tr.learn_train(1500, batch=16384, lr=1e-3, seed=3)
+        assert tr.learn_collect(sub, k_ref=k_ref, every=10, k_run=k) == 19 * n
+        tr.learn_train(4000, batch=65536, lr=1e-3, seed=3)
+        tr.learn_train(2000, batch=65536, lr=2e-4, seed=4)
         net = tr.learn_params()
-    res = {}
-    for learned in (False, True):
+
+    def final_pos(kk, learned):
         with make(42) as sim:
-            sim.set_iterations(8)
+            sim.set_iterations(kk)
             if learned:
                 sim.set_learn_params(*net)
                 sim.learn_enable(True)
-            sim.set_residual_tracking(True)
-            sim.step(1, 120)
-            res[learned] = sim.residuals()[0][1:8].astype(np.float64)
-            assert np.isfinite(mpm.state_columns(sim.download())).all()
-    gain = 1.0 - res[True] / res[False]
-    print("residual gain per iteration:", np.round(gain, 4).tolist())
-    assert np.all(res[True] <= 1.03 * res[False])
+            sim.step(1, sub)
+            p = sim.download()["pos"][:, :3].astype(np.float64)
+        assert np.isfinite(p).all()
+        return p
+
+    ref = final_pos(k_ref, False)
+
+    def err(p):
+        return float(np.sqrt((np.linalg.norm(p - ref, axis=1) ** 2).mean()))
+
+    e_learn, e48, e16 = err(final_pos(k, True)), err(final_pos(48, False)), err(final_pos(k, False))
+    print(f"rms error vs 64 iterations: learned k=16 {e_learn:.4f}, plain k=48 {e48:.4f}, plain k=16 {e16:.4f}")
+    assert e_learn < e48 and e_learn < 0.25 * e16

@@ -37,8 +37,7 @@ def _problem(n=48, seed=3):
     rng = np.random.default_rng(seed)
     feats = rng.standard_normal((n, 16)).astype(np.float32) * 2 + 0.5
     targs = rng.standard_normal((n, 16)).astype(np.float32) * 1e-3
-    feats[:, 15] = 0
-    targs[:, 9:] = 0
+    targs[:, 12:] = 0
     norm = np.zeros((4, 16), np.float32)
     norm[0] = 0.5
     norm[1] = 2.0
@@ -54,8 +53,7 @@ def test_analytic_gradient_matches_finite_differences():
     assert abs(loss / f.shape[0] - mlp.loss64(p, norm, f, t)) < 1e-9 * max(1.0, loss)
     rng = np.random.default_rng(7)
     live = [i for i in range(mlp.NPARAM)
-            if not (mlp.W1 + 15 * 64 <= i < mlp.B1)                     # padding input row
-            and not (mlp.W3 <= i < mlp.B3 and (i - mlp.W3) % 16 >= mlp.NOUT)
+            if not (mlp.W3 <= i < mlp.B3 and (i - mlp.W3) % 16 >= mlp.NOUT)
             and not (mlp.B3 + mlp.NOUT <= i)]
     idx = rng.choice(live, 80, replace=False)
     idx = np.concatenate([idx, [mlp.B1 + 5, mlp.B2 + 60, mlp.B3 + 2, mlp.W3 + 16 * 63 + 8]])
@@ -68,7 +66,6 @@ def test_analytic_gradient_matches_finite_differences():
         fd = (mlp.loss64(a, norm, f, t) - mlp.loss64(b, norm, f, t)) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, np.abs(g).max()), (i, fd, g[i])
     # padding parameters get exactly zero gradient
-    assert np.all(g[mlp.W1 + 15 * 64:mlp.B1] == 0)
     assert np.all(g[mlp.W3:mlp.B3].reshape(64, 16)[:, mlp.NOUT:] == 0)
     assert np.all(g[mlp.B3 + mlp.NOUT:] == 0)
 
@@ -83,14 +80,15 @@ def test_bf16_emulation_is_close_to_float64():
 
 
 def test_features_of_a_rotated_stretch():
-    """diag(R^T Fe) of Fe = R diag(s) is s; det Fp; D column-major; |v|; m."""
+    """Singular values of Fe = R diag(s) U^T largest first; det Fp; D column-major; dprev."""
     th = 0.3
     r = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
-    fe = r @ np.diag([1.1, 0.9, 1.05])
+    u = np.array([[1, 0, 0], [0, np.cos(0.7), -np.sin(0.7)], [0, np.sin(0.7), np.cos(0.7)]])
+    fe = r @ np.diag([1.1, 0.9, 1.05]) @ u.T
     fp = np.diag([1.0, 2.0, 0.5]) * 1.01
     d = np.arange(9.0).reshape(3, 3) * 1e-3
-    f = mlp.features(fe, fp, d, [3.0, 4.0, 0.0], 0.25)
-    assert np.allclose(f[:3], [1.1, 0.9, 1.05], atol=1e-12)
+    f = mlp.features(fe, fp, d, [3e-5, -4e-5, 1e-6])
+    assert np.allclose(f[:3], [1.1, 1.05, 0.9], atol=1e-12)
     assert abs(f[3] - np.linalg.det(fp)) < 1e-12
     assert np.allclose(f[4:13], d.T.ravel())
-    assert f[13] == 5.0 and f[14] == 0.25 and f[15] == 0.0
+    assert np.array_equal(f[13:16], [3e-5, -4e-5, 1e-6])

@@ -124,11 +124,12 @@ _SIGS = [
     ("mpm_selftest", ctypes.c_int, [ctypes.c_int, _P, _P, _P, _SZ]),
     ("mpm_divisor_fast", ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_int)]),
     ("mpm_set_iterations", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float]),
-    ("mpm_learn_collect", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    ("mpm_learn_collect", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     ("mpm_learn_train", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint32, ctypes.POINTER(ctypes.c_float)]),
     ("mpm_learn_get_params", ctypes.c_int, [_P, _P, _P]),
     ("mpm_learn_set_params", ctypes.c_int, [_P, _P, _P]),
     ("mpm_learn_enable", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float]),
+    ("mpm_learn_dprev", ctypes.c_int, [_P, _P, _P, ctypes.c_size_t]),
     ("mpm_mlp_grad", ctypes.c_int, [_P, _P, _P, _P, ctypes.c_uint32, _P, _P, _P]),
     ("mpm_set_residual_tracking", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_get_residuals", ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),

@@ -121,6 +121,8 @@ struct mpm_ctx {
     uint32_t *resid_max = nullptr, *resid_peak = nullptr;
     float* resid_sum = nullptr;
     bool resid_on = false;
+    float* learn_dprev = nullptr;            // learned warm start input: last per-iteration change of dx, by row
+    bool collecting = false;                 // inside mpm_learn_collect
     uint64_t resid_substeps = 0;
     std::map<uint64_t, bool> div_verified;   // (divisor, scale) bits -> fixed_div exact for all inputs
     std::map<uint64_t, bool> quot_verified;  // divisor bits -> quot_i32 exact for all |x| <= 2^100
@@ -279,7 +281,11 @@ Bufs make_bufs(mpm_ctx* c) {
     B.counters = c->counters;
     B.cqueue = c->cqueue;
     B.coll_data = c->coll_data; B.crec = c->crec; B.ctri = c->ctri;
-    B.resid_dx = c->resid_on ? c->resid_dx : nullptr;
+    // the per-iteration dx bookkeeping runs for residual tracking and for the learned warm start
+    // (which reads the last iteration's change of dx, dprev, as an input of the next substep)
+    const bool learn = (c->learn_on && c->mlp_ready) || c->collecting;
+    B.resid_dx = (c->resid_on || learn) ? c->resid_dx : nullptr;
+    B.dprev = learn ? c->learn_dprev : nullptr;
     B.resid_max = c->resid_max;
     B.resid_sum = c->resid_sum;
     return B;
@@ -494,7 +500,10 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
         c->last_pbmpm = true;
         if (stage == 1) return false;
         for (int i = 1; i < k; ++i) {
-            if (B.resid_dx) k_pbmpm<PB_MID, true><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3, i - 1);
+            // the residual of every iteration when it is tracked; for the warm start's dprev alone,
+            // only the dx of the G2P before LAST's (the last MID pass)
+            const bool track = B.resid_dx && ((P.learn_flags & 2u) || i == k - 1);
+            if (track) k_pbmpm<PB_MID, true><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3, i - 1);
             else k_pbmpm<PB_MID><<<G, 256, 0, s>>>(P, B, (i - 1) % 3, i % 3, (i + 1) % 3, i - 1);
             kt(c, KID_PB_MID);
             c->last_grid = i % 3;
@@ -1291,9 +1300,10 @@ static int ensure_mlp(mpm_ctx* c) {
         (rc = dalloc(c, &c->mlp_norm, 64)) || (rc = dalloc(c, &c->mlp_wbf, 16 * 64 + 64 * 64 + 64 * 16)) ||
         (rc = dalloc(c, &c->mlp_colsum, 1024 * 64)) || (rc = dalloc(c, &c->mlp_loss, 1)))
         return rc;
-    HIPCHK(c, hipMemset(c->mlp_grads, 0, MLP_NPARAM * 4));
-    HIPCHK(c, hipMemset(c->mlp_m1, 0, MLP_NPARAM * 4));
-    HIPCHK(c, hipMemset(c->mlp_m2, 0, MLP_NPARAM * 4));
+    HIPCHK(c, hipMemsetAsync(c->mlp_grads, 0, MLP_NPARAM * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mlp_m1, 0, MLP_NPARAM * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mlp_m2, 0, MLP_NPARAM * 4, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return MPM_OK;
 }
 
@@ -1308,8 +1318,28 @@ static void init_params(std::vector<float>& w, uint32_t seed) {
     for (int a = 0; a < 64; ++a) for (int o = 0; o < MLP_NOUT; ++o) w[MLP_W3 + a * 16 + o] = (float)(nrm() / 8.0 * 0.1);
 }
 
-int mpm_learn_collect(mpm_ctx* c, int n_substeps, int k_ref, int every, uint64_t max_pairs, uint64_t* n_pairs) {
-    if (!c || n_substeps < 0 || k_ref < 1 || k_ref > 64 || every < 1) return MPM_E_INVALID;
+// The per-iteration dx planes (resid_dx, and the residual maxima) and the warm start's input
+// planes (dprev, zeroed: no particle has been iterated with them yet).
+int ensure_learn_buffers(mpm_ctx* c) {
+    int rc;
+    if (!c->resid_dx) {
+        if ((rc = dalloc(c, &c->resid_dx, 3 * (size_t)c->stride)) || (rc = dalloc(c, &c->resid_max, 64 * RESID_SLOTS)) ||
+            (rc = dalloc(c, &c->resid_peak, 64)) || (rc = dalloc(c, &c->resid_sum, 64)))
+            return rc;
+        HIPCHK(c, hipMemsetAsync(c->resid_max, 0, 64 * RESID_SLOTS * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->resid_peak, 0, 64 * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->resid_sum, 0, 64 * 4, c->stream));
+    }
+    if (!c->learn_dprev) {
+        if ((rc = dalloc(c, &c->learn_dprev, 3 * (size_t)c->stride))) return rc;
+        HIPCHK(c, hipMemsetAsync(c->learn_dprev, 0, 3 * (size_t)c->stride * 4, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+
+int mpm_learn_collect(mpm_ctx* c, int n_substeps, int k_run, int k_ref, int every, uint64_t max_pairs, uint64_t* n_pairs) {
+    if (!c || n_substeps < 0 || k_run < 1 || k_run > 64 || k_ref < 1 || k_ref > 64 || every < 1) return MPM_E_INVALID;
     if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
     HIPCHK(c, hipSetDevice(c->device));
     if (max_pairs > c->ds_cap) {
@@ -1318,25 +1348,54 @@ int mpm_learn_collect(mpm_ctx* c, int n_substeps, int k_ref, int every, uint64_t
         if ((rc = dalloc(c, &c->ds_feats, max_pairs * 16)) || (rc = dalloc(c, &c->ds_targs, max_pairs * 16))) return rc;
         c->ds_cap = max_pairs;
     }
+    int rc;
+    if ((rc = ensure_learn_buffers(c))) return rc;
+    if (k_run < 2 || k_ref < 2) return fail(c, MPM_E_INVALID, "collection needs k_run, k_ref >= 2 (the features use the last iteration's change)");
     const int k_save = c->k_cur; const float fd_save = c->force_div_cur; const bool learn_save = c->learn_on;
-    c->k_cur = k_ref; c->force_div_cur = (float)k_ref; c->learn_on = false;
-    SimParams P;
-    int rc = make_params(c, P);
+    c->learn_on = false;
+    c->collecting = true;
+    SimParams Prun, Pref;
+    c->k_cur = k_run; c->force_div_cur = (float)k_run;
+    rc = make_params(c, Prun);
+    c->k_cur = k_ref; c->force_div_cur = (float)k_ref;
+    if (rc == MPM_OK) rc = make_params(c, Pref);
     if (rc == MPM_OK) {
+        // what the iterations' gravity shares add to dx over the substep (k_ref x grav_div dt^2)
+        const float gdt2[3] = {Pref.grav_div[0] * (Pref.dt * Pref.dt) * (float)k_ref, Pref.grav_div[1] * (Pref.dt * Pref.dt) * (float)k_ref,
+                               Pref.grav_div[2] * (Pref.dt * Pref.dt) * (float)k_ref};
         for (int s = 0; s < n_substeps; ++s) {
-            const bool log = (s % every == 0) && c->ds_n + c->n <= c->ds_cap;
-            if (log) { Bufs B = make_bufs(c); k_mlp_log_features<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(B, c->n, c->ds_feats, c->ds_n); }
-            enqueue_substep(c, MPM_METHOD_PBMPM, P, -1, false);
-            c->cur ^= 1; c->substeps++;
+            // substep 0 only produces dprev (the features' last-iteration change) for the ones after it
+            const bool log = s > 0 && (s % every == 0) && c->ds_n + c->n <= c->ds_cap;
             if (log) {
                 Bufs B = make_bufs(c);
-                k_mlp_log_targets<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(B, c->n, c->ds_feats, c->ds_targs, c->ds_n);
+                k_mlp_log_features<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(B, c->n, c->ds_feats, c->ds_targs, c->ds_n);
+                if (k_ref != k_run) {
+                    // the target substep: k_ref iterations from the same start state into the other
+                    // planes (a substep reads `cur` and writes only the other set), then the logged
+                    // targets from there; the trajectory's own substep below overwrites them
+                    enqueue_substep(c, MPM_METHOD_PBMPM, Pref, -1, false);
+                    c->cur ^= 1;
+                    B = make_bufs(c);
+                    c->cur ^= 1;
+                    k_mlp_log_targets<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(B, c->n, c->ds_feats, c->ds_targs, c->ds_n,
+                                                                                     gdt2[0], gdt2[1], gdt2[2]);
+                }
+            }
+            enqueue_substep(c, MPM_METHOD_PBMPM, k_ref != k_run ? Prun : Pref, -1, false);
+            c->cur ^= 1; c->substeps++;
+            if (log) {
+                if (k_ref == k_run) {
+                    Bufs B = make_bufs(c);
+                    k_mlp_log_targets<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(B, c->n, c->ds_feats, c->ds_targs, c->ds_n,
+                                                                                     gdt2[0], gdt2[1], gdt2[2]);
+                }
                 c->ds_n += c->n;
             }
         }
         hipError_t e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = fail(c, MPM_E_HIP, std::string("collect: ") + hipGetErrorString(e));
     }
+    c->collecting = false;
     c->k_cur = k_save; c->force_div_cur = fd_save; c->learn_on = learn_save;
     if (n_pairs) *n_pairs = c->ds_n;
     return rc;
@@ -1419,13 +1478,51 @@ int mpm_learn_set_params(mpm_ctx* c, const float* params, const float* norm) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->mlp_ready = true;
     c->adam_t = 0;
-    HIPCHK(c, hipMemset(c->mlp_m1, 0, MLP_NPARAM * 4));
-    HIPCHK(c, hipMemset(c->mlp_m2, 0, MLP_NPARAM * 4));
+    HIPCHK(c, hipMemsetAsync(c->mlp_m1, 0, MLP_NPARAM * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->mlp_m2, 0, MLP_NPARAM * 4, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return MPM_OK;
+}
+// The warm start's per-particle input dprev, by particle id (upload order): set from in_xyz and/or
+// read into out_xyz (3 floats per particle each).
+int mpm_learn_dprev(mpm_ctx* c, const float* in_xyz, float* out_xyz, size_t n) {
+    if (!c || (!in_xyz && !out_xyz)) return MPM_E_INVALID;
+    if (c->ens) return fail(c, MPM_E_INVALID, "batched context: the warm start is for single-scene contexts");
+    if (n != c->n) return fail(c, MPM_E_INVALID, "one dprev per particle");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc;
+    if ((rc = ensure_learn_buffers(c))) return rc;
+    std::vector<uint32_t> pid(c->n);
+    std::vector<float> planes(3 * (size_t)c->stride);
+    HIPCHK(c, hipMemcpy(pid.data(), c->pid[c->cur], c->n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(planes.data(), c->learn_dprev, planes.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t r = 0; r < c->n; ++r) {
+        const uint32_t id = pid[r];
+        if (id >= c->n) continue;
+        for (int a = 0; a < 3; ++a) {
+            if (out_xyz) out_xyz[3 * (size_t)id + a] = planes[(size_t)a * c->stride + r];
+            if (in_xyz) planes[(size_t)a * c->stride + r] = in_xyz[3 * (size_t)id + a];
+        }
+    }
+    if (in_xyz) HIPCHK(c, hipMemcpy(c->learn_dprev, planes.data(), planes.size() * 4, hipMemcpyHostToDevice));
     return MPM_OK;
 }
 int mpm_learn_enable(mpm_ctx* c, int on, float limit) {
     if (!c || !(limit > 0.0f)) return MPM_E_INVALID;
     if (on && !c->mlp_ready) return fail(c, MPM_E_STATE, "no trained network");
+    if (on) {
+        HIPCHK(c, hipSetDevice(c->device));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        int rc;
+        if ((rc = ensure_learn_buffers(c))) return rc;
+        // the first warm-started substep has no previous iteration change: every particle keeps
+        // plain PBMPM there (and counts as a fallback); the ones after use the network
+        if (!c->learn_on) {
+            HIPCHK(c, hipMemsetAsync(c->learn_dprev, 0, 3 * (size_t)c->stride * 4, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+    }
     c->learn_on = on != 0;
     c->learn_limit = limit;
     return MPM_OK;
@@ -1436,15 +1533,16 @@ int mpm_set_residual_tracking(mpm_ctx* c, int on) {
     HIPCHK(c, hipSetDevice(c->device));
     if (on && !c->resid_dx) {
         int rc;
-        if ((rc = dalloc(c, &c->resid_dx, 3 * (size_t)c->stride)) || (rc = dalloc(c, &c->resid_max, 64)) ||
+        if ((rc = dalloc(c, &c->resid_dx, 3 * (size_t)c->stride)) || (rc = dalloc(c, &c->resid_max, 64 * RESID_SLOTS)) ||
             (rc = dalloc(c, &c->resid_peak, 64)) || (rc = dalloc(c, &c->resid_sum, 64)))
             return rc;
     }
     if (c->resid_dx) {
+        // on the context's stream (non-blocking: a null-stream memset would not order with it)
+        HIPCHK(c, hipMemsetAsync(c->resid_max, 0, 64 * RESID_SLOTS * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->resid_peak, 0, 64 * 4, c->stream));
+        HIPCHK(c, hipMemsetAsync(c->resid_sum, 0, 64 * 4, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
-        HIPCHK(c, hipMemset(c->resid_max, 0, 64 * 4));
-        HIPCHK(c, hipMemset(c->resid_peak, 0, 64 * 4));
-        HIPCHK(c, hipMemset(c->resid_sum, 0, 64 * 4));
     }
     c->resid_on = on != 0;
     c->resid_substeps = c->substeps;
@@ -1595,7 +1693,8 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
         if (e == hipSuccess) e = hipMemcpy(c->d_key_base, c->ens_key_base.data(), (n_scenes + 1) * 4, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(c->d_nphase, c->ens_nphase.data(), n_scenes * 4, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(c->d_method, meth.data(), n_scenes * 4, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemset(c->d_ctl, 0, (ENS_CTL_PHASE + 128) * 4);
+        if (e == hipSuccess) e = hipMemsetAsync(c->d_ctl, 0, (ENS_CTL_PHASE + 128) * 4, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = MPM_E_HIP;
     }
     if (rc) { release(c); delete c; return rc; }

@@ -125,7 +125,10 @@ struct Bufs {
     uint32_t* counters;
     uint32_t* cqueue;          // sorted indices of particles whose path reaches the collider's box
     float* resid_dx;           // optional: dx of the previous G2P per particle (3 planes, stride)
-    uint32_t* resid_max;       // optional: per G2P iteration, max over particles of |dx_i - dx_{i-1}|_2 (float bits)
+    float* dprev;              // optional (learned warm start): per row, the substep's last per-iteration
+                               // change of dx, dx_{k-1} - dx_{k-2} - grav_div dt^2 (3 planes, stride);
+                               // 0 for particles no iteration reached
+    uint32_t* resid_max;       // optional: [RESID_SLOTS][64] per G2P iteration, max over particles of |dx_i - dx_{i-1}|_2 (float bits)
     float* resid_sum;          // optional: per G2P iteration, sum over substeps of that max
     const uint32_t* coll_data;
     const float4* crec;        // threaded BVH: 2 float4 per record (visiting order of the stack DFS)
@@ -1326,31 +1329,43 @@ enum { PB_FIRST = 0, PB_MID = 1, PB_LAST = 2 };
 // every iteration.
 // Residual bookkeeping of the G2P of iteration `it` (optional):
 // |dx_it - dx_{it-1} - (g / force_div) dt^2|_2 / domain.
-MPM_DEV float track_residual(const SimParams& P, const Bufs& B, uint32_t q, v3 dx, int it) {
+MPM_DEV float track_residual(const SimParams& P, const Bufs& B, uint32_t q, v3 dx, int it, bool last = false) {
     const uint32_t sd = B.stride;
     float r = 0.0f;
+    v3 e = mk(0.0f, 0.0f, 0.0f);
     if (it > 0) {
         const v3 pr = ld3(B.resid_dx, sd, 0, q);
         // each iteration adds grav_div dt to every node velocity: dx grows by grav_div dt^2
-        const v3 e = dx - pr - mk(P.grav_div[0], P.grav_div[1], P.grav_div[2]) * (P.dt * P.dt);
+        e = dx - pr - mk(P.grav_div[0], P.grav_div[1], P.grav_div[2]) * (P.dt * P.dt);
         r = __builtin_sqrtf(e.x * e.x + e.y * e.y + e.z * e.z) * P.inv_domain;
     }
+    // the last iteration's change is the learned warm start's input of the next substep
+    if (last && B.dprev) st3(B.dprev, sd, 0, q, e);
     st3(B.resid_dx, sd, 0, q, dx);
     return r;
 }
+// The per-iteration maxima live in RESID_SLOTS copies, 256 B apart ([slot][64 iterations]), and a
+// wave publishes into the copy of its workgroup: one address for every wave of the grid would
+// serialise ~16k same-address atomics per pass in one L2 channel.
+enum : uint32_t { RESID_SLOTS = 128 };
 MPM_DEV void publish_residual(const Bufs& B, float r, int it) {
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) r = fmaxf(r, __shfl_xor(r, o, 64));
-    if (lane_id() == 0 && r > 0.0f) atomicMax(&B.resid_max[it], as_u(r));
+    if (lane_id() == 0 && r > 0.0f) atomicMax(&B.resid_max[(blockIdx.x % RESID_SLOTS) * 64u + (uint32_t)it], as_u(r));
 }
-// Per-substep roll-up of the residuals: sum and peak over substeps, then reset.
+// Per-substep roll-up of the residuals: max over the slots, then sum and peak over substeps, and
+// the slots reset.
 __global__ void k_resid_roll(Bufs B, int k, uint32_t* peak) {
     const int it = threadIdx.x;
     if (it >= k) return;
-    const uint32_t v = B.resid_max[it];
+    uint32_t v = 0u;
+    for (uint32_t s = 0; s < RESID_SLOTS; ++s) {
+        const uint32_t x = B.resid_max[s * 64u + (uint32_t)it];
+        v = x > v ? x : v;
+        B.resid_max[s * 64u + (uint32_t)it] = 0u;
+    }
     B.resid_sum[it] += as_f(v);
     peak[it] = peak[it] > v ? peak[it] : v;
-    B.resid_max[it] = 0u;
 }
 
 // Slot, within a round of FIRST (blockDim.x rows), of this thread's particle when the round's rows
@@ -1498,16 +1513,17 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                     v3 nv; m3 Bm;
                     g2p_pbmpm(T, s, off, nv, Bm);
                     p.dx = nv * P.dt;
-                    if (TRACK) resid = track_residual(P, B, q, p.dx, it);
+                    if (TRACK) resid = track_residual(P, B, q, p.dx, it, true);
                     p.D = mscale_r(mscale_r(mscale_r(Bm, P.dt), 4.0f), P.grad_scale);
                     p.vel = nv;
                 } else {
                     p.dx = ld3(B.nxt, sd, DXX, q); p.D = ldm(B.nxt, sd, DD, q); p.vel = ld3(B.nxt, sd, VX, q);
+                    if (TRACK && B.dprev) st3(B.dprev, sd, 0, q, mk(0.0f, 0.0f, 0.0f));
                 }
                 cand = integrate_store(P, B, p, q);
             }
             if (MODE == PB_LAST && P.has_collider) queue_collider_wg(B, T, cand, q);   // uniform: P is the chunk's
-            if (MODE != PB_FIRST && TRACK && it > 0) publish_residual(B, resid, it);
+            if (MODE != PB_FIRST && TRACK && it > 0 && (P.learn_flags & 2u)) publish_residual(B, resid, it);
         }
         __syncthreads();
         if (MODE == PB_FIRST) tile_flush_pb<true>(T, go, B.gmass);
@@ -1729,6 +1745,7 @@ MPM_DEV void straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_
         st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, VX, q, p.vel);
         stm(B.nxt, sd, FE, q, p.Fe);
         st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
+        if (B.dprev) st3(B.dprev, sd, 0, q, mk(0.0f, 0.0f, 0.0f));   // not iterated this substep
     }
 }
 __global__ __launch_bounds__(256) void k_straggle(SimParams P, Bufs B) {

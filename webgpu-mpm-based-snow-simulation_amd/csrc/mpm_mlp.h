@@ -1,11 +1,16 @@
-// Learned PBMPM warm start: a 15 -> 64 -> 64 -> 9 MLP (GELU) with BF16 weights and FP32
+// Learned PBMPM warm start: a 16 -> 64 -> 64 -> 12 MLP (GELU) with BF16 weights and FP32
 // accumulation on the gfx950 matrix cores (v_mfma_f32_16x16x16_bf16).
 //
 // Per particle it maps the state at the start of a substep
-//   x = [diag(R^T F_e) (3), det F_p (1), D (9, column-major), |v| (1), m (1)]      (15, pad 16)
-// to the correction of the deformation displacement D that the PBMPM iterations converge to,
-// y = D_converged - D (9, pad 16). Features and targets are standardised with per-column mean and
-// standard deviation of the logged training set.
+//   x = [singular values of F_e, largest first (3), det F_p (1), D (9, column-major),
+//        dprev (3): the previous substep's last per-iteration change of dx]                  (16)
+// to the change that k_ref PBMPM iterations make,
+//   y = [D_converged - D (9, column-major), dx_converged - dx - g dt^2 (3)]                 (12, pad 16)
+// (g dt^2: what the iterations' gravity shares add to dx). The slow part of the PBMPM iteration is a
+// smooth, body-wide mode whose per-iteration change of dx varies little from one substep to the
+// next, so dprev carries most of where the iterations will take dx; a prediction of D alone does not
+// move the residual. Features and targets are standardised with per-column mean and standard
+// deviation of the logged training set.
 //
 // MFMA 16x16x16 bf16 operand layout (lane l of a wave64):
 //   A[row = l % 16][k = 4 (l / 16) + j], B[k = 4 (l / 16) + j][col = l % 16],
@@ -21,12 +26,12 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 
 // parameter block (FP32 master copy), offsets in floats
 enum : int {
-    MLP_IN = 16, MLP_H = 64, MLP_OUT = 16, MLP_NIN = 15, MLP_NOUT = 9,
+    MLP_IN = 16, MLP_H = 64, MLP_OUT = 16, MLP_NIN = 16, MLP_NOUT = 12,
     MLP_W1 = 0,                       // [16][64]   row = input feature
     MLP_B1 = MLP_W1 + 16 * 64,        // [64]
     MLP_W2 = MLP_B1 + 64,             // [64][64]   row = layer-1 unit
     MLP_B2 = MLP_W2 + 64 * 64,        // [64]
-    MLP_W3 = MLP_B2 + 64,             // [64][16]   row = layer-2 unit, columns 9..15 zero
+    MLP_W3 = MLP_B2 + 64,             // [64][16]   row = layer-2 unit, columns 12..15 zero
     MLP_B3 = MLP_W3 + 64 * 16,        // [16]
     MLP_NPARAM = MLP_B3 + 16          // 6288
 };
@@ -38,10 +43,12 @@ MPM_DEV short to_bf16(float x) {      // round to nearest even
 }
 MPM_DEV float from_bf16(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
 
-// GELU (tanh form) and its derivative; tanh(u) = 1 - 2 / (exp(2u) + 1) with fexp.
+// GELU (tanh form) and its derivative; tanh(u) = 1 - 2 / (exp(2u) + 1) with fexp and the
+// hardware reciprocal (v_rcp_f32, 1 ulp): the network's arithmetic is BF16-limited, and 128 of
+// these per sample dominate the warm start's VALU work with a correctly rounded divide.
 MPM_DEV float gelu_t(float z) {
     const float u = 0.7978845608f * (z + 0.044715f * ((z * z) * z));
-    return 1.0f - 2.0f / (fexp(2.0f * u) + 1.0f);
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(fexp(2.0f * u) + 1.0f);
 }
 MPM_DEV float gelu(float z) { return (0.5f * z) * (1.0f + gelu_t(z)); }
 MPM_DEV float gelu_grad(float z) {
@@ -395,30 +402,46 @@ __global__ __launch_bounds__(256) void k_mlp_grad_fixed(MlpBufs M, const float* 
 }
 
 // Features of a particle at the start of a substep (see the header comment).
-MPM_DEV void mlp_features(const m3& Fe, float Jp, const m3& D, v3 vel, float mass, float f[16]) {
-    const m3 R = polar_rotation(Fe);
-    const m3 S = mmul(transpose(R), Fe);
-    f[0] = S.c[0].x; f[1] = S.c[1].y; f[2] = S.c[2].z;
+MPM_DEV void mlp_features(const m3& Fe, float Jp, const m3& D, v3 dprev, float f[16]) {
+    float a[9], v[9], b[9], sig[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { a[3 * c] = Fe.c[c].x; a[3 * c + 1] = Fe.c[c].y; a[3 * c + 2] = Fe.c[c].z; }
+    svd3<float>(a, v, b, sig);
+    // largest first
+    float s0 = sig[0], s1 = sig[1], s2 = sig[2], t;
+    if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
+    if (s1 < s2) { t = s1; s1 = s2; s2 = t; }
+    if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
+    f[0] = s0; f[1] = s1; f[2] = s2;
     f[3] = Jp;
     for (int c = 0; c < 3; ++c) { f[4 + 3 * c] = D.c[c].x; f[5 + 3 * c] = D.c[c].y; f[6 + 3 * c] = D.c[c].z; }
-    f[13] = length(vel);
-    f[14] = mass;
-    f[15] = 0.0f;
+    f[13] = dprev.x; f[14] = dprev.y; f[15] = dprev.z;
+}
+MPM_DEV v3 mlp_dprev(const Bufs& B, uint32_t i) {
+    return B.dprev ? ld3(B.dprev, B.stride, 0, i) : mk(0.0f, 0.0f, 0.0f);
 }
 
-// Logs the features of every particle (current order) into the dataset rows base + particle id.
-__global__ __launch_bounds__(256) void k_mlp_log_features(Bufs B, uint32_t n, float* feats, uint64_t base) {
+// Logs the features of every particle (current order) into the dataset rows base + particle id,
+// and the particle's dx into columns 12..14 of its target row (read back by k_mlp_log_targets).
+__global__ __launch_bounds__(256) void k_mlp_log_features(Bufs B, uint32_t n, float* feats, float* targs, uint64_t base) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t sd = B.stride;
     float f[16];
-    mlp_features(ldm(B.cur, sd, FE, i), ld(B.cur, sd, JP, i), ldm(B.cur, sd, DD, i), ld3(B.cur, sd, VX, i), ld(B.cur, sd, PM, i), f);
-    float4* o = reinterpret_cast<float4*>(feats + (base + B.pid_cur[i]) * 16);
+    mlp_features(ldm(B.cur, sd, FE, i), ld(B.cur, sd, JP, i), ldm(B.cur, sd, DD, i), mlp_dprev(B, i), f);
+    const uint64_t row = base + B.pid_cur[i];
+    float4* o = reinterpret_cast<float4*>(feats + row * 16);
     o[0] = make_float4(f[0], f[1], f[2], f[3]); o[1] = make_float4(f[4], f[5], f[6], f[7]);
     o[2] = make_float4(f[8], f[9], f[10], f[11]); o[3] = make_float4(f[12], f[13], f[14], f[15]);
+    const v3 dx = ld3(B.cur, sd, DXX, i);
+    reinterpret_cast<float4*>(targs + row * 16)[3] = make_float4(dx.x, dx.y, dx.z, 0.0f);
 }
-// After the substep (state in `cur`): target = D_converged - D_start (D_start = features 4..12).
-__global__ __launch_bounds__(256) void k_mlp_log_targets(Bufs B, uint32_t n, const float* feats, float* targs, uint64_t base) {
+// After the substep (state in `cur`): target = (D_converged - D_start, dx_converged - dx_start -
+// g dt^2), D_start = features 4..12, dx_start = target columns 12..14 (logged with the features),
+// dx_converged = the last G2P's dx (resid_dx, before integrate's walls and collider). A particle that
+// no iteration reached this substep (dprev 0) gets a zero dx target.
+__global__ __launch_bounds__(256) void k_mlp_log_targets(Bufs B, uint32_t n, const float* feats, float* targs, uint64_t base,
+                                                         float gx, float gy, float gz) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t sd = B.stride;
@@ -430,7 +453,14 @@ __global__ __launch_bounds__(256) void k_mlp_log_targets(Bufs B, uint32_t n, con
         t[3 * c + 1] = D.c[c].y - feats[row * 16 + 5 + 3 * c];
         t[3 * c + 2] = D.c[c].z - feats[row * 16 + 6 + 3 * c];
     }
-    for (int c = 9; c < 16; ++c) t[c] = 0.0f;
+    const v3 dp = mlp_dprev(B, i);
+    const bool iterated = dp.x != 0.0f || dp.y != 0.0f || dp.z != 0.0f;
+    const v3 dxc = ld3(B.resid_dx, sd, 0, i);
+    const float* dx0 = targs + row * 16 + 12;
+    t[9] = iterated ? (dxc.x - dx0[0]) - gx : 0.0f;
+    t[10] = iterated ? (dxc.y - dx0[1]) - gy : 0.0f;
+    t[11] = iterated ? (dxc.z - dx0[2]) - gz : 0.0f;
+    for (int c = 12; c < 16; ++c) t[c] = 0.0f;
     float4* o = reinterpret_cast<float4*>(targs + row * 16);
     for (int c = 0; c < 4; ++c) o[c] = make_float4(t[4 * c], t[4 * c + 1], t[4 * c + 2], t[4 * c + 3]);
 }
@@ -469,9 +499,11 @@ __global__ __launch_bounds__(256) void k_mlp_colstats(const float* feats, const 
     }
 }
 
-// Warm start of a PBMPM substep: D += de-standardised MLP prediction, per particle, in `cur`.
-// A particle falls back to the plain method (prediction dropped) when any standardised output
-// exceeds `limit` in magnitude or is not finite; the count of fallbacks goes to counters[C_FALLBACK].
+// Warm start of a PBMPM substep, per particle in `cur`: D += the predicted D change and dx += the
+// predicted dx change (de-standardised). A particle falls back to the plain method (prediction
+// dropped) when any standardised output exceeds `limit` in magnitude or is not finite, or when no
+// iteration reached it in the previous substep (dprev 0: no input to predict from); the count of
+// fallbacks goes to counters[C_FALLBACK].
 __global__ __launch_bounds__(256) void k_mlp_warmstart(MlpBufs M, Bufs B, uint32_t n, float limit, uint32_t* n_fallback) {
     __shared__ MlpLds L;
     __shared__ MlpWave Wv[4];
@@ -486,9 +518,11 @@ __global__ __launch_bounds__(256) void k_mlp_warmstart(MlpBufs M, Bufs B, uint32
         const uint32_t i = base + (uint32_t)l;
         float f[16];
         m3 D = zero3();
+        v3 dp = mk(0.0f, 0.0f, 0.0f);
         if (i < n) {
             D = ldm(B.cur, sd, DD, i);
-            mlp_features(ldm(B.cur, sd, FE, i), ld(B.cur, sd, JP, i), D, ld3(B.cur, sd, VX, i), ld(B.cur, sd, PM, i), f);
+            dp = mlp_dprev(B, i);
+            mlp_features(ldm(B.cur, sd, FE, i), ld(B.cur, sd, JP, i), D, dp, f);
         } else {
             for (int c = 0; c < 16; ++c) f[c] = L.mu_x[c];
         }
@@ -502,18 +536,20 @@ __global__ __launch_bounds__(256) void k_mlp_warmstart(MlpBufs M, Bufs B, uint32
         }
         bool ok = true;
         if (i < n) {
-            float dd[9];
-            for (int c = 0; c < 9; ++c) {
+            float dd[MLP_NOUT];
+            for (int c = 0; c < MLP_NOUT; ++c) {
                 const float ys_c = ys[w][l][c];
                 if (!(ys_c <= limit && ys_c >= -limit)) ok = false;
                 dd[c] = ys_c * L.sd_y[c] + L.mu_y[c];
             }
+            if (dp.x == 0.0f && dp.y == 0.0f && dp.z == 0.0f) ok = false;
             if (ok) {
                 for (int c = 0; c < 3; ++c) { D.c[c].x += dd[3 * c]; D.c[c].y += dd[3 * c + 1]; D.c[c].z += dd[3 * c + 2]; }
                 stm(B.cur, sd, DD, i, D);
+                st3(B.cur, sd, DXX, i, ld3(B.cur, sd, DXX, i) + mk(dd[9], dd[10], dd[11]));
             }
         }
-        const uint64_t fb = __ballot(!ok);
+        const uint64_t fb = __ballot(i < n && !ok);
         if (l == 0 && fb) atomicAdd(n_fallback, (uint32_t)__popcll(fb));
         wave_sync();
     }

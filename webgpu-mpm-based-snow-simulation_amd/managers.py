@@ -372,11 +372,14 @@ class MpmSimulation:
         fd = float(k if force_divisor is None else force_divisor)
         check(self.lib, self.ctx, self.lib.mpm_set_iterations(self.ctx, int(k), fd))
 
-    def learn_collect(self, n_substeps, k_ref, every=1, max_pairs=None):
-        """Advances n_substeps PBMPM substeps with k_ref iterations, logging (features, target)."""
+    def learn_collect(self, n_substeps, k_ref, every=1, max_pairs=None, k_run=None):
+        """Advances n_substeps PBMPM substeps with k_run iterations (default k_ref), logging
+        (features, target) pairs every `every` substeps; targets from k_ref-iteration substeps
+        run from the same start states."""
         n = ctypes.c_uint64()
         mp = self.nParticles * max(1, (n_substeps + every - 1) // every) if max_pairs is None else max_pairs
-        check(self.lib, self.ctx, self.lib.mpm_learn_collect(self.ctx, int(n_substeps), int(k_ref), int(every), int(mp), ctypes.byref(n)))
+        kr = int(k_ref if k_run is None else k_run)
+        check(self.lib, self.ctx, self.lib.mpm_learn_collect(self.ctx, int(n_substeps), kr, int(k_ref), int(every), int(mp), ctypes.byref(n)))
         return n.value
 
     def learn_train(self, steps, batch=4096, lr=1e-3, seed=1):
@@ -396,6 +399,14 @@ class MpmSimulation:
         if p.size != _lib.MLP_NPARAM or nm.size != 64:
             raise ValueError("expected 6288 parameters and 64 standardisation values")
         check(self.lib, self.ctx, self.lib.mpm_learn_set_params(self.ctx, _ptr(p), _ptr(nm)))
+
+    def learn_dprev(self, dprev=None):
+        """The warm start's input dprev (previous substep's last per-iteration change of dx) by
+        particle id: sets it when given, returns the current values (n, 3)."""
+        out = np.zeros((self.nParticles, 3), np.float32)
+        src = None if dprev is None else np.ascontiguousarray(dprev, np.float32).reshape(self.nParticles, 3)
+        check(self.lib, self.ctx, self.lib.mpm_learn_dprev(self.ctx, None if src is None else _ptr(src), _ptr(out), self.nParticles))
+        return out if src is None else src.copy()
 
     def learn_enable(self, on=True, limit=8.0):
         check(self.lib, self.ctx, self.lib.mpm_learn_enable(self.ctx, 1 if on else 0, float(limit)))
