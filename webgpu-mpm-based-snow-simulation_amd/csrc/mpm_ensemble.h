@@ -106,34 +106,44 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
             });
             B.key[i] = key; B.mask[i] = mask;
         }
+        // grouped by key as in k_keys_count: one round trip of the leaders' atomics per wave
         uint64_t active = __ballot(valid);
+        int my_leader = 0;
+        uint32_t my_rank = 0, lead_cnt = 0, lead_m = 0;
         while (active) {
-            int leader = __ffsll((unsigned long long)active) - 1;
-            uint32_t lk = __shfl(key, leader, 64);
-            bool peer = valid && key == lk;
-            uint64_t peers = __ballot(peer);
-            uint32_t m = wave_or(peer ? mask : 0u);
-            uint32_t b = 0, fresh = 0;
-            if (lane_id() == leader) {
-                b = atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
-                fresh = m & ~atomicOr(&B.kmask[lk], m);
-            }
-            b = __shfl(b, leader, 64);
-            if (peer) B.lofs[i] = b + (uint32_t)__popcll(peers & lanemask_lt());
-            fresh = __shfl(fresh, leader, 64);
-            if (P0.direct_blocks && fresh) {
-                // lk and the new bits are the same in every lane: scene and parameters read once
-                // per wave; each (key, neighbour) pair is flagged once
+            const int leader = __ffsll((unsigned long long)active) - 1;
+            const uint32_t lk = __shfl(key, leader, 64);
+            const bool peer = valid && key == lk;
+            const uint64_t peers = __ballot(peer);
+            const uint32_t m = wave_or(peer ? mask : 0u);
+            if (peer) { my_leader = leader; my_rank = (uint32_t)__popcll(peers & lanemask_lt()); }
+            if (lane_id() == leader) { lead_cnt = (uint32_t)__popcll(peers); lead_m = m; }
+            active &= ~peers;
+        }
+        const bool lead = valid && my_leader == lane_id();
+        uint32_t b = 0, fresh = 0;
+        if (lead) {
+            b = atomicAdd(&B.kcount[key], lead_cnt);
+            fresh = lead_m & ~atomicOr(&B.kmask[key], lead_m);
+        }
+        b = __shfl(b, my_leader, 64);
+        if (valid) B.lofs[i] = b + my_rank;
+        if (P0.direct_blocks) {
+            uint64_t fl = __ballot(fresh != 0u);
+            while (fl) {
+                // the key and its new bits are the same in every lane: scene and parameters read
+                // once per key; each (key, neighbour) pair is flagged once
+                const int l = __ffsll((unsigned long long)fl) - 1;
                 const int j = lane_id();
-                const uint32_t uk = __builtin_amdgcn_readfirstlane(lk);
-                const uint32_t um = __builtin_amdgcn_readfirstlane(fresh);
+                const uint32_t uk = __builtin_amdgcn_readfirstlane(__shfl(key, l, 64));
+                const uint32_t um = __builtin_amdgcn_readfirstlane(__shfl(fresh, l, 64));
                 const SimParams& P = E.sp[__builtin_amdgcn_readfirstlane(seg_find(kb, E.S, uk))];
                 if (j < 27 && ((um >> j) & 1u)) {
-                    int b[3]; block_coords(P, uk, b);
-                    B.kalloc[block_key_at(P, b[0], b[1], b[2], j)] = 1u;
+                    int bc[3]; block_coords(P, uk, bc);
+                    B.kalloc[block_key_at(P, bc[0], bc[1], bc[2], j)] = 1u;
                 }
+                fl &= fl - 1ull;
             }
-            active &= ~peers;
         }
     }
 }

@@ -414,24 +414,40 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
             }
             B.key[i] = key; B.mask[i] = mask;
         }
+        // group the wave's lanes by key: the lowest lane of each key leads it and holds the key's
+        // particle count and mask OR; every lane keeps its leader and its rank among its peers
         uint64_t active = __ballot(valid);
+        int my_leader = 0;
+        uint32_t my_rank = 0, lead_cnt = 0, lead_m = 0;
         while (active) {
-            int leader = __ffsll((unsigned long long)active) - 1;
-            uint32_t lk = __shfl(key, leader, 64);
-            bool peer = valid && key == lk;
-            uint64_t peers = __ballot(peer);
-            uint32_t m = wave_or(peer ? mask : 0u);
-            uint32_t b = 0, fresh = 0;
-            if (lane_id() == leader) {
-                b = atomicAdd(&B.kcount[lk], (uint32_t)__popcll(peers));
-                fresh = m & ~atomicOr(&B.kmask[lk], m);
-            }
-            b = __shfl(b, leader, 64);
-            if (peer) B.lofs[i] = b + (uint32_t)__popcll(peers & lanemask_lt());
-            // each (key, neighbour) pair is flagged once: by the wave whose OR first set its bit
-            fresh = __shfl(fresh, leader, 64);
-            if (P.direct_blocks && jl < 27 && ((fresh >> jl) & 1u)) B.kalloc[lk + koff] = 1u;
+            const int leader = __ffsll((unsigned long long)active) - 1;
+            const uint32_t lk = __shfl(key, leader, 64);
+            const bool peer = valid && key == lk;
+            const uint64_t peers = __ballot(peer);
+            const uint32_t m = wave_or(peer ? mask : 0u);
+            if (peer) { my_leader = leader; my_rank = (uint32_t)__popcll(peers & lanemask_lt()); }
+            if (lane_id() == leader) { lead_cnt = (uint32_t)__popcll(peers); lead_m = m; }
             active &= ~peers;
+        }
+        // the leaders' atomics of all the wave's keys issue together (one round trip per wave); the
+        // count atomic's old value is the base of the key's offsets
+        const bool lead = valid && my_leader == lane_id();
+        uint32_t b = 0, fresh = 0;
+        if (lead) {
+            b = atomicAdd(&B.kcount[key], lead_cnt);
+            fresh = lead_m & ~atomicOr(&B.kmask[key], lead_m);
+        }
+        b = __shfl(b, my_leader, 64);
+        if (valid) B.lofs[i] = b + my_rank;
+        // each (key, neighbour) pair is flagged once: by the wave whose OR first set its bit
+        if (P.direct_blocks) {
+            uint64_t fl = __ballot(fresh != 0u);
+            while (fl) {
+                const int l = __ffsll((unsigned long long)fl) - 1;
+                const uint32_t fk = __shfl(key, l, 64), ff = __shfl(fresh, l, 64);
+                if (jl < 27 && ((ff >> jl) & 1u)) B.kalloc[fk + koff] = 1u;
+                fl &= fl - 1ull;
+            }
         }
     }
 }
@@ -1210,7 +1226,13 @@ MPM_DEV void stencil_offsets(const Stencil& s, float off[3][3]) {
 
 // gridToParticle.cs.wgsl:82-130 from the LDS velocity tile. Nodes are visited in the reference
 // order (z, y, x) so the FP32 sums round identically; nodes without mass are skipped as there.
-MPM_DEV void g2p_pbmpm(const Tile& T, const Stencil& s, const float off[3][3], v3& nv, m3& Bm) {
+// ALL: every node is added, massless ones too. Their tile velocity is +0 exactly, so with finite
+// weights and offsets their terms are +-0, and adding +-0 leaves every sum unchanged: the sums start
+// at +0 and a round-to-nearest sum is -0 only when both operands are -0, so no sum is ever -0.
+// (A particle whose weights are not finite, a NaN position binned at cell 0, takes the skipping
+// form: there a skipped node keeps the sum finite.)
+template <bool ALL>
+MPM_DEV void g2p_pbmpm_nodes(const Tile& T, const Stencil& s, const float off[3][3], v3& nv, m3& Bm) {
     nv = mk(0, 0, 0);
     Bm = zero3();
     const int tb = tile_of(s.t0[0] - 1, s.t0[1] - 1, s.t0[2] - 1);
@@ -1221,7 +1243,7 @@ MPM_DEV void g2p_pbmpm(const Tile& T, const Stencil& s, const float off[3][3], v
 #pragma unroll
     for (int ox = 0; ox < 3; ++ox) {
         const float4 q = T.vel[tb + ox + 6 * oy + 36 * oz];
-        if (q.w == 2.0f) {
+        if (ALL || q.w == 2.0f) {
             const float cw = (s.w[ox].x * s.w[oy].y) * s.w[oz].z;
             const v3 cv = cw * mk(q.x, q.y, q.z);
             nv = nv + cv;
@@ -1230,6 +1252,11 @@ MPM_DEV void g2p_pbmpm(const Tile& T, const Stencil& s, const float off[3][3], v
             Bm.c[2] = Bm.c[2] + cv * off[2][oz];
         }
     }
+}
+MPM_DEV void g2p_pbmpm(const Tile& T, const Stencil& s, const float off[3][3], v3& nv, m3& Bm) {
+    const bool finite = isfinite(s.f.x) && isfinite(s.f.y) && isfinite(s.f.z);
+    if (__builtin_expect(__ballot(!finite) == 0, 1)) g2p_pbmpm_nodes<true>(T, s, off, nv, Bm);
+    else g2p_pbmpm_nodes<false>(T, s, off, nv, Bm);
 }
 
 // pbmpm.wgsl:41-76 + :79-147 into the LDS accumulators. The two i32 contributions of a node's
@@ -1663,6 +1690,32 @@ __global__ __launch_bounds__(256) void k_grid_update(SimParams P, Bufs B) {
         B.grid0[c] = q;
     }
 }
+// gridToParticle.cs.wgsl:28-80 from the LDS tile of updated node velocities: velocity and the
+// velocity gradient sum (cell units), nodes in the reference order (z, y, x). ALL adds the massless
+// nodes too (+0 velocity, so +-0 terms that leave every sum unchanged; see g2p_pbmpm_nodes).
+template <bool ALL>
+MPM_DEV void g2p_explicit_nodes(const Tile& T, const Stencil& s, v3& nv, m3& G) {
+    v3 d[3]; bspline_d(s.f, d);
+    nv = mk(0, 0, 0);
+    G = zero3();
+#pragma unroll
+    for (int oz = -1; oz <= 1; ++oz)
+#pragma unroll
+    for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll
+    for (int ox = -1; ox <= 1; ++ox) {
+        const float4 qv = T.vel[tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz)];
+        if (!ALL && qv.w != 2.0f) continue;
+        const v3 v = mk(qv.x, qv.y, qv.z);
+        const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
+        nv = nv + cw * v;
+        const v3 g = mk((d[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z,
+                        (s.w[ox + 1].x * d[oy + 1].y) * s.w[oz + 1].z,
+                        (s.w[ox + 1].x * s.w[oy + 1].y) * d[oz + 1].z);
+        m3 o; o.c[0] = g.x * v; o.c[1] = g.y * v; o.c[2] = g.z * v;
+        G = madd(G, o);
+    }
+}
 // gridToParticle.cs.wgsl:28-80 + integrateParticles.wgsl:12-72
 // One work chunk of the explicit G2P + integrate. FUSED: the tile applies the grid update
 // (gridUpdate.cs.wgsl:35-53) to its nodes itself, reading the P2G sums instead of the updated grid.
@@ -1687,26 +1740,10 @@ MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             p.Fe = ldm(B.nxt, sd, FE, q);
             if (ok) {
                 Stencil s; stencil<POW2>(P, T, p.pos, s);
-                v3 d[3]; bspline_d(s.f, d);
-                v3 nv = mk(0, 0, 0);
-                m3 G = zero3();
-#pragma unroll
-                for (int oz = -1; oz <= 1; ++oz)
-#pragma unroll
-                for (int oy = -1; oy <= 1; ++oy)
-#pragma unroll
-                for (int ox = -1; ox <= 1; ++ox) {
-                    const float4 qv = T.vel[tile_of(s.t0[0] + ox, s.t0[1] + oy, s.t0[2] + oz)];
-                    if (qv.w != 2.0f) continue;
-                    const v3 v = mk(qv.x, qv.y, qv.z);
-                    const float cw = (s.w[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z;
-                    nv = nv + cw * v;
-                    const v3 g = mk((d[ox + 1].x * s.w[oy + 1].y) * s.w[oz + 1].z,
-                                    (s.w[ox + 1].x * d[oy + 1].y) * s.w[oz + 1].z,
-                                    (s.w[ox + 1].x * s.w[oy + 1].y) * d[oz + 1].z);
-                    m3 o; o.c[0] = g.x * v; o.c[1] = g.y * v; o.c[2] = g.z * v;
-                    G = madd(G, o);
-                }
+                v3 nv; m3 G;
+                const bool finite = isfinite(s.f.x) && isfinite(s.f.y) && isfinite(s.f.z);
+                if (__builtin_expect(__ballot(!finite) == 0, 1)) g2p_explicit_nodes<true>(T, s, nv, G);
+                else g2p_explicit_nodes<false>(T, s, nv, G);
                 p.vel = nv;
                 p.dx = nv * P.dt;
                 p.D = mscale_r(mscale_r(G, P.dt), P.grad_scale);
