@@ -430,6 +430,8 @@ int make_params(mpm_ctx* c, SimParams& P) {
     if (c->ens) return fail(c, MPM_E_STATE, "this entry point is for single-scene contexts; use mpm_ensemble_*");
     const int rc = params_from(c, c->uni, c->cfg, c->k_cur, c->force_div_cur, false, P);
     P.per_particle_lame = c->per_particle_lame ? 1u : 0u;
+    // one scene: chunks longest first (a pass hands them out in list order and ends on the short ones)
+    P.chunk_lpt = 1u;
     return rc;
 }
 
@@ -575,7 +577,7 @@ int alloc_buffers(mpm_ctx* c) {
         (rc = dalloc(c, &c->relax, c->stride)) || (rc = dalloc(c, &c->key, c->stride)) || (rc = dalloc(c, &c->lofs, c->stride)) ||
         (rc = dalloc(c, &c->mask, c->stride)) || (rc = dalloc(c, &c->perm, c->stride)) ||
         (rc = dalloc(c, &c->kcount, K)) || (rc = dalloc(c, &c->kmask, K)) || (rc = dalloc(c, &c->kcursor, K)) ||
-        (rc = dalloc(c, &c->tile_sums, 4 * (size_t)c->ntiles)) || (rc = dalloc(c, &c->tile_ofs, 4 * (size_t)c->ntiles)) ||
+        (rc = dalloc(c, &c->tile_sums, SCAN_NS * (size_t)c->ntiles)) || (rc = dalloc(c, &c->tile_ofs, SCAN_NS * (size_t)c->ntiles)) ||
         (rc = dalloc(c, &c->kalloc, K)) || (rc = dalloc(c, &c->kslot, K)) ||
         (rc = dalloc(c, &c->run_key, c->max_runs)) || (rc = dalloc(c, &c->run_start, c->max_runs)) ||
         (rc = dalloc(c, &c->run_count, c->max_runs)) || (rc = dalloc(c, &c->run_mask, c->max_runs)) ||

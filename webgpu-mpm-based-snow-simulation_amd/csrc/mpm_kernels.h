@@ -51,6 +51,7 @@ enum Counter : int {
     C_NONFIN,        // particles with a non-finite position, counted by the keys pass of this substep
     C_NONFIN_LAST,   // the same count, latched by the scatter pass (which also clears C_NONFIN)
     C_SVD64,         // SVD plasticity: particles whose F_e was decomposed in FP64 this substep
+    C_NLONG,         // long work chunks (as many 256-row rounds as a full chunk; the rest are short)
     C_NCOUNTERS = 16
 };
 
@@ -65,6 +66,8 @@ struct SimParams {
     uint32_t blk_base, blk_cap; // batched scenes: dense block range [blk_base, blk_base + blk_cap)
     uint32_t n;                // particles
     uint32_t chunk;            // particles per work chunk
+    uint32_t chunk_lpt;        // chunk list longest first: every long chunk (key order), then the short
+                               // ones; else each run's chunks together (batched scenes)
     uint32_t cap;              // hash table slots
     uint32_t max_blocks, max_probes;
     // material / method
@@ -114,7 +117,7 @@ struct Bufs {
     uint32_t* key; uint32_t* mask; uint32_t* perm;
     uint32_t* lofs;            // per particle: its offset among the particles of its key (from the count atomic)
     uint32_t* kcount; uint32_t* kmask; uint32_t* kcursor;
-    uint32_t* tile_sums; uint32_t* tile_ofs;     // 4 per tile
+    uint32_t* tile_sums; uint32_t* tile_ofs;     // SCAN_NS per tile
     uint32_t* kalloc;          // per key: block touched by some particle's stencil (direct addressing)
     uint32_t* kslot;           // per key: dense block index or EMPTY (direct addressing)
     uint32_t* run_key; uint32_t* run_start; uint32_t* run_count; uint32_t* run_mask;
@@ -273,16 +276,17 @@ MPM_DEV uint32_t wave_incl_scan(uint32_t v) {
     for (int d = 1; d < 64; d <<= 1) { uint32_t t = __shfl_up(v, d, 64); if (l >= d) v += t; }
     return v;
 }
-// exclusive scan of three u32 streams over a 256-thread workgroup
-MPM_DEV void block_excl_scan4(uint32_t v[4], uint32_t ex[4], uint32_t tot[4], uint32_t (*wt)[4]) {
+// exclusive scan of NS u32 streams over a 256-thread workgroup (wt: [4 waves][NS] in LDS)
+template <int NS>
+MPM_DEV void block_excl_scan(uint32_t v[NS], uint32_t ex[NS], uint32_t tot[NS], uint32_t (*wt)[NS]) {
     const int w = threadIdx.x >> 6, l = lane_id();
-    uint32_t inc[4];
+    uint32_t inc[NS];
 #pragma unroll
-    for (int c = 0; c < 4; ++c) inc[c] = wave_incl_scan(v[c]);
-    if (l == 63) { wt[w][0] = inc[0]; wt[w][1] = inc[1]; wt[w][2] = inc[2]; wt[w][3] = inc[3]; }
+    for (int c = 0; c < NS; ++c) inc[c] = wave_incl_scan(v[c]);
+    if (l == 63) for (int c = 0; c < NS; ++c) wt[w][c] = inc[c];
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
+    for (int c = 0; c < NS; ++c) {
         uint32_t pre = 0, all = 0;
         for (int k = 0; k < 4; ++k) { if (k < w) pre += wt[k][c]; all += wt[k][c]; }
         ex[c] = pre + inc[c] - v[c];
@@ -290,11 +294,12 @@ MPM_DEV void block_excl_scan4(uint32_t v[4], uint32_t ex[4], uint32_t tot[4], ui
     }
     __syncthreads();
 }
-// Sum over the workgroup of v[4] (every thread gets the totals).
-MPM_DEV void block_sum4(uint32_t v[4], uint32_t (*wt)[4]) {
-    uint32_t ex[4], tot[4];
-    block_excl_scan4(v, ex, tot, wt);
-    for (int c = 0; c < 4; ++c) v[c] = tot[c];
+// Sum over the workgroup of v[NS] (every thread gets the totals).
+template <int NS>
+MPM_DEV void block_sum(uint32_t v[NS], uint32_t (*wt)[NS]) {
+    uint32_t ex[NS], tot[NS];
+    block_excl_scan<NS>(v, ex, tot, wt);
+    for (int c = 0; c < NS; ++c) v[c] = tot[c];
 }
 
 // ================================================================== binning
@@ -453,84 +458,103 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
 }
 
 // Scan over the key space, 256·KPT keys per tile (KPT consecutive keys per thread). Streams:
-// particle count, run flag, chunk count, allocated-block flag (direct addressing; 0 in hash-table mode).
+// particle count, run flag, long work chunks, allocated-block flag (direct addressing; 0 in
+// hash-table mode), short work chunks. A run of c particles has c / chunk full chunks and a
+// remainder chunk when chunk does not divide c; a chunk is long when it takes as many 256-row
+// rounds as a full one (the full ones and remainders above short_max), short otherwise.
 // KPT is 1 when that keeps the tile count within SCAN_OWN_MAX (k_scan_apply then sums the tile
 // offsets itself and k_scan_top is not launched), else 4.
 enum : uint32_t { SCAN_OWN_MAX = 512u };
-MPM_DEV void key_streams(const SimParams& P, const Bufs& B, uint32_t k, uint32_t v[4]) {
-    const uint32_t c = k < P.K ? B.kcount[k] : 0u;
-    v[0] += c; v[1] += c ? 1u : 0u; v[2] += (c + P.chunk - 1) / P.chunk;
-    if (P.direct_blocks && k < P.K) v[3] += B.kalloc[k];
+enum : int { SCAN_NS = 5 };
+MPM_DEV uint32_t short_max(const SimParams& P) { return ((P.chunk + 255u) / 256u - 1u) * 256u; }
+MPM_DEV void key_streams(const SimParams& P, uint32_t c, uint32_t alloc, uint32_t v[SCAN_NS]) {
+    const uint32_t r = c % P.chunk;
+    v[0] += c; v[1] += c ? 1u : 0u; v[2] += c / P.chunk + (r > short_max(P) ? 1u : 0u); v[3] += alloc;
+    v[4] += (r && r <= short_max(P)) ? 1u : 0u;
 }
 template <uint32_t KPT>
 __global__ __launch_bounds__(256) void k_scan_tiles(SimParams P, Bufs B) {
-    __shared__ uint32_t wt[4][4];
+    __shared__ uint32_t wt[4][SCAN_NS];
     const uint32_t base = blockIdx.x * 256u * KPT + threadIdx.x * KPT;
-    uint32_t v[4] = {0, 0, 0, 0};
-    for (uint32_t q = 0; q < KPT; ++q) key_streams(P, B, base + q, v);
-    block_sum4(v, wt);
-    if (threadIdx.x == 0) for (int c = 0; c < 4; ++c) B.tile_sums[4 * blockIdx.x + c] = v[c];
+    uint32_t v[SCAN_NS] = {0, 0, 0, 0, 0};
+    for (uint32_t q = 0; q < KPT; ++q) {
+        const uint32_t k = base + q;
+        key_streams(P, k < P.K ? B.kcount[k] : 0u, (P.direct_blocks && k < P.K) ? B.kalloc[k] : 0u, v);
+    }
+    block_sum<SCAN_NS>(v, wt);
+    if (threadIdx.x == 0) for (int c = 0; c < SCAN_NS; ++c) B.tile_sums[SCAN_NS * blockIdx.x + c] = v[c];
 }
-// Totals of the four streams: counters for the later passes. Direct addressing also settles the
-// block count and the blocks beyond max_blocks (dropped) here.
-MPM_DEV void publish_totals(const SimParams& P, const Bufs& B, const uint32_t tot[4]) {
-    B.counters[C_NVALID] = tot[0]; B.counters[C_NRUNS] = tot[1]; B.counters[C_NCHUNKS] = tot[2];
+// Totals of the streams: counters for the later passes. Direct addressing also settles the block
+// count and the blocks beyond max_blocks (dropped) here.
+MPM_DEV void publish_totals(const SimParams& P, const Bufs& B, const uint32_t tot[SCAN_NS]) {
+    B.counters[C_NVALID] = tot[0]; B.counters[C_NRUNS] = tot[1]; B.counters[C_NCHUNKS] = tot[2] + tot[4];
+    B.counters[C_NLONG] = tot[2];
     if (P.direct_blocks) { B.counters[C_NBLOCKS] = tot[3]; B.counters[C_DROPS] = tot[3] > P.max_blocks ? tot[3] - P.max_blocks : 0u; }
 }
 // Tile offsets when there are more tiles than one workgroup of k_scan_apply sums cheaply.
 __global__ __launch_bounds__(256) void k_scan_top(SimParams P, Bufs B, uint32_t ntiles) {
-    __shared__ uint32_t wt[4][4];
+    __shared__ uint32_t wt[4][SCAN_NS];
     const uint32_t per = (ntiles + 255u) / 256u;
     const uint32_t t0 = threadIdx.x * per;
-    uint32_t v[4] = {0, 0, 0, 0};
-    for (uint32_t t = t0; t < t0 + per && t < ntiles; ++t) for (int c = 0; c < 4; ++c) v[c] += B.tile_sums[4 * t + c];
-    uint32_t ex[4], tot[4];
-    block_excl_scan4(v, ex, tot, wt);
+    uint32_t v[SCAN_NS] = {0, 0, 0, 0, 0};
+    for (uint32_t t = t0; t < t0 + per && t < ntiles; ++t) for (int c = 0; c < SCAN_NS; ++c) v[c] += B.tile_sums[SCAN_NS * t + c];
+    uint32_t ex[SCAN_NS], tot[SCAN_NS];
+    block_excl_scan<SCAN_NS>(v, ex, tot, wt);
     for (uint32_t t = t0; t < t0 + per && t < ntiles; ++t)
-        for (int c = 0; c < 4; ++c) { B.tile_ofs[4 * t + c] = ex[c]; ex[c] += B.tile_sums[4 * t + c]; }
+        for (int c = 0; c < SCAN_NS; ++c) { B.tile_ofs[SCAN_NS * t + c] = ex[c]; ex[c] += B.tile_sums[SCAN_NS * t + c]; }
     if (threadIdx.x == 0) publish_totals(P, B, tot);
 }
 // Writes run descriptors (particle blocks), work chunks, scatter cursors and (direct addressing)
 // the dense block index of every key; zeroes the per-key accumulators for the next substep.
 // own_ofs: the tile offsets are summed here from tile_sums (few tiles), without k_scan_top.
+// Work chunks (run, sub): with P.chunk_lpt every long chunk in key order, then every short one (the
+// passes hand chunks out in list order, so a pass ends on the short ones); else each run's chunks
+// together (the batched scenes keep every scene's chunks contiguous).
 template <uint32_t KPT>
 __global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B, uint32_t ntiles, int own_ofs) {
-    __shared__ uint32_t wt[4][4];
-    uint32_t tofs[4];
+    __shared__ uint32_t wt[4][SCAN_NS];
+    uint32_t tofs[SCAN_NS], nlong;
     if (own_ofs) {
-        uint32_t v[4] = {0, 0, 0, 0}, all[4] = {0, 0, 0, 0};
+        uint32_t v[SCAN_NS] = {0, 0, 0, 0, 0}, all[SCAN_NS] = {0, 0, 0, 0, 0};
         for (uint32_t t = threadIdx.x; t < ntiles; t += blockDim.x)
-            for (int c = 0; c < 4; ++c) { const uint32_t x = B.tile_sums[4 * t + c]; all[c] += x; if (t < blockIdx.x) v[c] += x; }
-        block_sum4(v, wt);
-        block_sum4(all, wt);
-        for (int c = 0; c < 4; ++c) tofs[c] = v[c];
+            for (int c = 0; c < SCAN_NS; ++c) { const uint32_t x = B.tile_sums[SCAN_NS * t + c]; all[c] += x; if (t < blockIdx.x) v[c] += x; }
+        block_sum<SCAN_NS>(v, wt);
+        block_sum<SCAN_NS>(all, wt);
+        for (int c = 0; c < SCAN_NS; ++c) tofs[c] = v[c];
+        nlong = all[2];
         if (blockIdx.x == 0 && threadIdx.x == 0) publish_totals(P, B, all);
     } else {
-        for (int c = 0; c < 4; ++c) tofs[c] = B.tile_ofs[4 * blockIdx.x + c];
+        for (int c = 0; c < SCAN_NS; ++c) tofs[c] = B.tile_ofs[SCAN_NS * blockIdx.x + c];
+        nlong = B.counters[C_NLONG];
     }
     const uint32_t base = blockIdx.x * 256u * KPT + threadIdx.x * KPT;
-    uint32_t cnt[KPT], alc[KPT], v[4] = {0, 0, 0, 0};
+    uint32_t cnt[KPT], alc[KPT], v[SCAN_NS] = {0, 0, 0, 0, 0};
     for (uint32_t q = 0; q < KPT; ++q) {
         const uint32_t k = base + q;
-        const uint32_t c = k < P.K ? B.kcount[k] : 0u;
-        cnt[q] = c;
+        cnt[q] = k < P.K ? B.kcount[k] : 0u;
         alc[q] = (P.direct_blocks && k < P.K) ? B.kalloc[k] : 0u;
-        v[0] += c; v[1] += c ? 1u : 0u; v[2] += (c + P.chunk - 1) / P.chunk; v[3] += alc[q];
+        key_streams(P, cnt[q], alc[q], v);
     }
-    uint32_t ex[4], tot[4];
-    block_excl_scan4(v, ex, tot, wt);
-    uint32_t o0 = ex[0] + tofs[0], o1 = ex[1] + tofs[1], o2 = ex[2] + tofs[2], o3 = ex[3] + tofs[3];
+    uint32_t ex[SCAN_NS], tot[SCAN_NS];
+    block_excl_scan<SCAN_NS>(v, ex, tot, wt);
+    uint32_t o0 = ex[0] + tofs[0], o1 = ex[1] + tofs[1], of = ex[2] + tofs[2], o3 = ex[3] + tofs[3], op = ex[4] + tofs[4];
     for (uint32_t q = 0; q < KPT; ++q) {
         uint32_t k = base + q;
         if (k >= P.K) break;
         uint32_t c = cnt[q];
         if (c) {
-            uint32_t nch = (c + P.chunk - 1) / P.chunk;
+            const uint32_t r = c % P.chunk;
+            const uint32_t nf = c / P.chunk + (r > short_max(P) ? 1u : 0u), np = (r && r <= short_max(P)) ? 1u : 0u;
             B.kcursor[k] = o0;
             B.run_key[o1] = k; B.run_start[o1] = o0; B.run_count[o1] = c; B.run_mask[o1] = B.kmask[k];
-            for (uint32_t j = 0; j < nch; ++j) B.chunks[o2 + j] = make_uint2(o1, j);
+            if (P.chunk_lpt) {
+                for (uint32_t j = 0; j < nf; ++j) B.chunks[of + j] = make_uint2(o1, j);
+                if (np) B.chunks[nlong + op] = make_uint2(o1, nf);
+            } else {
+                for (uint32_t j = 0; j < nf + np; ++j) B.chunks[of + op + j] = make_uint2(o1, j);
+            }
             B.kcount[k] = 0u; B.kmask[k] = 0u;
-            o0 += c; o1 += 1; o2 += nch;
+            o0 += c; o1 += 1; of += nf; op += np;
         }
         if (P.direct_blocks) {
             uint32_t slot = EMPTY;
