@@ -379,6 +379,61 @@ MPM_DEV uint32_t block_key_at(const SimParams& P, int bx, int by, int bz, int j)
     bx += j % 3 - 1; by += (j / 3) % 3 - 1; bz += j / 9 - 1;
     return P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
 }
+// One particle of a lane in the keys pass: its key and mask, and its place in the wave's key groups.
+struct KeyLane {
+    bool valid;
+    uint32_t key, mask;
+    int leader;                 // lowest lane of the wave with the same key
+    uint32_t rank;              // lanes before this one with the same key
+    uint32_t lead_cnt, lead_m;  // on a leader: its key's lanes and the OR of their masks
+};
+// Key and touched-neighbour mask of particle i (active: i < n), stored per particle.
+MPM_DEV KeyLane key_of(const SimParams& P, const Bufs& B, bool active, uint32_t i, v3 pos) {
+    KeyLane a; a.valid = false; a.key = EMPTY; a.mask = 0u;
+    count_nonfinite(B, active, pos);
+    if (active) {
+        int c[3]; cell_number(P, pos, c);
+        a.mask = touched_mask(P, c);
+        int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
+        if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
+            a.key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+            a.valid = true;
+        } else if (P.direct_blocks && a.mask) {
+            // own block outside the block grid: the in-range blocks its stencil touches
+            for (int j = 0; j < 27; ++j)
+                if ((a.mask >> j) & 1u) B.kalloc[block_key_at(P, bx, by, bz, j)] = 1u;
+        }
+        B.key[i] = a.key; B.mask[i] = a.mask;
+    }
+    return a;
+}
+// Groups the wave's lanes by key: the lowest lane of each key leads it and holds the key's particle
+// count and mask OR; every lane keeps its leader and its rank among its peers.
+MPM_DEV void group_keys(KeyLane& a) {
+    uint64_t active = __ballot(a.valid);
+    a.leader = 0; a.rank = 0; a.lead_cnt = 0; a.lead_m = 0;
+    while (active) {
+        const int leader = __ffsll((unsigned long long)active) - 1;
+        const uint32_t lk = __shfl(a.key, leader, 64);
+        const bool peer = a.valid && a.key == lk;
+        const uint64_t peers = __ballot(peer);
+        const uint32_t m = wave_or(peer ? a.mask : 0u);
+        if (peer) { a.leader = leader; a.rank = (uint32_t)__popcll(peers & lanemask_lt()); }
+        if (lane_id() == leader) { a.lead_cnt = (uint32_t)__popcll(peers); a.lead_m = m; }
+        active &= ~peers;
+    }
+}
+// Each (key, neighbour) pair is flagged once: by the wave whose OR first set its bit (fresh, on the
+// key's leader lane).
+MPM_DEV void flag_fresh(const Bufs& B, const KeyLane& a, uint32_t fresh, uint32_t koff) {
+    uint64_t fl = __ballot(fresh != 0u);
+    while (fl) {
+        const int l = __ffsll((unsigned long long)fl) - 1;
+        const uint32_t fk = __shfl(a.key, l, 64), ff = __shfl(fresh, l, 64);
+        if (lane_id() < 27 && ((ff >> lane_id()) & 1u)) B.kalloc[fk + koff] = 1u;
+        fl &= fl - 1ull;
+    }
+}
 // Per particle: own-block key, touched-neighbour mask, offset among its key's particles; per key:
 // particle count and OR of masks (countParticlesPerBlock.wgsl:7-23 + the block set of
 // mapAffectedBlocks.wgsl:64-86). Atomics are aggregated per wave over lanes that share a key (input
@@ -393,67 +448,33 @@ __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
         for (uint32_t s = gi; s < P.cap; s += gsz) B.table[s] = EMPTY;
     if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; B.counters[C_SVD64] = 0; }
     const uint32_t n = P.n;
-    const uint32_t nloop = (n + gsz - 1) / gsz;
     // key offset of the neighbour block this lane flags (lane j < 27: offset j, x fastest); a
     // touched neighbour is inside the block grid, so key + offset is its key
     const int jl = lane_id();
     const uint32_t koff = (uint32_t)((jl % 3 - 1) + P.nb[0] * ((jl / 3) % 3 - 1 + P.nb[1] * (jl / 9 - 1)));
+    // two particles per lane and iteration (rows i0 and i0 + gsz, each read coalesced), so the key
+    // atomics of both are in flight together: one round trip of the leaders' atomics per iteration.
+    // Every lane runs every iteration (the wave-wide groupings need all lanes).
+    const uint32_t nloop = (n + 2u * gsz - 1u) / (2u * gsz);
     for (uint32_t it = 0; it < nloop; ++it) {
-        uint32_t i = it * gsz + gi;
-        bool valid = false;
-        uint32_t key = EMPTY, mask = 0;
-        v3 pos = {0.0f, 0.0f, 0.0f};
-        if (i < n) pos = ld3(B.cur, B.stride, PX, i);
-        count_nonfinite(B, i < n, pos);
-        if (i < n) {
-            int c[3]; cell_number(P, pos, c);
-            mask = touched_mask(P, c);
-            int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
-            if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
-                key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
-                valid = true;
-            } else if (P.direct_blocks && mask) {
-                // own block outside the block grid: the in-range blocks its stencil touches
-                for (int j = 0; j < 27; ++j)
-                    if ((mask >> j) & 1u) B.kalloc[block_key_at(P, bx, by, bz, j)] = 1u;
-            }
-            B.key[i] = key; B.mask[i] = mask;
-        }
-        // group the wave's lanes by key: the lowest lane of each key leads it and holds the key's
-        // particle count and mask OR; every lane keeps its leader and its rank among its peers
-        uint64_t active = __ballot(valid);
-        int my_leader = 0;
-        uint32_t my_rank = 0, lead_cnt = 0, lead_m = 0;
-        while (active) {
-            const int leader = __ffsll((unsigned long long)active) - 1;
-            const uint32_t lk = __shfl(key, leader, 64);
-            const bool peer = valid && key == lk;
-            const uint64_t peers = __ballot(peer);
-            const uint32_t m = wave_or(peer ? mask : 0u);
-            if (peer) { my_leader = leader; my_rank = (uint32_t)__popcll(peers & lanemask_lt()); }
-            if (lane_id() == leader) { lead_cnt = (uint32_t)__popcll(peers); lead_m = m; }
-            active &= ~peers;
-        }
-        // the leaders' atomics of all the wave's keys issue together (one round trip per wave); the
-        // count atomic's old value is the base of the key's offsets
-        const bool lead = valid && my_leader == lane_id();
-        uint32_t b = 0, fresh = 0;
-        if (lead) {
-            b = atomicAdd(&B.kcount[key], lead_cnt);
-            fresh = lead_m & ~atomicOr(&B.kmask[key], lead_m);
-        }
-        b = __shfl(b, my_leader, 64);
-        if (valid) B.lofs[i] = b + my_rank;
-        // each (key, neighbour) pair is flagged once: by the wave whose OR first set its bit
-        if (P.direct_blocks) {
-            uint64_t fl = __ballot(fresh != 0u);
-            while (fl) {
-                const int l = __ffsll((unsigned long long)fl) - 1;
-                const uint32_t fk = __shfl(key, l, 64), ff = __shfl(fresh, l, 64);
-                if (jl < 27 && ((ff >> jl) & 1u)) B.kalloc[fk + koff] = 1u;
-                fl &= fl - 1ull;
-            }
-        }
+        const uint32_t i0 = it * 2u * gsz + gi, i1 = i0 + gsz;
+        v3 pos0 = {0.0f, 0.0f, 0.0f}, pos1 = {0.0f, 0.0f, 0.0f};
+        if (i0 < n) pos0 = ld3(B.cur, B.stride, PX, i0);
+        if (i1 < n) pos1 = ld3(B.cur, B.stride, PX, i1);
+        KeyLane a = key_of(P, B, i0 < n, i0, pos0);
+        KeyLane b = key_of(P, B, i1 < n, i1, pos1);
+        group_keys(a);
+        group_keys(b);
+        // the count atomic's old value is the base of the key's offsets
+        uint32_t ba = 0, bb = 0, fa = 0, fb = 0;
+        if (a.valid && a.leader == lane_id()) { ba = atomicAdd(&B.kcount[a.key], a.lead_cnt); fa = atomicOr(&B.kmask[a.key], a.lead_m); }
+        if (b.valid && b.leader == lane_id()) { bb = atomicAdd(&B.kcount[b.key], b.lead_cnt); fb = atomicOr(&B.kmask[b.key], b.lead_m); }
+        fa = a.lead_m & ~fa; fb = b.lead_m & ~fb;
+        ba = __shfl(ba, a.leader, 64);
+        bb = __shfl(bb, b.leader, 64);
+        if (a.valid) B.lofs[i0] = ba + a.rank;
+        if (b.valid) B.lofs[i1] = bb + b.rank;
+        if (P.direct_blocks) { flag_fresh(B, a, fa, koff); flag_fresh(B, b, fb, koff); }
     }
 }
 
