@@ -11,7 +11,9 @@ achieved = the reference formulation's algorithmic bytes for the work of one lau
 duration: a PBMPM iteration re-streams {x, m, F_e, F_p, D} and writes {D, v}, 172 B per particle
 (the SURVEY's per-unit figure; 268 + 172 k B per particle-substep in all), so one k_pbmpm<MID>
 launch is 172 B x binned particles. The implemented layout's own compulsory bytes for the kernel
-are reported beside it. Fields read from committed rocprofv3 summaries carry their source file.
+are reported beside it. bound names what limits the kernel by its counters (VALU issue for the
+transfer passes); fields read from committed rocprofv3 summaries carry their source file, and the
+LDS atomic peak the atomics fields are priced on is measured in the run (mpm_selftest kind 11).
 cpu_baseline: the oracle (FP32-faithful CPU restatement of the reference substep) with OpenMP
 over the host's cores, 10 warm-up substeps, then the median of 5 timed substeps of the same 1M scene.
 """
@@ -268,6 +270,20 @@ def copy_calibration(mpm, nbytes):
     assert rc == 0 and out[-1] == 1.0
 
 
+def lds_atomic_rate(mpm, iters=65536):
+    """Conflict-free LDS atomic throughput of this GPU (mpm_selftest kind 11): wave-wide ds_add_u32
+    instructions per second over all CUs, and clocks per instruction per CU at the 2.4 GHz peak clock."""
+    import ctypes
+    lib = mpm.load_library()
+    out = np.zeros(3, np.float32)
+    rc = lib.mpm_selftest(11, None, None, out.ctypes.data_as(ctypes.c_void_p), iters)
+    if rc != 0 or out[0] <= 0:
+        return None
+    rate = float(out[1]) / (float(out[0]) * 1e-3)
+    return {"wave_instr_per_s": rate, "clocks_per_wave_instr_per_cu": float(out[2]) * CLOCK_HZ / rate,
+            "measured": "mpm_selftest kind 11 (conflict-free ds_add_u32 loop, 8 workgroups per CU)"}
+
+
 def profile_entry(name, kernel):
     """(entry, source) of a kernel in a committed rocprofv3 summary under profiles/ (None if absent):
     these numbers come from separate counter runs, not from this run."""
@@ -281,8 +297,9 @@ def profile_entry(name, kernel):
     return d.get(kernel), d.get("_source", f"profiles/{name}")
 
 
-def profiled_fields(kernel, avg_s):
-    """Counter-derived figures of the dominant kernel, each with the committed summary it is read from."""
+def profiled_fields(kernel, avg_s, lds_rate=None):
+    """Counter-derived figures of the dominant kernel, each with the committed summary it is read from
+    (the LDS atomic peak is measured in this run when lds_rate is given)."""
     out = {}
     tr, src = profile_entry("traffic.json", kernel)
     if tr:
@@ -298,7 +315,10 @@ def profiled_fields(kernel, avg_s):
                        "issue_utilisation_at_this_run_duration": va["valu_wave_instructions_per_launch"] * 4 / (SIMDS * CLOCK_HZ * avg_s)}
     at, src = profile_entry("atomics.json", kernel)
     if at:
-        pk = profile_entry("atomics.json", "_peaks")[0] or {}
+        pk = dict(profile_entry("atomics.json", "_peaks")[0] or {})
+        if lds_rate:
+            pk["lds_atomic_wave_instr_per_s"] = lds_rate["wave_instr_per_s"]
+            pk["lds_measured_in_this_run"] = lds_rate
         g, l = at["tcc_atomic_requests"] / avg_s, at["lds_atomic_wave_instr"] / avg_s
         out["atomics"] = {"source": src, "global_requests_per_launch": at["tcc_atomic_requests"], "global_per_s": g,
                           "global_frac": g / pk["global_atomic_requests_per_s"] if pk else None,
@@ -418,7 +438,7 @@ def main():
     achieved = algo_bytes / avg_s / 1e9
     own_bytes = OWN_LAYOUT_BYTES_PER_PARTICLE[dom] * binned
     eager_ms = sum(per_substep.values())
-    prof = profiled_fields(dom, avg_s)
+    prof = profiled_fields(dom, avg_s, lds_atomic_rate(mpm))
     traffic = (prof["traffic_fetch_bytes"] + prof["traffic_write_bytes"]) if "traffic_fetch_bytes" in prof else None
 
     line = {
@@ -437,15 +457,15 @@ def main():
         "config": {"workload": "pbmpm_two_spheres", "particles": n, "grid": "128^3 sparse, 4^3 blocks addressed directly by block key",
                    "pbmpm_iterations": k, "dt": sc["sim"]["dt"], "collider": "12-triangle box, BVH",
                    "parallelism": "single GPU"},
-        # priced on HBM bandwidth; the counters show the kernel's limiter is VALU issue (see "limiter")
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        # the counters show the kernel's limiter is VALU issue ("valu": issue utilisation); achieved /
+        # peak / frac price the reference formulation's HBM bytes for the same work against 8 TB/s
+        "roofline": {"bound": "valu", "priced_on": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": dom, "avg_launch_us": avg_s * 1e6, "launches_timed": dom_cnt,
                      "algorithmic_bytes_per_particle": REFERENCE_BYTES_PER_PARTICLE[dom], "particles_per_launch": binned,
                      "algorithmic_bytes_per_launch": algo_bytes,
                      "own_layout_bytes_per_launch": own_bytes, "own_layout_achieved": own_bytes / avg_s / 1e9,
                      "own_layout_frac": own_bytes / avg_s / 1e9 / HBM_PEAK_GBS,
-                     "limiter": "valu" if "valu" in prof else None,
                      "step_frac_reference_bytes": (STEP_ALGO_BYTES(k) * value / 1e9) / HBM_PEAK_GBS,
                      **prof},
         "kernel_ms_per_substep": {nm: round(v, 5) for nm, v in sorted(per_substep.items(), key=lambda x: -x[1])},

@@ -229,8 +229,9 @@ int  mpm_dump_particle_blocks(mpm_ctx* ctx, int32_t* block_xyz, uint32_t* counts
 /* Device evaluation of the step's scalar / 3x3 arithmetic for bitwise comparison with a host
  * restatement (no reference counterpart). kind: 0 exp(a) 1 log(a) 2 pow(a,b) 3 sqrt(a) 4 a/b
  * 5 hash3(bits a, bits b, index) 6 det3 7 inverse3 8 polar rotation 9 i32(a) 10 copy (a plain
- * device copy of n floats, for counter calibration); matrix kinds take and return 9 column-major
- * floats per item. */
+ * device copy of n floats, for counter calibration) 11 LDS atomic throughput (n iterations of a
+ * conflict-free wave-wide LDS add on every wave of 8 workgroups per CU; a and b unused, out = {ms,
+ * wave-instructions, CUs}); matrix kinds take and return 9 column-major floats per item. */
 int  mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n);
 /* Whether P2G forms i32((x / d) * scale) with the reciprocal + FMA quotient: it does only after
  * a device check over all 2^32 float x found it equal to the IEEE-division form (no reference

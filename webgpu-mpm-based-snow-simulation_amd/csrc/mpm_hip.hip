@@ -1265,6 +1265,31 @@ int mpm_divisor_fast(mpm_ctx* c, float d, float scale, int* fast) {
 }
 
 int mpm_selftest(int kind, const float* a, const float* b, float* out, size_t n) {
+    if (kind == 11) {
+        // LDS atomic throughput: n iterations of a conflict-free ds_add_u32 per wave, 8 workgroups
+        // of 256 per CU; out = {elapsed ms, wave-instructions, CUs}
+        if (!out || n == 0 || n > (1u << 20)) return MPM_E_INVALID;
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return MPM_E_HIP;
+        const uint32_t blocks = (uint32_t)ncu * 8u;
+        uint32_t* dout = nullptr;
+        if (hipMalloc(&dout, blocks * 4) != hipSuccess) return MPM_E_NOMEM;
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0); hipEventCreate(&e1);
+        float ms = 0.0f;
+        hipError_t e = hipSuccess;
+        for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {   // the second launch is the one timed
+            hipEventRecord(e0, 0);
+            k_lds_rate<<<blocks, 256, 0, 0>>>((uint32_t)n, dout);
+            hipEventRecord(e1, 0);
+            e = hipEventSynchronize(e1);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        }
+        hipEventDestroy(e0); hipEventDestroy(e1); hipFree(dout);
+        out[0] = ms; out[1] = (float)((double)blocks * 4.0 * (double)n); out[2] = (float)ncu;
+        return e == hipSuccess ? MPM_OK : MPM_E_HIP;
+    }
     if (kind < 0 || kind > 10 || !a || !out || n == 0 || n > (1u << 25)) return MPM_E_INVALID;
     if ((kind == 2 || kind == 4 || kind == 5) && !b) return MPM_E_INVALID;
     const size_t in_w = (kind >= 6 && kind <= 8) ? 9 : 1;

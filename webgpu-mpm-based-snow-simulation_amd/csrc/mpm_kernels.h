@@ -1867,6 +1867,17 @@ __global__ __launch_bounds__(256) void k_verify_fixed_div(float d, float rd, flo
 // Elementwise evaluation of the device arithmetic for bitwise checks against the host.
 // kind: 0 exp(a) 1 log(a) 2 pow(a,b) 3 sqrt(a) 4 a/b 5 hash3(bits a, bits b, i) 6 det3
 //       7 inverse3 8 polar rotation 9 f2i(a) 10 copy (matrices: 9 floats per item, column-major)
+// LDS atomic throughput: every wave issues `iters` ds_add_u32 whose 64 lanes hit 64 distinct banks.
+__global__ __launch_bounds__(256) void k_lds_rate(uint32_t iters, uint32_t* out) {
+    __shared__ uint32_t acc[256];
+    acc[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t a = (uint32_t)lane_id();
+#pragma unroll 8
+    for (uint32_t it = 0; it < iters; ++it) atomicAdd(&acc[a], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = acc[0];
+}
 __global__ void k_selftest(int kind, const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ out, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
