@@ -301,8 +301,9 @@ int  mpm_ensemble_download(mpm_ctx* ctx, int scene, void* aos192, size_t n);
 int  mpm_ensemble_step(mpm_ctx* ctx, int n_substeps);
 int  mpm_ensemble_set_tuning(mpm_ctx* ctx, const mpm_tuning* t);
 int  mpm_ensemble_get_tuning(mpm_ctx* ctx, mpm_tuning* t);
-/* Autotuner: times each candidate (chunk size {256, 512, 1024} x workgroups per CU {1..4} x
- * persistent / per phase, 24 in all) for
+/* Autotuner: times each candidate of the one-launch-per-phase schedule (chunk size {512, 1024, 2048}
+ * x workgroups per CU {2, 3, 4}, 9 in all; the persistent schedule stays selectable by
+ * mpm_ensemble_set_tuning) for
  * trial_substeps substeps on the live scenes, restores the state after every trial, and keeps
  * the fastest. Candidates and their times go to out[0..cap), their count to *n_out. */
 int  mpm_ensemble_autotune(mpm_ctx* ctx, int trial_substeps, mpm_tuning* out, int cap, int* n_out, mpm_tuning* best);

@@ -319,8 +319,11 @@ MPM_DEV void ens_clear_slice(const SimParams& P, const Bufs& B, const Ens& E, ui
 }
 
 // The transfer phases of every scene. phase < 0: persistent, all phases with dependency waits;
-// phase >= 0: only that phase's tasks (the launch boundary orders the phases). OCC: resident
-// workgroups per CU the register allocation targets (2, 3 or 4; the autotuner measures which pays).
+// phase >= 0: only that phase's tasks (the launch boundary orders the phases). Tasks are dequeued
+// from an atomic counter by the resident workgroups, which load the task tables into LDS once (an
+// oversubscribed grid with one task per workgroup, reloading them, measured 4% slower). OCC:
+// resident workgroups per CU the register allocation targets (2, 3 or 4; the autotuner measures
+// which pays).
 template <int OCC>
 __global__ __launch_bounds__(256, OCC) void k_ens_transfer(Bufs B, Ens E, int phase) {
     __shared__ Tile T;

@@ -148,8 +148,8 @@ struct mpm_ctx {
     uint32_t *d_part_base = nullptr, *d_key_base = nullptr, *d_nphase = nullptr, *d_method = nullptr, *d_nblk = nullptr;
     uint32_t* d_rbase = nullptr;      // batched scenes, direct addressing: first block-list entry per scene
     uint32_t *d_cbeg = nullptr, *d_seg = nullptr, *d_done = nullptr, *d_ctl = nullptr;
-    int ens_persistent = 1;
-    uint32_t ens_wg_per_cu = 2;
+    int ens_persistent = 0;
+    uint32_t ens_wg_per_cu = 4;
     bool ens_graph_ok[2] = {false, false};
     float* snap_planes = nullptr;
     uint32_t* snap_pid = nullptr;
@@ -1652,7 +1652,7 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
     c->n = (uint32_t)n;
     c->stride = (c->n + 63u) & ~63u;
     c->K = (uint32_t)K;
-    c->chunk = cfgs[0].chunk_particles ? cfgs[0].chunk_particles : 512u;
+    c->chunk = cfgs[0].chunk_particles ? cfgs[0].chunk_particles : 1024u;   // the autotuner's usual choice
     c->max_blocks = (uint32_t)K;
     c->cap = cfgs[0].hash_map_size ? cfgs[0].hash_map_size
                                    : next_prime((uint32_t)std::min<uint64_t>(2ull * std::min<uint64_t>(K, 8ull * n) + 1, 0xFFFFFF00ull));
@@ -1726,7 +1726,7 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
     }
     if (rc) { release(c); delete c; return rc; }
     c->ens_loaded.assign(n_scenes, false);
-    c->ens_wg_per_cu = cfgs[0].grid_workgroups ? std::max<uint32_t>(1u, cfgs[0].grid_workgroups / (uint32_t)c->n_cu) : 2u;
+    c->ens_wg_per_cu = cfgs[0].grid_workgroups ? std::max<uint32_t>(1u, cfgs[0].grid_workgroups / (uint32_t)c->n_cu) : 4u;
     *out = c;
     return MPM_OK;
 }
@@ -1837,12 +1837,14 @@ int mpm_ensemble_autotune(mpm_ctx* c, int trial_substeps, mpm_tuning* out, int c
     HIPCHK(c, hipMemcpy(c->snap_fpid, c->fpid, (size_t)c->stride * 48, hipMemcpyDeviceToDevice));
     const int cur0 = c->cur;
     const uint64_t sub0 = c->substeps;
-    const uint32_t chunks[] = {256, 512, 1024};
-    const uint32_t wgs[] = {1, 2, 3, 4};
+    // one launch per phase only: the persistent schedule (selectable with mpm_ensemble_set_tuning)
+    // measured 3.8-7 ms per substep of the 64-scene workload against 2.0 ms, its agent-scope
+    // release / acquire per dependency costing more than the launches it saves
+    const uint32_t chunks[] = {512, 1024, 2048};
+    const uint32_t wgs[] = {2, 3, 4};
     std::vector<mpm_tuning> cand;
-    for (int pers = 1; pers >= 0; --pers)
-        for (uint32_t ch : chunks)
-            for (uint32_t w : wgs) { mpm_tuning t{}; t.chunk_particles = ch; t.workgroups_per_cu = w; t.persistent = pers; cand.push_back(t); }
+    for (uint32_t ch : chunks)
+        for (uint32_t w : wgs) { mpm_tuning t{}; t.chunk_particles = ch; t.workgroups_per_cu = w; t.persistent = 0; cand.push_back(t); }
     hipEvent_t e0, e1;
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));

@@ -488,7 +488,7 @@ class MpmEnsemble:
     def sync(self):
         check(self.lib, self.ctx, self.lib.mpm_sync(self.ctx))
 
-    def set_tuning(self, chunk_particles=512, workgroups_per_cu=2, persistent=True):
+    def set_tuning(self, chunk_particles=1024, workgroups_per_cu=4, persistent=False):
         t = _lib.MpmTuning(chunk_particles, workgroups_per_cu, 1 if persistent else 0, 0.0)
         check(self.lib, self.ctx, self.lib.mpm_ensemble_set_tuning(self.ctx, ctypes.byref(t)))
 
