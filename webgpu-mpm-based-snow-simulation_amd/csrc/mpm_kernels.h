@@ -1123,6 +1123,8 @@ struct Tile {
     uint32_t start, count, center_ok;
     // FIRST's row order (cell-interleaved): particles per cell, one buffer per round modulo 3
     uint32_t ocnt[3][64];
+    uint32_t xold[256];    // FIRST: previous-order row of the particle in each row slot of the round
+    float xpos[3][256];    //        and its position
     uint32_t qcnt, qbase;  // LAST: collider-queue entries of the round, and their first slot
 };
 
@@ -1527,13 +1529,21 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
             float resid = 0.0f;
             if (MODE == PB_FIRST) {
                 const bool mine = i < T.count;
-                const uint32_t old = old_next;
+                uint32_t old = old_next;
                 old_next = i + blockDim.x < T.count ? B.perm[q + blockDim.x] : 0u;
-                const v3 pos = mine ? ld3(B.cur, sd, PX, old) : mk(0, 0, 0);
-                // row of this particle in nxt: its slot in the cell-interleaved order of the round
-                // (allocated runs), else its sorted row
-                uint32_t qd = q;
-                if (ok) qd = T.start + r * blockDim.x + round_slot<POW2>(P, T, r, mine, pos);
+                v3 pos = mine ? ld3(B.cur, sd, PX, old) : mk(0, 0, 0);
+                // rows of an allocated run go to nxt in the cell-interleaved order of the round: the
+                // particles are exchanged through LDS so that the lane of row slot s works on the
+                // particle written there (its P2G deposits then come from distinct cells, as in the
+                // later passes, and its row stores are contiguous); a run whose own block is not
+                // allocated keeps its sorted order
+                const uint32_t qd = q;
+                if (ok) {
+                    const uint32_t slot = round_slot<POW2>(P, T, r, mine, pos);
+                    if (mine) { T.xold[slot] = old; T.xpos[0][slot] = pos.x; T.xpos[1][slot] = pos.y; T.xpos[2][slot] = pos.z; }
+                    __syncthreads();
+                    if (mine) { old = T.xold[threadIdx.x]; pos = mk(T.xpos[0][threadIdx.x], T.xpos[1][threadIdx.x], T.xpos[2][threadIdx.x]); }
+                }
                 if (!mine) continue;
                 const float mass = ld(B.cur, sd, PM, old);
                 const m3 Fe = ldm(B.cur, sd, FE, old);
