@@ -214,6 +214,13 @@ int  mpm_splat_noalloc(mpm_ctx* ctx, int kind, const uint32_t* res, uint32_t* ho
  * context's stream and returns; mpm_sync waits. */
 int  mpm_step(mpm_ctx* ctx, int method, int n_substeps);
 int  mpm_sync(mpm_ctx* ctx);
+/* No reference counterpart (the reference creates its compute pipelines up front,
+ * GpuMpmPipelineManager.ts:42-66): captures, without running anything, the hipGraphs mpm_step
+ * replays for `method` with the current uniforms and buffers: one substep, and a run of 8 substeps
+ * (mpm_step replays a run from one graph when it is captured, or when a call has 32 or more
+ * substeps to go), for both buffer parities. Lets a caller keep graph capture out of a timed or
+ * latency-bound call. A no-op without use_graph. */
+int  mpm_prepare_graphs(mpm_ctx* ctx, int method);
 
 /* Test hooks (no reference counterpart). mpm_debug_stage runs one substep prefix:
  * explicit: 0 binning, 1 +P2G, 2 +grid update, >=3 full; PBMPM: 0 binning, s in 1..k the grid

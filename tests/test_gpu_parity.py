@@ -202,6 +202,33 @@ def test_graph_replay_matches_eager(mpm, orc):
     a.close(); b.close()
 
 
+def test_multi_substep_graph_matches_eager_and_oracle(mpm, orc):
+    """Runs of 8 substeps replay one graph holding all of them (both buffer parities captured in
+    sequence); remainders take the one-substep graph. With the graphs prepared up front, 19 substeps
+    in calls of 11 and 8 equal 19 eager substeps and the oracle; without, a call of 40 captures the
+    run graph itself and equals 40 eager substeps. Both methods."""
+    for method, sc in ((1, _small_pbmpm(mpm, n=6000, seed=21)), (0, mpm.scenes.scene("explicit_small", n_particles=6000, seed=22))):
+        a, o = make_pair(mpm, orc, sc, use_graph=1)
+        b, _ = make_pair(mpm, orc, sc, use_graph=0)
+        c, _ = make_pair(mpm, orc, sc, use_graph=1)
+        a.prepare_graphs(method)
+        a.step(method, 11)
+        a.step(method, 8)
+        b.step(method, 19)
+        o.step(method, 19)
+        x, y = a.download(), b.download()
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8)), f"method {method}: graph runs differ from eager"
+        assert_particles_equal(mpm, x, o.particles(), f"method {method}, 19 substeps through the graphs")
+        assert a.stats()["substeps"] == 19
+        c.step(method, 40)
+        b.step(method, 21)
+        assert np.array_equal(c.download().view(np.uint8), b.download().view(np.uint8)), f"method {method}: 40-substep call differs"
+        assert c.stats()["substeps"] == 40
+        a.close()
+        b.close()
+        c.close()
+
+
 def test_divisor_fast_path_verified(mpm):
     """P2G's reciprocal+FMA quotient is enabled only for (dt, scale) pairs whose exhaustive device
     check (all 2^32 float numerators) matched i32((x / dt) * scale); the scene time steps pass."""

@@ -117,6 +117,7 @@ _SIGS = [
     ("mpm_download_particles", ctypes.c_int, [_P, _P, _SZ]),
     ("mpm_step", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     ("mpm_sync", ctypes.c_int, [_P]),
+    ("mpm_prepare_graphs", ctypes.c_int, [_P, ctypes.c_int]),
     ("mpm_debug_stage", ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int]),
     ("mpm_dump_grid", ctypes.c_int, [_P, _P, _P, _SZ, ctypes.POINTER(_SZ)]),
     ("mpm_dump_blocks", ctypes.c_int, [_P, _P, _SZ, ctypes.POINTER(_SZ)]),

@@ -5,7 +5,9 @@
 // src/lib/gpu/uniforms/GpuUniformsBufferManager.ts (uniform blob) and
 // src/lib/gpu/collider/GpuColliderBufferManager.ts:79-350 (collider packing, BVH build).
 // One substep is a fixed sequence of kernels on the context's stream; with use_graph it is
-// captured once per (method, buffer parity, parameters) into a hipGraph and replayed.
+// captured once per (method, buffer parity, parameters) into a hipGraph and replayed, and runs of
+// GRAPH_SUBSTEPS substeps are replayed from one graph holding that many (a graph launch costs ~8 us
+// between replays; inside a graph the kernels follow each other without gaps).
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstring>
@@ -94,11 +96,12 @@ struct mpm_ctx {
     uint32_t coll_num_indices = 0;
     int last_grid = 0;
     bool has_particles = false;
-    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    hipGraph_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    SimParams gparams[2][2];
-    Bufs gbufs[2][2];                 // device addresses a captured graph's kernels were given
-    MlpBufs gmlp[2][2];
+    // [method][buffer parity][0: one substep, 1: GRAPH_SUBSTEPS substeps] (batched scenes: [0][parity][0])
+    hipGraphExec_t gexec[2][2][2] = {};
+    hipGraph_t graph[2][2][2] = {};
+    SimParams gparams[2][2][2];
+    Bufs gbufs[2][2][2];              // device addresses a captured graph's kernels were given
+    MlpBufs gmlp[2][2][2];
     bool timing = false;
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   // phases of a substep; [4], [5] the whole call
     mpm_timings tm{};
@@ -162,6 +165,8 @@ struct mpm_ctx {
 namespace {
 
 const char* k_version = "libmpm_hip 0.1 (gfx950)";
+// substeps per graph for runs of substeps (even: the buffer parity after the graph is the one before)
+enum { GRAPH_SUBSTEPS = 8 };
 const char* k_kernel_names[MPM_KERNEL_COUNT] = {
     "k_keys_count", "k_scan_tiles", "k_scan_top", "k_scan_apply", "k_scatter", "k_insert", "k_compact", "k_nbr",
     "k_pbmpm<FIRST>", "k_pbmpm<MID>", "k_pbmpm<LAST>", "k_explicit_p2g", "k_grid_update", "k_explicit_g2p", "k_straggle",
@@ -542,27 +547,40 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
 // A captured substep holds its kernels' arguments by value: the parameters and every device
 // address. It is replayed only while both are unchanged; a buffer freed and reallocated since
 // (a new collider, residual tracking or network buffers switched on) forces a new capture.
-int ensure_graph(mpm_ctx* c, int method, const SimParams& P) {
+// A captured graph for (method, current parity, many) that these parameters and buffers can replay.
+bool graph_valid(mpm_ctx* c, int method, const SimParams& P, int many) {
     const int par = c->cur;
     const Bufs B = make_bufs(c);
     const MlpBufs M = mlp_bufs(c);
-    if (c->gexec[method][par] && std::memcmp(&c->gparams[method][par], &P, sizeof(SimParams)) == 0 &&
-        std::memcmp(&c->gbufs[method][par], &B, sizeof(Bufs)) == 0 && std::memcmp(&c->gmlp[method][par], &M, sizeof(MlpBufs)) == 0)
-        return MPM_OK;
-    if (c->gexec[method][par]) { hipGraphExecDestroy(c->gexec[method][par]); c->gexec[method][par] = nullptr; }
-    if (c->graph[method][par]) { hipGraphDestroy(c->graph[method][par]); c->graph[method][par] = nullptr; }
+    return c->gexec[method][par][many] && std::memcmp(&c->gparams[method][par][many], &P, sizeof(SimParams)) == 0 &&
+           std::memcmp(&c->gbufs[method][par][many], &B, sizeof(Bufs)) == 0 &&
+           std::memcmp(&c->gmlp[method][par][many], &M, sizeof(MlpBufs)) == 0;
+}
+// many = 0: a graph of one substep from the current buffer parity; 1: of GRAPH_SUBSTEPS substeps
+// (an even count, so the parity after it is the one before).
+int ensure_graph(mpm_ctx* c, int method, const SimParams& P, int many = 0) {
+    if (graph_valid(c, method, P, many)) return MPM_OK;
+    const int par = c->cur;
+    const Bufs B = make_bufs(c);
+    const MlpBufs M = mlp_bufs(c);
+    if (c->gexec[method][par][many]) { hipGraphExecDestroy(c->gexec[method][par][many]); c->gexec[method][par][many] = nullptr; }
+    if (c->graph[method][par][many]) { hipGraphDestroy(c->graph[method][par][many]); c->graph[method][par][many] = nullptr; }
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_substep(c, method, P, -1, false);
+    const int n = many ? GRAPH_SUBSTEPS : 1;
+    for (int i = 0; i < n; ++i) {
+        enqueue_substep(c, method, P, -1, false);
+        if (many) c->cur ^= 1;        // the next substep of the graph reads what this one wrote
+    }
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamEndCapture(c->stream, &g));
     hipGraphExec_t e = nullptr;
     hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
     if (r != hipSuccess) { hipGraphDestroy(g); return fail(c, MPM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(r)); }
-    c->graph[method][par] = g;
-    c->gexec[method][par] = e;
-    c->gparams[method][par] = P;
-    c->gbufs[method][par] = B;
-    c->gmlp[method][par] = M;
+    c->graph[method][par][many] = g;
+    c->gexec[method][par][many] = e;
+    c->gparams[method][par][many] = P;
+    c->gbufs[method][par][many] = B;
+    c->gmlp[method][par][many] = M;
     return MPM_OK;
 }
 
@@ -649,9 +667,9 @@ void enqueue_ens_substep(mpm_ctx* c, bool full) {
 
 int ens_graph(mpm_ctx* c) {
     const int par = c->cur;
-    if (c->gexec[0][par] && c->ens_graph_ok[par]) return MPM_OK;
-    if (c->gexec[0][par]) { hipGraphExecDestroy(c->gexec[0][par]); c->gexec[0][par] = nullptr; }
-    if (c->graph[0][par]) { hipGraphDestroy(c->graph[0][par]); c->graph[0][par] = nullptr; }
+    if (c->gexec[0][par][0] && c->ens_graph_ok[par]) return MPM_OK;
+    if (c->gexec[0][par][0]) { hipGraphExecDestroy(c->gexec[0][par][0]); c->gexec[0][par][0] = nullptr; }
+    if (c->graph[0][par][0]) { hipGraphDestroy(c->graph[0][par][0]); c->graph[0][par][0] = nullptr; }
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     enqueue_ens_substep(c, false);
     hipGraph_t g = nullptr;
@@ -659,8 +677,8 @@ int ens_graph(mpm_ctx* c) {
     hipGraphExec_t e = nullptr;
     hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
     if (r != hipSuccess) { hipGraphDestroy(g); return fail(c, MPM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(r)); }
-    c->graph[0][par] = g;
-    c->gexec[0][par] = e;
+    c->graph[0][par][0] = g;
+    c->gexec[0][par][0] = e;
     c->ens_graph_ok[par] = true;
     return MPM_OK;
 }
@@ -728,9 +746,9 @@ void mpm_config_defaults(mpm_config* g) {
 }
 
 static void release(mpm_ctx* c) {
-    for (int m = 0; m < 2; ++m) for (int p = 0; p < 2; ++p) {
-        if (c->gexec[m][p]) hipGraphExecDestroy(c->gexec[m][p]);
-        if (c->graph[m][p]) hipGraphDestroy(c->graph[m][p]);
+    for (int m = 0; m < 2; ++m) for (int p = 0; p < 2; ++p) for (int k = 0; k < 2; ++k) {
+        if (c->gexec[m][p][k]) hipGraphExecDestroy(c->gexec[m][p][k]);
+        if (c->graph[m][p][k]) hipGraphDestroy(c->graph[m][p][k]);
     }
     for (int i = 0; i < 6; ++i) if (c->ev[i]) hipEventDestroy(c->ev[i]);
     for (hipEvent_t e : c->kt_ev) hipEventDestroy(e);
@@ -1103,9 +1121,21 @@ int mpm_step(mpm_ctx* c, int method, int n_substeps) {
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     for (int s = 0; s < n_substeps; ++s) {
         const bool timed = c->timing && s == n_substeps - 1;
+        // the substeps before the timed last one; a run of GRAPH_SUBSTEPS replays the graph holding
+        // them when it is captured already (mpm_prepare_graphs, or an earlier call) or when the call
+        // is long enough to pay for capturing it here
+        const int untimed_left = n_substeps - s - (c->timing ? 1 : 0);
+        if (c->cfg.use_graph && !c->ktiming && untimed_left >= GRAPH_SUBSTEPS &&
+            (untimed_left >= 4 * GRAPH_SUBSTEPS || graph_valid(c, method, P, 1))) {
+            if ((rc = ensure_graph(c, method, P, 1))) return rc;
+            HIPCHK(c, hipGraphLaunch(c->gexec[method][c->cur][1], c->stream));
+            c->substeps += GRAPH_SUBSTEPS;
+            s += GRAPH_SUBSTEPS - 1;          // the parity is back where it was
+            continue;
+        }
         if (c->cfg.use_graph && !timed && !c->ktiming) {
             if ((rc = ensure_graph(c, method, P))) return rc;
-            HIPCHK(c, hipGraphLaunch(c->gexec[method][c->cur], c->stream));
+            HIPCHK(c, hipGraphLaunch(c->gexec[method][c->cur][0], c->stream));
         } else {
             enqueue_substep(c, method, P, -1, timed);
             HIPCHK(c, hipGetLastError());
@@ -1116,6 +1146,24 @@ int mpm_step(mpm_ctx* c, int method, int n_substeps) {
     }
     if (c->timing && (rc = finish_timing(c, n_substeps))) return rc;
     kt_collect(c);
+    return MPM_OK;
+}
+
+int mpm_prepare_graphs(mpm_ctx* c, int method) {
+    if (!c) return MPM_E_INVALID;
+    if (method != MPM_METHOD_EXPLICIT && method != MPM_METHOD_PBMPM) return fail(c, MPM_E_INVALID, "unknown method");
+    if (!c->has_particles) return fail(c, MPM_E_STATE, "no particles uploaded");
+    if (!c->cfg.use_graph) return MPM_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+    SimParams P;
+    int rc = make_params(c, P);
+    if (rc) return rc;
+    wru<uint32_t>(c->uni, 8, method == MPM_METHOD_PBMPM ? 1u : 0u);   // writeUsePbmpm, as mpm_step
+    for (int p = 0; p < 2; ++p) {
+        for (int many = 0; many < 2 && !rc; ++many) rc = ensure_graph(c, method, P, many);
+        c->cur ^= 1;
+        if (rc) { if (p == 0) c->cur ^= 1; return rc; }
+    }
     return MPM_OK;
 }
 
@@ -1780,7 +1828,7 @@ int mpm_ensemble_step(mpm_ctx* c, int n_substeps) {
         const bool timed = c->timing && s == n_substeps - 1;
         if (c->cfg.use_graph && !timed && !c->ktiming) {
             if ((rc = ens_graph(c))) return rc;
-            HIPCHK(c, hipGraphLaunch(c->gexec[0][c->cur], c->stream));
+            HIPCHK(c, hipGraphLaunch(c->gexec[0][c->cur][0], c->stream));
         } else {
             enqueue_ens_substep(c, timed);
             HIPCHK(c, hipGetLastError());

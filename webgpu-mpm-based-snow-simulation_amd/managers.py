@@ -309,6 +309,10 @@ class MpmSimulation:
     def step(self, method, n_substeps=1):
         check(self.lib, self.ctx, self.lib.mpm_step(self.ctx, int(method), int(n_substeps)))
 
+    def prepare_graphs(self, method):
+        """Captures the step's graphs (one substep and a run of 8, both buffer parities) now."""
+        check(self.lib, self.ctx, self.lib.mpm_prepare_graphs(self.ctx, int(method)))
+
     def sync(self):
         check(self.lib, self.ctx, self.lib.mpm_sync(self.ctx))
 

@@ -20,6 +20,7 @@ const SYMBOLS = {
   mpm_download_particles: { parameters: ["pointer", "buffer", "usize"], result: "i32" },
   mpm_step: { parameters: ["pointer", "i32", "i32"], result: "i32" },
   mpm_step_async: { name: "mpm_step", parameters: ["pointer", "i32", "i32"], result: "i32", nonblocking: true },
+  mpm_prepare_graphs: { parameters: ["pointer", "i32"], result: "i32" },
   mpm_sync: { parameters: ["pointer"], result: "i32" },
   mpm_set_timing: { parameters: ["pointer", "i32"], result: "i32" },
   mpm_get_timings: { parameters: ["pointer", "buffer"], result: "i32" },
