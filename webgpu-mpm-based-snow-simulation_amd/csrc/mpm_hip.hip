@@ -87,6 +87,10 @@ struct mpm_ctx {
     int4* grid[3] = {nullptr, nullptr, nullptr};
     int* gmass = nullptr;
     bool last_pbmpm = false;
+    // fused keys pass (SimParams::fuse_keys): the buffer parity whose rows the stored keys and key-space
+    // counts describe (-1: none), and the grid they were made for
+    int keys_parity = -1;
+    float keys_grid[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     uint32_t* counters = nullptr;
     uint32_t* cqueue = nullptr;
     uint32_t* coll_data = nullptr;
@@ -437,7 +441,26 @@ int make_params(mpm_ctx* c, SimParams& P) {
     P.per_particle_lame = c->per_particle_lame ? 1u : 0u;
     // one scene: chunks longest first (a pass hands them out in list order and ends on the short ones)
     P.chunk_lpt = 1u;
+    P.fuse_keys = c->direct_blocks ? 1u : 0u;
+    P.keys_ready = 0u;       // set per substep by keys_state
     return rc;
+}
+
+// Whether the stored keys (made by the previous substep's fused keys pass) describe the rows the next
+// substep bins with these parameters: same buffer parity, same grid.
+static void grid_of(const SimParams& P, float g[9]) {
+    for (int a = 0; a < 3; ++a) { g[a] = P.gmin[a]; g[3 + a] = P.cd[a]; g[6 + a] = (float)P.res[a]; }
+}
+void keys_state(mpm_ctx* c, SimParams& P) {
+    float g[9];
+    grid_of(P, g);
+    P.keys_ready = (P.fuse_keys && c->keys_parity == c->cur && std::memcmp(g, c->keys_grid, sizeof g) == 0) ? 1u : 0u;
+}
+// After a substep: with the fused keys pass its last passes made the keys of the rows that are now
+// current (`cur` flipped already); a partial substep leaves none.
+void keys_after(mpm_ctx* c, const SimParams& P, bool full) {
+    if (full && P.fuse_keys) { c->keys_parity = c->cur; grid_of(P, c->keys_grid); }
+    else c->keys_parity = -1;
 }
 
 uint32_t hash_host(uint32_t n) {
@@ -462,7 +485,17 @@ void enqueue_binning(mpm_ctx* c, const SimParams& P, const Bufs& B) {
     // per thread in two dispatch waves (1M particles: 30.3 -> 26.7 us)
     const uint32_t gk = grid_for(P.direct_blocks ? (uint64_t)c->n : std::max<uint64_t>(c->n, c->cap), (uint32_t)c->n_cu * 8u);
     kt(c, -1);
-    k_keys_count<<<gk, 256, 0, s>>>(P, B); kt(c, KID_KEYS);
+    if (!P.keys_ready) {
+        if (P.fuse_keys) {
+            // the key space may hold the counts of a fused keys pass whose keys are not used (new
+            // particles, another grid, a discarded substep): clear it and the non-finite count first
+            k_fill_u32<<<grid_for(c->K, 4096), 256, 0, s>>>(B.kcount, 0u, c->K);
+            k_fill_u32<<<grid_for(c->K, 4096), 256, 0, s>>>(B.kmask, 0u, c->K);
+            k_fill_u32<<<grid_for(c->K, 4096), 256, 0, s>>>(B.kalloc, 0u, c->K);
+            k_fill_u32<<<1, 256, 0, s>>>(B.counters + C_NONFIN, 0u, 1u);
+        }
+        k_keys_count<<<gk, 256, 0, s>>>(P, B); kt(c, KID_KEYS);
+    }
     enqueue_scan(c, P, B, s);
     const uint32_t gi = grid_for((uint64_t)c->max_runs * 27u, 4096);
     if (P.direct_blocks) {
@@ -927,6 +960,7 @@ int mpm_upload_particles(mpm_ctx* c, const void* aos, size_t n) {
     hipFree(tmp);
     if (e != hipSuccess) return fail(c, MPM_E_HIP, std::string("particle upload: ") + hipGetErrorString(e));
     c->has_particles = true;
+    c->keys_parity = -1;
     return MPM_OK;
 }
 
@@ -949,6 +983,7 @@ int mpm_scatter_particles(mpm_ctx* c, const float* mesh_xyzw, size_t n_vertices)
     hipFree(dm);
     if (e != hipSuccess) return fail(c, MPM_E_HIP, std::string("particle scatter: ") + hipGetErrorString(e));
     c->has_particles = true;
+    c->keys_parity = -1;
     return MPM_OK;
 }
 
@@ -1121,19 +1156,24 @@ int mpm_step(mpm_ctx* c, int method, int n_substeps) {
     if (c->timing) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     for (int s = 0; s < n_substeps; ++s) {
         const bool timed = c->timing && s == n_substeps - 1;
+        keys_state(c, P);
+        // graphs hold the steady state only: with the fused keys pass, a substep whose keys the
+        // previous one did not make (after an upload or a grid change) runs eagerly
+        const bool steady = !P.fuse_keys || P.keys_ready;
         // the substeps before the timed last one; a run of GRAPH_SUBSTEPS replays the graph holding
         // them when it is captured already (mpm_prepare_graphs, or an earlier call) or when the call
         // is long enough to pay for capturing it here
         const int untimed_left = n_substeps - s - (c->timing ? 1 : 0);
-        if (c->cfg.use_graph && !c->ktiming && untimed_left >= GRAPH_SUBSTEPS &&
+        if (c->cfg.use_graph && !c->ktiming && steady && untimed_left >= GRAPH_SUBSTEPS &&
             (untimed_left >= 4 * GRAPH_SUBSTEPS || graph_valid(c, method, P, 1))) {
             if ((rc = ensure_graph(c, method, P, 1))) return rc;
             HIPCHK(c, hipGraphLaunch(c->gexec[method][c->cur][1], c->stream));
             c->substeps += GRAPH_SUBSTEPS;
             s += GRAPH_SUBSTEPS - 1;          // the parity is back where it was
+            keys_after(c, P, true);
             continue;
         }
-        if (c->cfg.use_graph && !timed && !c->ktiming) {
+        if (c->cfg.use_graph && !timed && !c->ktiming && steady) {
             if ((rc = ensure_graph(c, method, P))) return rc;
             HIPCHK(c, hipGraphLaunch(c->gexec[method][c->cur][0], c->stream));
         } else {
@@ -1142,6 +1182,7 @@ int mpm_step(mpm_ctx* c, int method, int n_substeps) {
         }
         c->cur ^= 1;
         c->substeps++;
+        keys_after(c, P, true);
         if (c->ktiming && c->kt_n > 512) kt_collect(c);
     }
     if (c->timing && (rc = finish_timing(c, n_substeps))) return rc;
@@ -1159,6 +1200,7 @@ int mpm_prepare_graphs(mpm_ctx* c, int method) {
     int rc = make_params(c, P);
     if (rc) return rc;
     wru<uint32_t>(c->uni, 8, method == MPM_METHOD_PBMPM ? 1u : 0u);   // writeUsePbmpm, as mpm_step
+    P.keys_ready = P.fuse_keys;      // the steady state: keys made by the previous substep
     for (int p = 0; p < 2; ++p) {
         for (int many = 0; many < 2 && !rc; ++many) rc = ensure_graph(c, method, P, many);
         c->cur ^= 1;
@@ -1184,10 +1226,12 @@ int mpm_debug_stage(mpm_ctx* c, int method, int stage) {
     SimParams P;
     int rc = make_params(c, P);
     if (rc) return rc;
+    keys_state(c, P);
     bool done = enqueue_substep(c, method, P, stage == MPM_STAGE_WARM_START ? stage : stage < 0 ? 1 << 30 : stage);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (done) { c->cur ^= 1; c->substeps++; }
+    keys_after(c, P, done);
     return MPM_OK;
 }
 
@@ -1448,16 +1492,21 @@ int mpm_learn_collect(mpm_ctx* c, int n_substeps, int k_run, int k_ref, int ever
                     // the target substep: k_ref iterations from the same start state into the other
                     // planes (a substep reads `cur` and writes only the other set), then the logged
                     // targets from there; the trajectory's own substep below overwrites them
+                    keys_state(c, Pref);
                     enqueue_substep(c, MPM_METHOD_PBMPM, Pref, -1, false);
                     c->cur ^= 1;
                     B = make_bufs(c);
                     c->cur ^= 1;
+                    keys_after(c, Pref, false);      // its keys are of rows the trajectory discards
                     k_mlp_log_targets<<<grid_for(c->n, 1u << 30), 256, 0, c->stream>>>(B, c->n, c->ds_feats, c->ds_targs, c->ds_n,
                                                                                      gdt2[0], gdt2[1], gdt2[2]);
                 }
             }
-            enqueue_substep(c, MPM_METHOD_PBMPM, k_ref != k_run ? Prun : Pref, -1, false);
+            SimParams& Pt = k_ref != k_run ? Prun : Pref;
+            keys_state(c, Pt);
+            enqueue_substep(c, MPM_METHOD_PBMPM, Pt, -1, false);
             c->cur ^= 1; c->substeps++;
+            keys_after(c, Pt, true);
             if (log) {
                 if (k_ref == k_run) {
                     Bufs B = make_bufs(c);

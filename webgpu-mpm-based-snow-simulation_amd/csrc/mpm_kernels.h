@@ -78,6 +78,11 @@ struct SimParams {
     float rdt;                 // RN(1/dt) for fixed_div
     uint32_t fast_dt;          // fixed_div(., dt, rdt, S) verified equal to i32((x/dt) S) for all x
     uint32_t direct_blocks;    // block set by direct key addressing (no hash table)
+    uint32_t fuse_keys;        // one scene, direct addressing: the passes that leave a particle's position
+                               // final (LAST or the explicit G2P, the collider, the unbinned particles)
+                               // also do the next substep's keys pass for it (bin_next)
+    uint32_t keys_ready;       // the keys of the rows this substep bins were made by the previous
+                               // substep (fuse_keys): k_keys_count is not launched
     uint32_t block_identity;   // direct addressing with dense block index = key (batched scenes)
     float rcd[3];              // RN(1 / cell dims)
     uint32_t fast_cd;          // qdiv(., cd[a], rcd[a]) verified equal to x / cd[a] for all |x| <= 2^100
@@ -441,12 +446,28 @@ MPM_DEV void flag_fresh(const Bufs& B, const KeyLane& a, uint32_t fresh, uint32_
 // Hash-table mode also resets the table (clearHashMap.wgsl:5-19); direct mode instead flags every
 // touched block in kalloc (plain stores of 1, one lane per neighbour bit that the wave's OR added
 // to its key's mask, so each pair is stored once). Resets the per-substep counters.
+// The keys pass for particles whose position is final (active lanes; row q of nxt, which the next
+// substep bins as its current order), done by the pass that finalises them: key, mask, offset in
+// its key from the count atomic, key count / OR, touched-block flags, non-finite count. The key-space
+// accumulators were cleared by this substep's scan. Every lane of the wave calls it.
+MPM_DEV void bin_next(const SimParams& P, const Bufs& B, bool active, uint32_t q, v3 pos) {
+    KeyLane a = key_of(P, B, active, q, pos);
+    group_keys(a);
+    uint32_t b = 0, fresh = 0;
+    if (a.valid && a.leader == lane_id()) { b = atomicAdd(&B.kcount[a.key], a.lead_cnt); fresh = atomicOr(&B.kmask[a.key], a.lead_m); }
+    fresh = a.lead_m & ~fresh;
+    b = __shfl(b, a.leader, 64);
+    if (a.valid) B.lofs[q] = b + a.rank;
+    const int jl = lane_id();
+    const uint32_t koff = (uint32_t)((jl % 3 - 1) + P.nb[0] * ((jl / 3) % 3 - 1 + P.nb[1] * (jl / 9 - 1)));
+    flag_fresh(B, a, fresh, koff);
+}
 __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
     const uint32_t gsz = gridDim.x * blockDim.x;
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (!P.direct_blocks)
         for (uint32_t s = gi; s < P.cap; s += gsz) B.table[s] = EMPTY;
-    if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; B.counters[C_SVD64] = 0; }
+    if (gi == 0) { B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; }
     const uint32_t n = P.n;
     // key offset of the neighbour block this lane flags (lane j < 27: offset j, x fastest); a
     // touched neighbour is inside the block grid, so key + offset is its key
@@ -534,6 +555,10 @@ __global__ __launch_bounds__(256) void k_scan_top(SimParams P, Bufs B, uint32_t 
 template <uint32_t KPT>
 __global__ __launch_bounds__(256) void k_scan_apply(SimParams P, Bufs B, uint32_t ntiles, int own_ofs) {
     __shared__ uint32_t wt[4][SCAN_NS];
+    // per-substep counters the later passes of this substep count into
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        B.counters[C_STRAG] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; B.counters[C_SVD64] = 0;
+    }
     uint32_t tofs[SCAN_NS], nlong;
     if (own_ofs) {
         uint32_t v[SCAN_NS] = {0, 0, 0, 0, 0}, all[SCAN_NS] = {0, 0, 0, 0, 0};
@@ -1038,15 +1063,14 @@ MPM_DEV void integrate(const SimParams& P, const Bufs& B, Part& p, uint32_t q) {
 // writes x, v, dx of the new-order state. LDS = true: the records and triangles are first copied
 // into LDS (colliders of at most COLL_LDS_REC records and COLL_LDS_TRI triangles).
 constexpr uint32_t COLL_LDS_REC = 64, COLL_LDS_TRI = 128;
-MPM_DEV void straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_t old);
+MPM_DEV v3 straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_t old);
+MPM_DEV void straggle_all(const SimParams& P, const Bufs& B);
 // Also integrates the unbinned particles (k_straggle's work) after the queue.
 template <bool LDS>
 __global__ __launch_bounds__(256) void k_collide(SimParams P, Bufs B) {
     const uint32_t nq = B.counters[C_NCOLL];
     const uint32_t sd = B.stride;
-    const uint32_t nvalid = B.counters[C_NVALID];
-    for (uint32_t q = nvalid + blockIdx.x * blockDim.x + threadIdx.x; q < P.n; q += gridDim.x * blockDim.x)
-        straggle_one(P, B, q, B.perm[q]);
+    straggle_all(P, B);
     if (blockIdx.x * blockDim.x >= nq) return;
     const float4* rec = B.crec;
     const float4* tri = B.ctri;
@@ -1059,12 +1083,19 @@ __global__ __launch_bounds__(256) void k_collide(SimParams P, Bufs B) {
         rec = srec;
         tri = stri;
     }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
-        const uint32_t q = B.cqueue[i];
+    // workgroup-uniform rounds over the queue (the fused keys pass needs whole waves)
+    for (uint32_t base = blockIdx.x * blockDim.x; base < nq; base += gridDim.x * blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t q = 0;
         Part p;
-        p.pos = ld3(B.nxt, sd, PX, q); p.dx = ld3(B.nxt, sd, DXX, q); p.vel = ld3(B.nxt, sd, VX, q);
-        resolve_collision_narrow(P, rec, tri, p);
-        st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, DXX, q, p.dx); st3(B.nxt, sd, VX, q, p.vel);
+        p.pos = mk(0.0f, 0.0f, 0.0f);
+        if (i < nq) {
+            q = B.cqueue[i];
+            p.pos = ld3(B.nxt, sd, PX, q); p.dx = ld3(B.nxt, sd, DXX, q); p.vel = ld3(B.nxt, sd, VX, q);
+            resolve_collision_narrow(P, rec, tri, p);
+            st3(B.nxt, sd, PX, q, p.pos); st3(B.nxt, sd, DXX, q, p.dx); st3(B.nxt, sd, VX, q, p.vel);
+        }
+        if (P.fuse_keys) bin_next(P, B, i < nq, q, p.pos);
     }
 }
 
@@ -1526,6 +1557,7 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
             const uint32_t i = r * blockDim.x + threadIdx.x;
             const uint32_t q = T.start + i;
             bool cand = false;
+            v3 fin = mk(0.0f, 0.0f, 0.0f);   // LAST: the particle's position after integrate
             float resid = 0.0f;
             if (MODE == PB_FIRST) {
                 const bool mine = i < T.count;
@@ -1603,8 +1635,12 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
                     if (TRACK && B.dprev) st3(B.dprev, sd, 0, q, mk(0.0f, 0.0f, 0.0f));
                 }
                 cand = integrate_store(P, B, p, q);
+                fin = p.pos;
             }
             if (MODE == PB_LAST && P.has_collider) queue_collider_wg(B, T, cand, q);   // uniform: P is the chunk's
+            // the next substep's keys of the particles that are final here (the collider pass does the
+            // queued ones); one scene only, so not compiled into the batched pass (POW2 = false)
+            if (MODE == PB_LAST && POW2 && P.fuse_keys) bin_next(P, B, i < T.count && !cand, q, fin);
             if (MODE != PB_FIRST && TRACK && it > 0 && (P.learn_flags & 2u)) publish_residual(B, resid, it);
         }
         __syncthreads();
@@ -1788,6 +1824,7 @@ MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
             const uint32_t i = r * blockDim.x + threadIdx.x;
             const uint32_t q = T.start + i;
             bool cand = false;
+            v3 fin = mk(0.0f, 0.0f, 0.0f);   // the particle's position after integrate
             if (i < T.count) {
             Part p;
             p.pos = ld3(B.nxt, sd, PX, q);
@@ -1806,8 +1843,10 @@ MPM_DEV void explicit_g2p_chunk(const SimParams& P, const Bufs& B, Tile& T, uint
                 p.dx = ld3(B.nxt, sd, DXX, q); p.D = ldm(B.nxt, sd, DD, q); p.vel = ld3(B.nxt, sd, VX, q);
             }
             cand = integrate_store(P, B, p, q);
+            fin = p.pos;
             }
             queue_collider(B, cand, q);
+            if (!FUSED && P.fuse_keys) bin_next(P, B, i < T.count && !cand, q, fin);   // one scene only
         }
         __syncthreads();
     }
@@ -1822,7 +1861,7 @@ __global__ __launch_bounds__(256, 4) void k_explicit_g2p(SimParams P, Bufs B) {
 
 // Unbinned particles: copied to the new order and integrated with their stored dx / D
 // (no grid transfer reaches them). Runs over the sorted tail [n_valid, N).
-MPM_DEV void straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_t old) {
+MPM_DEV v3 straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_t old) {
     const uint32_t sd = B.stride;
     {
         B.pid_nxt[q] = B.pid_cur[old];
@@ -1838,13 +1877,21 @@ MPM_DEV void straggle_one(const SimParams& P, const Bufs& B, uint32_t q, uint32_
         stm(B.nxt, sd, FE, q, p.Fe);
         st3(B.nxt, sd, DXX, q, p.dx); stm(B.nxt, sd, DD, q, p.D);
         if (B.dprev) st3(B.dprev, sd, 0, q, mk(0.0f, 0.0f, 0.0f));   // not iterated this substep
+        return p.pos;
     }
 }
-__global__ __launch_bounds__(256) void k_straggle(SimParams P, Bufs B) {
+// The unbinned particles [n_valid, n) of the sorted order, a workgroup-uniform number of rounds
+// (the fused keys pass needs whole waves).
+MPM_DEV void straggle_all(const SimParams& P, const Bufs& B) {
     const uint32_t nvalid = B.counters[C_NVALID];
-    for (uint32_t q = nvalid + blockIdx.x * blockDim.x + threadIdx.x; q < P.n; q += gridDim.x * blockDim.x)
-        straggle_one(P, B, q, B.perm[q]);
+    for (uint32_t base = nvalid + blockIdx.x * blockDim.x; base < P.n; base += gridDim.x * blockDim.x) {
+        const uint32_t q = base + threadIdx.x;
+        v3 fin = mk(0.0f, 0.0f, 0.0f);
+        if (q < P.n) fin = straggle_one(P, B, q, B.perm[q]);
+        if (P.fuse_keys) bin_next(P, B, q < P.n, q, fin);
+    }
 }
+__global__ __launch_bounds__(256) void k_straggle(SimParams P, Bufs B) { straggle_all(P, B); }
 // Copy of the state without integration (debug-stage prefixes leave particles untouched).
 __global__ void k_copy_state(Bufs B, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
