@@ -191,10 +191,8 @@ MPM_DEV float det3(const m3& m) {
           - m.c[1].x * (m.c[0].y * m.c[2].z - m.c[2].y * m.c[0].z))
           + m.c[2].x * (m.c[0].y * m.c[1].z - m.c[1].y * m.c[0].z);
 }
-MPM_DEV m3 inverse3(const m3& m) {
-    float det = det3(m);
-    if (det == 0.0f) return ident();
-    float inv = 1.0f / det;
+MPM_DEV m3 adj_scaled(const m3& m, float det) {
+    const float inv = 1.0f / det;
     m3 r;
     r.c[0] = mk(m.c[1].y * m.c[2].z - m.c[1].z * m.c[2].y,
                 m.c[0].z * m.c[2].y - m.c[0].y * m.c[2].z,
@@ -206,6 +204,14 @@ MPM_DEV m3 inverse3(const m3& m) {
                 m.c[0].y * m.c[2].x - m.c[0].x * m.c[2].y,
                 m.c[0].x * m.c[1].y - m.c[0].y * m.c[1].x);
     return mscale_r(r, inv);
+}
+// A singular matrix inverts to the identity (matrixOps.wgsl:8-12). The test is made for the whole
+// wave first: when no lane has det = 0 (the usual case) no lane carries the identity's selects.
+MPM_DEV m3 inverse3(const m3& m) {
+    const float det = det3(m);
+    if (__builtin_expect(__ballot(det == 0.0f) == 0ull, 1)) return adj_scaled(m, det);
+    if (det == 0.0f) return ident();
+    return adj_scaled(m, det);
 }
 MPM_DEV m3 polar_rotation(const m3& f) {
     m3 r = f;
