@@ -229,6 +229,45 @@ def test_multi_substep_graph_matches_eager_and_oracle(mpm, orc):
         c.close()
 
 
+def test_keys_carried_between_substeps_follow_the_state(mpm, orc):
+    """A substep leaves the next one's block keys (made where positions become final). They are
+    used only for the state and grid they describe: after the grid is moved between calls
+    (writeGridMinCoords / writeGridMaxCoords, 2.5 cells in x and 1.25 in y, so keys change and some
+    particles leave the grid), after a fresh upload, and across a switch between the explicit and
+    PBMPM methods every substep still equals the oracle given the same uniforms and particles."""
+    sc = _small_pbmpm(mpm, n=6000, seed=31)
+    sim, o = make_pair(mpm, orc, sc, use_graph=1)
+    sim.step(1, 3)
+    o.step(1, 3)
+    assert_particles_equal(mpm, sim.download(), o.particles(), "before the grid moves")
+    um = sim.uniformsManager
+    mn = np.frombuffer(um.read(16, 12), np.float32).copy()
+    mx = np.frombuffer(um.read(32, 12), np.float32).copy()
+    res = np.frombuffer(um.read(208, 12), np.uint32)
+    cell = (mx - mn) / res.astype(np.float32)
+    shift = np.array([2.5, 1.25, 0.0], np.float32) * cell
+    um.writeGridMinCoords(list(mn + shift))
+    um.writeGridMaxCoords(list(mx + shift))
+    o.set_uniforms(sim.uniform_bytes())
+    for s in range(3):
+        sim.step(1, 1)
+        o.step(1, 1)
+        assert_particles_equal(mpm, sim.download(), o.particles(), f"moved grid, substep {s}")
+    # a fresh upload of the same particles: the next substep bins them afresh
+    p0 = o.particles()
+    sim.upload(p0)
+    sim.step(1, 2)
+    o.step(1, 2)
+    assert_particles_equal(mpm, sim.download(), o.particles(), "after a re-upload")
+    # explicit substeps bin from keys made by PBMPM's LAST and the other way round
+    sim.step(0, 2)
+    o.step(0, 2)
+    sim.step(1, 2)
+    o.step(1, 2)
+    assert_particles_equal(mpm, sim.download(), o.particles(), "after switching methods")
+    sim.close()
+
+
 def test_divisor_fast_path_verified(mpm):
     """P2G's reciprocal+FMA quotient is enabled only for (dt, scale) pairs whose exhaustive device
     check (all 2^32 float numerators) matched i32((x / dt) * scale); the scene time steps pass."""

@@ -1650,8 +1650,8 @@ MPM_DEV void pbmpm_chunk(const SimParams& P, const Bufs& B, Tile& T, uint2 chunk
     }
 }
 
-// Resident workgroups per CU: 4, without spills. (FIRST at 5 spills and is slower; MID at 5, with
-// 24 B/lane spilled, measured the same once its rows came in cell-interleaved order.)
+// Resident workgroups per CU: 4, without spills. (At 5, FIRST spills 80 B and MID 72 B per lane;
+// both are slower, MID 78 -> 89 us per launch.)
 // TRACK: the residual bookkeeping of the learned warm start (B.resid_dx set); a separate
 // instantiation so the untracked passes carry none of its registers.
 template <int MODE, bool TRACK = false>
