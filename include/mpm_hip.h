@@ -103,6 +103,7 @@ typedef struct mpm_stats {
     uint64_t nonfinite_particles;    /* particles whose position was NaN or infinite at binning (binned at
                                         cell 0 with zero grid contributions, as the reference's i32(NaN) = 0) */
     uint64_t svd_fp64_particles;     /* MPM_PLASTICITY_SVD_FP64: particles decomposed in FP64 (last substep) */
+    uint64_t collider_queued;        /* collider broad-phase candidates sent to the narrow phase (last substep) */
 } mpm_stats;
 
 /* Device time of the last mpm_step / mpm_ensemble_step made with timing on (mpm_set_timing).

@@ -268,6 +268,22 @@ def test_keys_carried_between_substeps_follow_the_state(mpm, orc):
     sim.close()
 
 
+def test_explicit_with_collider_substeps_bitexact(mpm, orc):
+    """Explicit substeps against a box collider that the falling cube overlaps: the G2P pass makes
+    the next substep's keys of the particles it finalises and the collider pass those of the queued
+    ones, over several substeps, in graphs and eagerly."""
+    sc = mpm.scenes.scene("explicit_small", n_particles=12000, seed=41)
+    sc = dict(sc, collider=mpm.scenes.box_mesh((0.5, 0.46, 0.5), 0.2))
+    for use_graph in (1, 0):
+        sim, o = make_pair(mpm, orc, sc, use_graph=use_graph)
+        for s in range(6):
+            sim.step(0, 1)
+            o.step(0, 1)
+            assert_particles_equal(mpm, sim.download(), o.particles(), f"explicit + collider substep {s} (graph {use_graph})")
+        assert sim.stats()["collider_queued"] > 0
+        sim.close()
+
+
 def test_divisor_fast_path_verified(mpm):
     """P2G's reciprocal+FMA quotient is enabled only for (dt, scale) pairs whose exhaustive device
     check (all 2^32 float numerators) matched i32((x / dt) * scale); the scene time steps pass."""

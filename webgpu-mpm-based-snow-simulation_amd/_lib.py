@@ -68,7 +68,7 @@ class MpmConfig(ctypes.Structure):
 class MpmStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "n_particles", "particles_binned", "particles_unbinned", "particle_blocks", "work_chunks",
-        "blocks_allocated", "blocks_dropped", "probe_failures", "substeps", "mlp_fallbacks", "nonfinite_particles", "svd_fp64_particles")]
+        "blocks_allocated", "blocks_dropped", "probe_failures", "substeps", "mlp_fallbacks", "nonfinite_particles", "svd_fp64_particles", "collider_queued")]
 
 
 class MpmTimings(ctypes.Structure):

@@ -1344,6 +1344,7 @@ int mpm_get_stats(mpm_ctx* c, mpm_stats* out) {
     out->mlp_fallbacks = h[C_FALLBACK];
     out->nonfinite_particles = h[C_NONFIN_LAST];
     out->svd_fp64_particles = h[C_SVD64];
+    out->collider_queued = h[C_NCOLL];
     return MPM_OK;
 }
 
