@@ -39,6 +39,10 @@ enum { MPM_METHOD_EXPLICIT = 0, MPM_METHOD_PBMPM = 1 };   /* GpuSimulationMethod
 
 /* Creation-time configuration. Every field that the reference hard-codes in WGSL or TS has
  * a runtime value here; mpm_config_defaults() fills the reference values. */
+/* Largest particle count of a context (the sum over the scenes of a batch): rows are addressed
+ * with 32-bit byte offsets. mpm_create / mpm_ensemble_create return MPM_E_RANGE above it. */
+#define MPM_MAX_PARTICLES 0x3FFFFF00u
+
 typedef struct mpm_config {
     uint32_t n_particles;          /* GpuMpmBufferManager.ts:15-21 nParticles */
     uint32_t grid_res[3];          /* uniforms gridResolution @208 */

@@ -189,6 +189,22 @@ def test_create_rejects_chunk_sizes_out_of_range(mpm, chunk):
                                    ctypes.byref(h)) == -4 and not h.value
 
 
+def test_create_rejects_particle_counts_above_the_row_limit(mpm):
+    """Rows are addressed with 32-bit byte offsets: a context of more than MPM_MAX_PARTICLES
+    (0x3FFFFF00) particles, or a batch whose scenes add up to more, is rejected before any
+    allocation."""
+    import ctypes
+    lib = mpm.load_library()
+    h = ctypes.c_void_p()
+    cfg = mpm.default_config(n_particles=0x3FFFFF01, grid_res=[32, 32, 32], grid_min=[0, 0, 0], grid_max=[1, 1, 1], dt=1e-3)
+    assert lib.mpm_create(ctypes.byref(cfg), ctypes.byref(h)) == -4 and not h.value
+    cfg.n_particles = 0x20000000
+    cfgs = (mpm.MpmConfig * 2)(cfg, cfg)
+    methods = (ctypes.c_int * 2)(0, 1)
+    assert lib.mpm_ensemble_create(ctypes.cast(cfgs, ctypes.c_void_p), ctypes.cast(methods, ctypes.c_void_p), 2,
+                                   ctypes.byref(h)) == -4 and not h.value
+
+
 def test_plasticity_mode_validated(mpm):
     """plasticity_mode 0..2 for a single scene; a batch takes only the default (the SVD pass runs
     in single-scene contexts)."""

@@ -795,8 +795,9 @@ int mpm_create(const mpm_config* cfg, mpm_ctx** out) {
     if (cfg->n_particles == 0) return MPM_E_INVALID;
     for (int a = 0; a < 3; ++a) if (cfg->grid_res[a] == 0 || cfg->grid_res[a] > 65536 || !(cfg->grid_max[a] > cfg->grid_min[a])) return MPM_E_INVALID;
     if (cfg->pbmpm_iterations < 1 || cfg->pbmpm_iterations > 64) return MPM_E_INVALID;
-    // chunk counts are formed as (count + chunk - 1) / chunk in u32: both stay far below 2^32
-    if (cfg->n_particles > 0x7FFFFFFFu) return MPM_E_RANGE;
+    // row byte offsets are 32-bit (the transfer passes address a plane's row i at base + 4 i), and
+    // chunk counts are formed as (count + chunk - 1) / chunk in u32
+    if (cfg->n_particles > MPM_MAX_PARTICLES) return MPM_E_RANGE;
     if (cfg->chunk_particles && (cfg->chunk_particles < 256 || cfg->chunk_particles > 65536)) return MPM_E_RANGE;
     if (cfg->plasticity_mode > 2) return MPM_E_INVALID;
     mpm_ctx* c = new mpm_ctx();
@@ -1735,7 +1736,7 @@ int mpm_ensemble_create(const mpm_config* cfgs, const int* methods, int n_scenes
         K += (uint64_t)((g.grid_res[0] + 3u) / 4u) * ((g.grid_res[1] + 3u) / 4u) * ((g.grid_res[2] + 3u) / 4u);
         R = std::max<uint32_t>(R, methods[s] == MPM_METHOD_PBMPM ? (uint32_t)g.pbmpm_iterations + 1u : 2u);
     }
-    if (n >= 0xFFFFFF00ull || K >= 0xFFFFFF00ull || (uint64_t)R * n_scenes > ENS_MAX_SEG) return MPM_E_RANGE;
+    if (n > MPM_MAX_PARTICLES || K >= 0xFFFFFF00ull || (uint64_t)R * n_scenes > ENS_MAX_SEG) return MPM_E_RANGE;
     mpm_ctx* c = new mpm_ctx();
     c->cfg = cfgs[0];
     c->device = cfgs[0].device;

@@ -152,8 +152,16 @@ struct Part {
     v3 dx;
 };
 
-MPM_DEV float ld(const float* s, uint32_t stride, int plane, uint32_t i) { return s[(size_t)plane * stride + i]; }
-MPM_DEV void st(float* s, uint32_t stride, int plane, uint32_t i, float v) { s[(size_t)plane * stride + i] = v; }
+// Row i of a plane: the plane's base is wave-uniform and the row's byte offset fits 32 bits (fewer
+// than 2^30 rows), so the access takes the scalar-base + 32-bit vector-offset address form.
+MPM_DEV float ld(const float* s, uint32_t stride, int plane, uint32_t i) {
+    const char* p = reinterpret_cast<const char*>(s + (size_t)plane * stride);
+    return *reinterpret_cast<const float*>(p + (i << 2));
+}
+MPM_DEV void st(float* s, uint32_t stride, int plane, uint32_t i, float v) {
+    char* p = reinterpret_cast<char*>(s + (size_t)plane * stride);
+    *reinterpret_cast<float*>(p + (i << 2)) = v;
+}
 MPM_DEV v3 ld3(const float* s, uint32_t stride, int plane, uint32_t i) {
     return mk(ld(s, stride, plane, i), ld(s, stride, plane + 1, i), ld(s, stride, plane + 2, i));
 }
