@@ -73,8 +73,8 @@ MPM_DEV void with_scene(const Ens& E, uint32_t sc, F&& f) {
 // lane per neighbour; unbinned particles per lane), listed in key order by the key-space scan
 // (block_key), and each scene's part of the list is found in k_ens_tasks.
 __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, Ens E) {
-    __shared__ uint32_t pb[ENS_MAX_SCENES + 1], kb[ENS_MAX_SCENES + 1];
-    for (uint32_t s = threadIdx.x; s <= E.S; s += blockDim.x) { pb[s] = E.part_base[s]; kb[s] = E.key_base[s]; }
+    __shared__ uint32_t pb[ENS_MAX_SCENES + 1];
+    for (uint32_t s = threadIdx.x; s <= E.S; s += blockDim.x) pb[s] = E.part_base[s];
     const uint32_t gsz = gridDim.x * blockDim.x;
     const uint32_t gi = blockIdx.x * blockDim.x + threadIdx.x;
     if (!P0.direct_blocks)
@@ -87,15 +87,18 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
     for (uint32_t it = 0; it < nloop; ++it) {
         const uint32_t i = it * gsz + gi;
         bool valid = false;
-        uint32_t key = EMPTY, mask = 0;
+        uint32_t key = EMPTY, mask = 0, sc = 0;
+        int bxyz[3] = {0, 0, 0};     // block coordinates of the key, in its scene
         v3 pos = {0.0f, 0.0f, 0.0f};
         if (i < n) pos = ld3(B.cur, B.stride, PX, i);
         count_nonfinite(B, i < n, pos);
         if (i < n) {
-            with_scene(E, seg_find(pb, E.S, B.pid_cur[i]), [&](const SimParams& P) {
+            sc = seg_find(pb, E.S, B.pid_cur[i]);
+            with_scene(E, sc, [&](const SimParams& P) {
                 int c[3]; cell_number(P, pos, c);
                 mask = touched_mask(P, c);
                 int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
+                bxyz[0] = bx; bxyz[1] = by; bxyz[2] = bz;
                 if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
                     key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
                     valid = true;
@@ -131,17 +134,15 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
         if (P0.direct_blocks) {
             uint64_t fl = __ballot(fresh != 0u);
             while (fl) {
-                // the key and its new bits are the same in every lane: scene and parameters read
-                // once per key; each (key, neighbour) pair is flagged once
+                // the leader's new bits, scene and block coordinates, read from its lane once per
+                // key (scalar); each (key, neighbour) pair is flagged once, one lane per neighbour
                 const int l = __ffsll((unsigned long long)fl) - 1;
                 const int j = lane_id();
-                const uint32_t uk = __builtin_amdgcn_readfirstlane(__shfl(key, l, 64));
-                const uint32_t um = __builtin_amdgcn_readfirstlane(__shfl(fresh, l, 64));
-                const SimParams& P = E.sp[__builtin_amdgcn_readfirstlane(seg_find(kb, E.S, uk))];
-                if (j < 27 && ((um >> j) & 1u)) {
-                    int bc[3]; block_coords(P, uk, bc);
-                    B.kalloc[block_key_at(P, bc[0], bc[1], bc[2], j)] = 1u;
-                }
+                const uint32_t um = (uint32_t)__builtin_amdgcn_readlane((int)fresh, l);
+                const SimParams& P = E.sp[(uint32_t)__builtin_amdgcn_readlane((int)sc, l)];
+                const int b0 = __builtin_amdgcn_readlane(bxyz[0], l), b1 = __builtin_amdgcn_readlane(bxyz[1], l),
+                          b2 = __builtin_amdgcn_readlane(bxyz[2], l);
+                if (j < 27 && ((um >> j) & 1u)) B.kalloc[block_key_at(P, b0, b1, b2, j)] = 1u;
                 fl &= fl - 1ull;
             }
         }
