@@ -233,8 +233,9 @@ def test_keys_carried_between_substeps_follow_the_state(mpm, orc):
     """A substep leaves the next one's block keys (made where positions become final). They are
     used only for the state and grid they describe: after the grid is moved between calls
     (writeGridMinCoords / writeGridMaxCoords, 2.5 cells in x and 1.25 in y, so keys change and some
-    particles leave the grid), after a fresh upload, and across a switch between the explicit and
-    PBMPM methods every substep still equals the oracle given the same uniforms and particles."""
+    particles leave the grid), after a fresh upload, across a switch between the explicit and PBMPM
+    methods and in the 8-substep run graphs, every substep still equals the oracle given the same
+    uniforms and particles."""
     sc = _small_pbmpm(mpm, n=6000, seed=31)
     sim, o = make_pair(mpm, orc, sc, use_graph=1)
     sim.step(1, 3)
@@ -265,6 +266,15 @@ def test_keys_carried_between_substeps_follow_the_state(mpm, orc):
     sim.step(1, 2)
     o.step(1, 2)
     assert_particles_equal(mpm, sim.download(), o.particles(), "after switching methods")
+    # the run graphs (8 substeps in one replay, prepared for the steady state): a call that starts
+    # right after the grid moves back runs its first substep eagerly, then replays
+    sim.prepare_graphs(1)
+    um.writeGridMinCoords(list(mn))
+    um.writeGridMaxCoords(list(mx))
+    o.set_uniforms(sim.uniform_bytes())
+    sim.step(1, 11)
+    o.step(1, 11)
+    assert_particles_equal(mpm, sim.download(), o.particles(), "run graphs after the grid moved back")
     sim.close()
 
 
