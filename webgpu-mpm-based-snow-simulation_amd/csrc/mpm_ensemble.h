@@ -68,6 +68,71 @@ MPM_DEV void with_scene(const Ens& E, uint32_t sc, F&& f) {
 }
 
 // ------------------------------------------------------------------ binning over all scenes
+// One particle of a lane in the batched keys pass: its scene, key, mask and block coordinates (the
+// particle's own scene's parameters), and its place in the wave's key groups.
+struct EnsKey {
+    bool valid;
+    uint32_t key, mask, sc;
+    int b[3];
+    int leader;                 // lowest lane of the wave with the same key
+    uint32_t rank;              // lanes before this one with the same key
+    uint32_t lead_cnt, lead_m;  // on a leader: its key's lanes and the OR of their masks
+};
+MPM_DEV EnsKey ens_key_of(const SimParams& P0, const Bufs& B, const Ens& E, const uint32_t* pb, bool active, uint32_t i) {
+    EnsKey k; k.valid = false; k.key = EMPTY; k.mask = 0u; k.sc = 0u; k.b[0] = k.b[1] = k.b[2] = 0;
+    v3 pos = {0.0f, 0.0f, 0.0f};
+    if (active) pos = ld3(B.cur, B.stride, PX, i);
+    count_nonfinite(B, active, pos);
+    if (active) {
+        k.sc = seg_find(pb, E.S, B.pid_cur[i]);
+        with_scene(E, k.sc, [&](const SimParams& P) {
+            int c[3]; cell_number(P, pos, c);
+            k.mask = touched_mask(P, c);
+            const int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
+            k.b[0] = bx; k.b[1] = by; k.b[2] = bz;
+            if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
+                k.key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
+                k.valid = true;
+            } else if (P0.direct_blocks && k.mask) {
+                for (int jj = 0; jj < 27; ++jj)
+                    if ((k.mask >> jj) & 1u) B.kalloc[block_key_at(P, bx, by, bz, jj)] = 1u;
+            }
+        });
+        B.key[i] = k.key; B.mask[i] = k.mask;
+    }
+    return k;
+}
+// Groups the wave's lanes by key (as group_keys): the lowest lane of each key leads it.
+MPM_DEV void ens_group(EnsKey& k) {
+    uint64_t active = __ballot(k.valid);
+    k.leader = 0; k.rank = 0; k.lead_cnt = 0; k.lead_m = 0;
+    while (active) {
+        const int leader = __ffsll((unsigned long long)active) - 1;
+        const uint32_t lk = __shfl(k.key, leader, 64);
+        const bool peer = k.valid && k.key == lk;
+        const uint64_t peers = __ballot(peer);
+        const uint32_t m = wave_or(peer ? k.mask : 0u);
+        if (peer) { k.leader = leader; k.rank = (uint32_t)__popcll(peers & lanemask_lt()); }
+        if (lane_id() == leader) { k.lead_cnt = (uint32_t)__popcll(peers); k.lead_m = m; }
+        active &= ~peers;
+    }
+}
+// Flags each touched block whose (key, neighbour) bit this wave's OR set first: the leader's new
+// bits, scene and block coordinates are read from its lane once per key (scalar), one lane per
+// neighbour.
+MPM_DEV void ens_flag_fresh(const Bufs& B, const Ens& E, const EnsKey& k, uint32_t fresh) {
+    uint64_t fl = __ballot(fresh != 0u);
+    while (fl) {
+        const int l = __ffsll((unsigned long long)fl) - 1;
+        const int j = lane_id();
+        const uint32_t um = (uint32_t)__builtin_amdgcn_readlane((int)fresh, l);
+        const SimParams& P = E.sp[(uint32_t)__builtin_amdgcn_readlane((int)k.sc, l)];
+        const int b0 = __builtin_amdgcn_readlane(k.b[0], l), b1 = __builtin_amdgcn_readlane(k.b[1], l),
+                  b2 = __builtin_amdgcn_readlane(k.b[2], l);
+        if (j < 27 && ((um >> j) & 1u)) B.kalloc[block_key_at(P, b0, b1, b2, j)] = 1u;
+        fl &= fl - 1ull;
+    }
+}
 // Direct addressing (P0.direct_blocks): a scene's dense block range is its key range, so a block's
 // index is its key (kslot[key] = key). The touched blocks are flagged here (per wave and key, one
 // lane per neighbour; unbinned particles per lane), listed in key order by the key-space scan
@@ -83,69 +148,23 @@ __global__ __launch_bounds__(256) void k_ens_keys_count(SimParams P0, Bufs B, En
     if (gi == 0) { B.counters[C_STRAG] = 0; B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; B.counters[C_NCOLL] = 0; B.counters[C_FALLBACK] = 0; B.counters[C_SVD64] = 0; }
     __syncthreads();
     const uint32_t n = P0.n;
-    const uint32_t nloop = (n + gsz - 1) / gsz;
+    // two particles per lane per iteration (i0 and i0 + gsz), their key groups' atomics issued
+    // together; every lane runs every iteration (the wave-wide groupings need all lanes)
+    const uint32_t nloop = (n + 2u * gsz - 1u) / (2u * gsz);
     for (uint32_t it = 0; it < nloop; ++it) {
-        const uint32_t i = it * gsz + gi;
-        bool valid = false;
-        uint32_t key = EMPTY, mask = 0, sc = 0;
-        int bxyz[3] = {0, 0, 0};     // block coordinates of the key, in its scene
-        v3 pos = {0.0f, 0.0f, 0.0f};
-        if (i < n) pos = ld3(B.cur, B.stride, PX, i);
-        count_nonfinite(B, i < n, pos);
-        if (i < n) {
-            sc = seg_find(pb, E.S, B.pid_cur[i]);
-            with_scene(E, sc, [&](const SimParams& P) {
-                int c[3]; cell_number(P, pos, c);
-                mask = touched_mask(P, c);
-                int bx = c[0] >> 2, by = c[1] >> 2, bz = c[2] >> 2;
-                bxyz[0] = bx; bxyz[1] = by; bxyz[2] = bz;
-                if (bx >= 0 && by >= 0 && bz >= 0 && bx < P.nb[0] && by < P.nb[1] && bz < P.nb[2]) {
-                    key = P.kbase + (uint32_t)bx + (uint32_t)P.nb[0] * ((uint32_t)by + (uint32_t)P.nb[1] * (uint32_t)bz);
-                    valid = true;
-                } else if (P0.direct_blocks && mask) {
-                    for (int jj = 0; jj < 27; ++jj)
-                        if ((mask >> jj) & 1u) B.kalloc[block_key_at(P, bx, by, bz, jj)] = 1u;
-                }
-            });
-            B.key[i] = key; B.mask[i] = mask;
-        }
-        // grouped by key as in k_keys_count: one round trip of the leaders' atomics per wave
-        uint64_t active = __ballot(valid);
-        int my_leader = 0;
-        uint32_t my_rank = 0, lead_cnt = 0, lead_m = 0;
-        while (active) {
-            const int leader = __ffsll((unsigned long long)active) - 1;
-            const uint32_t lk = __shfl(key, leader, 64);
-            const bool peer = valid && key == lk;
-            const uint64_t peers = __ballot(peer);
-            const uint32_t m = wave_or(peer ? mask : 0u);
-            if (peer) { my_leader = leader; my_rank = (uint32_t)__popcll(peers & lanemask_lt()); }
-            if (lane_id() == leader) { lead_cnt = (uint32_t)__popcll(peers); lead_m = m; }
-            active &= ~peers;
-        }
-        const bool lead = valid && my_leader == lane_id();
-        uint32_t b = 0, fresh = 0;
-        if (lead) {
-            b = atomicAdd(&B.kcount[key], lead_cnt);
-            fresh = lead_m & ~atomicOr(&B.kmask[key], lead_m);
-        }
-        b = __shfl(b, my_leader, 64);
-        if (valid) B.lofs[i] = b + my_rank;
-        if (P0.direct_blocks) {
-            uint64_t fl = __ballot(fresh != 0u);
-            while (fl) {
-                // the leader's new bits, scene and block coordinates, read from its lane once per
-                // key (scalar); each (key, neighbour) pair is flagged once, one lane per neighbour
-                const int l = __ffsll((unsigned long long)fl) - 1;
-                const int j = lane_id();
-                const uint32_t um = (uint32_t)__builtin_amdgcn_readlane((int)fresh, l);
-                const SimParams& P = E.sp[(uint32_t)__builtin_amdgcn_readlane((int)sc, l)];
-                const int b0 = __builtin_amdgcn_readlane(bxyz[0], l), b1 = __builtin_amdgcn_readlane(bxyz[1], l),
-                          b2 = __builtin_amdgcn_readlane(bxyz[2], l);
-                if (j < 27 && ((um >> j) & 1u)) B.kalloc[block_key_at(P, b0, b1, b2, j)] = 1u;
-                fl &= fl - 1ull;
-            }
-        }
+        const uint32_t i0 = it * 2u * gsz + gi, i1 = i0 + gsz;
+        EnsKey a = ens_key_of(P0, B, E, pb, i0 < n, i0);
+        EnsKey c = ens_key_of(P0, B, E, pb, i1 < n, i1);
+        ens_group(a);
+        ens_group(c);
+        uint32_t ba = 0, bc = 0, fa = 0, fc = 0;
+        if (a.valid && a.leader == lane_id()) { ba = atomicAdd(&B.kcount[a.key], a.lead_cnt); fa = a.lead_m & ~atomicOr(&B.kmask[a.key], a.lead_m); }
+        if (c.valid && c.leader == lane_id()) { bc = atomicAdd(&B.kcount[c.key], c.lead_cnt); fc = c.lead_m & ~atomicOr(&B.kmask[c.key], c.lead_m); }
+        ba = __shfl(ba, a.leader, 64);
+        bc = __shfl(bc, c.leader, 64);
+        if (a.valid) B.lofs[i0] = ba + a.rank;
+        if (c.valid) B.lofs[i1] = bc + c.rank;
+        if (P0.direct_blocks) { ens_flag_fresh(B, E, a, fa); ens_flag_fresh(B, E, c, fc); }
     }
 }
 
