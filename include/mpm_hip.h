@@ -216,9 +216,9 @@ int  mpm_splat_noalloc(mpm_ctx* ctx, int kind, const uint32_t* res, uint32_t* ho
 
 /* addExplicitMpmDispatches / addPbmpmDispatches x n_substeps, then queue.submit
  * (GpuSnowPipelineRunner.svelte.ts:392-445, GpuMpmPipelineManager.ts:327-521). Enqueues on the
- * context's stream and returns; mpm_sync waits. With blocks addressed by key (hash_map_size 0) each
- * substep also makes the next one's block keys; the first substep after an upload, a scatter or a
- * change of the grid uniforms makes its own and runs without a graph. */
+ * context's stream and returns; mpm_sync waits. Each substep also makes the next one's block keys;
+ * the first substep after an upload, a scatter or a change of the grid uniforms makes its own and
+ * runs without a graph. */
 int  mpm_step(mpm_ctx* ctx, int method, int n_substeps);
 int  mpm_sync(mpm_ctx* ctx);
 /* No reference counterpart (the reference creates its compute pipelines up front,

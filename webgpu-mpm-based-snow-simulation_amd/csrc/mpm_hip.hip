@@ -441,7 +441,7 @@ int make_params(mpm_ctx* c, SimParams& P) {
     P.per_particle_lame = c->per_particle_lame ? 1u : 0u;
     // one scene: chunks longest first (a pass hands them out in list order and ends on the short ones)
     P.chunk_lpt = 1u;
-    P.fuse_keys = c->direct_blocks ? 1u : 0u;
+    P.fuse_keys = 1u;
     P.keys_ready = 0u;       // set per substep by keys_state
     return rc;
 }

@@ -78,9 +78,9 @@ struct SimParams {
     float rdt;                 // RN(1/dt) for fixed_div
     uint32_t fast_dt;          // fixed_div(., dt, rdt, S) verified equal to i32((x/dt) S) for all x
     uint32_t direct_blocks;    // block set by direct key addressing (no hash table)
-    uint32_t fuse_keys;        // one scene, direct addressing: the passes that leave a particle's position
-                               // final (LAST or the explicit G2P, the collider, the unbinned particles)
-                               // also do the next substep's keys pass for it (bin_next)
+    uint32_t fuse_keys;        // one scene: the passes that leave a particle's position final (LAST or
+                               // the explicit G2P, the collider, the unbinned particles) also do the
+                               // next substep's keys pass for it (bin_next)
     uint32_t keys_ready;       // the keys of the rows this substep bins were made by the previous
                                // substep (fuse_keys): k_keys_count is not launched
     uint32_t block_identity;   // direct addressing with dense block index = key (batched scenes)
@@ -466,9 +466,11 @@ MPM_DEV void bin_next(const SimParams& P, const Bufs& B, bool active, uint32_t q
     fresh = a.lead_m & ~fresh;
     b = __shfl(b, a.leader, 64);
     if (a.valid) B.lofs[q] = b + a.rank;
-    const int jl = lane_id();
-    const uint32_t koff = (uint32_t)((jl % 3 - 1) + P.nb[0] * ((jl / 3) % 3 - 1 + P.nb[1] * (jl / 9 - 1)));
-    flag_fresh(B, a, fresh, koff);
+    if (P.direct_blocks) {
+        const int jl = lane_id();
+        const uint32_t koff = (uint32_t)((jl % 3 - 1) + P.nb[0] * ((jl / 3) % 3 - 1 + P.nb[1] * (jl / 9 - 1)));
+        flag_fresh(B, a, fresh, koff);
+    }
 }
 __global__ __launch_bounds__(256) void k_keys_count(SimParams P, Bufs B) {
     const uint32_t gsz = gridDim.x * blockDim.x;
@@ -637,6 +639,12 @@ __global__ __launch_bounds__(256) void k_scatter(SimParams P, Bufs B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t nvalid = B.counters[C_NVALID];
     if (i == 0) { B.counters[C_NONFIN_LAST] = B.counters[C_NONFIN]; B.counters[C_NONFIN] = 0u; }
+    if (!P.direct_blocks) {
+        // the block table of this substep's inserts (clearHashMap.wgsl:5-19), here because the keys
+        // may have been made by the previous substep
+        for (uint32_t s = i; s < P.cap; s += gridDim.x * blockDim.x) B.table[s] = EMPTY;
+        if (i == 0) { B.counters[C_NBLOCKS] = 0; B.counters[C_DROPS] = 0; B.counters[C_PROBEFAIL] = 0; }
+    }
     const uint32_t key = i < P.n ? B.key[i] : EMPTY;
     const bool strag = i < P.n && key == EMPTY;
     if (i < P.n && key != EMPTY) {
