@@ -278,6 +278,31 @@ def test_keys_carried_between_substeps_follow_the_state(mpm, orc):
     sim.close()
 
 
+@pytest.mark.parametrize("method", [1, 0])
+def test_non_cubic_partial_block_grid_substeps_bitexact(mpm, orc, method):
+    """A 30 x 22 x 41 grid over a non-cubic box (blocks of 4 cells, the last block along each axis
+    partly outside the grid, cells of three different sizes) with the box collider: the keys a
+    substep makes for the next use the grid's own block counts per axis, for both methods."""
+    rng = np.random.default_rng(61)
+    n = 7000
+    pos = (np.array([0.2, 0.15, 0.25]) + rng.uniform(0.0, 1.0, (n, 3)) * np.array([0.6, 0.5, 0.5])).astype(np.float32)
+    vel = rng.uniform(-1.5, 1.5, (n, 3)).astype(np.float32)
+    p = mpm.make_particles(pos, vel=vel)
+    sim_kw = dict(grid_res=(30, 22, 41), grid_min=(0.0, 0.0, 0.0), grid_max=(1.0, 0.75, 1.1), gravity=(0.0, -9.8, 0.0))
+    if method == 1:
+        p["dx"][:, :3] = vel * np.float32(1 / 600)
+        sim_kw.update(dt=1.0 / 600, pbmpm_iterations=3, pbmpm_force_divisor=3.0)
+    else:
+        sim_kw.update(dt=1e-4)
+    sc = dict(particles=p, sim=sim_kw, collider=mpm.scenes.box_mesh((0.5, 0.4, 0.55), 0.25))
+    sim, o = make_pair(mpm, orc, sc, use_graph=1)
+    for s in range(5):
+        sim.step(method, 1)
+        o.step(method, 1)
+        assert_particles_equal(mpm, sim.download(), o.particles(), f"method {method}, non-cubic grid, substep {s}")
+    sim.close()
+
+
 def test_explicit_with_collider_substeps_bitexact(mpm, orc):
     """Explicit substeps against a box collider that the falling cube overlaps: the G2P pass makes
     the next substep's keys of the particles it finalises and the collider pass those of the queued
