@@ -8,12 +8,16 @@ pytestmark = pytest.mark.gpu
 
 
 def test_full_pbmpm_scene_one_substep_bitexact(mpm, orc):
-    """The 1,048,576-particle two-sphere scene (k = 4, box collider): one substep, every field."""
+    """The 1,048,576-particle two-sphere scene (k = 4, box collider): one substep, every field, and
+    two more, binned from the keys the LAST and collider passes made."""
     sc = mpm.scenes.scene("pbmpm_two_spheres")
     sim, o = make_pair(mpm, orc, sc)
     sim.step(1, 1)
     o.step(1, 1)
     assert_particles_equal(mpm, sim.download(), o.particles(), "1M PBMPM substep")
+    sim.step(1, 2)
+    o.step(1, 2)
+    assert_particles_equal(mpm, sim.download(), o.particles(), "1M PBMPM substeps 2-3")
     st = sim.stats()
     assert st["particles_unbinned"] == 0 and st["blocks_dropped"] == 0 and st["probe_failures"] == 0
     sim.close()
