@@ -64,17 +64,44 @@ def _records(mpm, n, rng):
     return p, fe, fp, d
 
 
-def test_warmstart_inference_matches_oracle(mpm):
+def _special_fe(rng):
+    """F_e whose singular values the features must get right where a 3x3 eigen-solver is weakest:
+    the identity and pure rotations (all three equal; F^T F diagonal, every Jacobi pivot 0), diagonal
+    stretches (pivots 0, values distinct), and rotated stretches with two equal singular values."""
+    out = [np.eye(3), np.diag([1.1, 0.95, 1.0]), np.diag([0.9, 1.0, 1.1])]
+    for sig in ([1.0, 1.0, 1.0], [1.05, 1.05, 0.92], [1.08, 0.97, 0.97], [0.93, 1.02, 0.93]):
+        for _ in range(6):
+            u, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+            v, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+            u *= np.sign(np.linalg.det(u))
+            v *= np.sign(np.linalg.det(v))
+            out.append(u @ np.diag(sig) @ v.T)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("sig_sd", [None, 0.03])
+def test_warmstart_inference_matches_oracle(mpm, sig_sd):
     """One warm start over 4,000 particles: D and dx plus the destandardised predictions per
     particle, against the oracle network on features from numpy's SVD; a particle whose dprev is
-    0 (no iteration reached it) keeps plain PBMPM and counts as a fallback."""
+    0 (no iteration reached it) keeps plain PBMPM and counts as a fallback. The first particles
+    carry the special F_e of _special_fe. With sig_sd the singular-value features are standardised
+    as a trained set standardises them (mean 1, standard deviation sig_sd): an error of 1e-3 in a
+    singular value then moves its network input by 1e-3 / sig_sd."""
     rng = np.random.default_rng(12)
     n = 4000
     p, fe, fp, d = _records(mpm, n, rng)
+    sp = _special_fe(rng)
+    fe[:len(sp)] = sp
+    p["Fe"][:len(sp), :, :3] = np.swapaxes(sp, 1, 2)
     p["dx"][:, :3] = 1e-3 * rng.standard_normal((n, 3))
     dprev = (2e-5 * rng.standard_normal((n, 3))).astype(np.float32)
     dprev[::50] = 0.0
     params, norm = _net(4, out_scale=0.5)
+    if sig_sd is not None:
+        nm = norm.reshape(4, 16)
+        nm[0, :3] = 1.0
+        nm[1, :3] = sig_sd
+        norm = nm.ravel()
     with mpm.MpmSimulation(n, grid_res=(64, 64, 64), grid_min=(0, 0, 0), grid_max=(1, 1, 1), dt=1 / 600,
                            pbmpm_iterations=2, pbmpm_force_divisor=2.0) as sim:
         sim.upload(p)
@@ -97,6 +124,8 @@ def test_warmstart_inference_matches_oracle(mpm):
     assert np.quantile(err, 0.99) < 0.02 and err.max() < 0.1
     errx = np.abs(out["dx"][:, :3] - want_dx) / nm[3, 9:12]
     assert np.quantile(errx, 0.99) < 0.02 and errx.max() < 0.1
+    k = len(sp)
+    assert err[:k].max() < 0.1 and errx[:k].max() < 0.1
     assert np.array_equal(out["dx"][~used, :3], p["dx"][~used, :3]) and np.array_equal(out["D"][~used], p["D"][~used])
     # everything but D and dx is untouched
     assert np.array_equal(out["pos"], p["pos"]) and np.array_equal(out["Fe"], p["Fe"]) and np.array_equal(out["vel"], p["vel"])

@@ -530,7 +530,7 @@ bool enqueue_substep(mpm_ctx* c, int method, const SimParams& P, int stage, bool
     if (method == MPM_METHOD_PBMPM) {
         const int k = P.k;
         if (P.learn_flags & 1u) {
-            k_mlp_warmstart<<<std::min<uint32_t>(grid_for(c->n, 1u << 30), (uint32_t)c->n_cu * 2u), 256, 0, s>>>(
+            k_mlp_warmstart<<<std::min<uint32_t>(grid_for(c->n, 1u << 30), (uint32_t)c->n_cu * 4u), 256, 0, s>>>(
                 mlp_bufs(c), B, c->n, P.learn_limit, c->counters + C_FALLBACK);
             kt(c, KID_MLP);
         }

@@ -15,7 +15,9 @@
 // MFMA 16x16x16 bf16 operand layout (lane l of a wave64):
 //   A[row = l % 16][k = 4 (l / 16) + j], B[k = 4 (l / 16) + j][col = l % 16],
 //   D[row = 4 (l / 16) + i][col = l % 16],  i, j = 0..3.
-// A wave processes 16 samples at a time; activations are transposed through LDS between layers.
+// A wave processes 16 samples at a time. Training transposes the activations through LDS between
+// layers (the backward pass reuses them there); inference computes the transposed products and
+// keeps them in registers (mlp_infer16).
 #pragma once
 #include "mpm_kernels.h"
 
@@ -43,17 +45,21 @@ MPM_DEV short to_bf16(float x) {      // round to nearest even
 }
 MPM_DEV float from_bf16(short b) { return __uint_as_float(((uint32_t)(uint16_t)b) << 16); }
 
-// GELU (tanh form) and its derivative; tanh(u) = 1 - 2 / (exp(2u) + 1) with fexp and the
-// hardware reciprocal (v_rcp_f32, 1 ulp): the network's arithmetic is BF16-limited, and 128 of
-// these per sample dominate the warm start's VALU work with a correctly rounded divide.
-MPM_DEV float gelu_t(float z) {
-    const float u = 0.7978845608f * (z + 0.044715f * ((z * z) * z));
-    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(fexp(2.0f * u) + 1.0f);
+// GELU (tanh form) and its derivative through the logistic function: 0.5 (1 + tanh u) = s(2u),
+// s(a) = 1 / (1 + exp(-a)), u = sqrt(2/pi) (z + 0.044715 z^3), so GELU(z) = z s(2u) and
+// GELU'(z) = s + 2 z s (1 - s) u'. exp(-2u) = exp2(z (K0 + K1 z^2)) with K0 = -2 sqrt(2/pi) log2(e),
+// K1 = 0.044715 K0, on the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32, 1 ulp each): the
+// network's arithmetic is BF16-limited, and 128 of these per sample are most of the warm start's
+// VALU work. No cancellation for negative z (1 + tanh u is never formed), and exp2 -> inf for very
+// negative z gives s = 0.
+MPM_DEV float gelu_s(float z) {
+    const float a = z * __builtin_fmaf(-0.10294324f, z * z, -2.3022082f);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(a));
 }
-MPM_DEV float gelu(float z) { return (0.5f * z) * (1.0f + gelu_t(z)); }
+MPM_DEV float gelu(float z) { return z * gelu_s(z); }
 MPM_DEV float gelu_grad(float z) {
-    const float t = gelu_t(z);
-    return 0.5f * (1.0f + t) + ((0.5f * z) * (1.0f - t * t)) * (0.7978845608f * (1.0f + 0.134145f * (z * z)));
+    const float s = gelu_s(z);
+    return s + ((2.0f * z) * (s * (1.0f - s))) * (0.7978845608f * (1.0f + 0.134145f * (z * z)));
 }
 
 struct MlpBufs {
@@ -63,23 +69,28 @@ struct MlpBufs {
     const float* norm;        // [4][16]: feature mean, feature std, target mean, target std
 };
 
-// LDS images of the weights in MFMA B-operand order, 8 bytes per lane per tile.
+// LDS images of the weights in MFMA B-operand order, 8 bytes per lane per tile (training's
+// Z = X W; inference's Z^T = W^T X^T reads the same images as A operands).
 //   w1b[n][l]      = W1[4(l/16)+j][16n + l%16]                     (layer 1, B = W1)
 //   w2b[kk][n][l]  = W2[16kk + 4(l/16)+j][16n + l%16]              (layer 2, B = W2)
 //   w3b[kk][l]     = W3[16kk + 4(l/16)+j][l%16]                    (layer 3, B = W3)
+// The backward images (training only, MlpLdsBack):
 //   w3t[n][l]      = W3[16n + l%16][4(l/16)+j]                     (backward dH2 = dY W3^T)
 //   w2t[kk][n][l]  = W2[16n + l%16][16kk + 4(l/16)+j]              (backward dH1 = dZ2 W2^T)
+// isd_x = 1 / sd_x: features are standardised as (x - mu_x) * isd_x.
 struct MlpLds {
     bf4 w1b[4][64];
     bf4 w2b[4][4][64];
     bf4 w3b[4][64];
+    float b1[64], b2[64], b3[16];
+    float mu_x[16], isd_x[16], mu_y[16], sd_y[16];
+};
+struct MlpLdsBack {
     bf4 w3t[4][64];
     bf4 w2t[4][4][64];
-    float b1[64], b2[64], b3[16];
-    float mu_x[16], sd_x[16], mu_y[16], sd_y[16];
 };
 
-MPM_DEV void mlp_stage_weights(MlpLds& L, const MlpBufs& M, bool backward) {
+MPM_DEV void mlp_stage_weights(MlpLds& L, MlpLdsBack* T, const MlpBufs& M) {
     const short* W1 = M.wbf;
     const short* W2 = M.wbf + 16 * 64;
     const short* W3 = M.wbf + 16 * 64 + 64 * 64;
@@ -91,7 +102,8 @@ MPM_DEV void mlp_stage_weights(MlpLds& L, const MlpBufs& M, bool backward) {
             c[j] = W3[(16 * n + 4 * (l / 16) + j) * 16 + l % 16];
             d[j] = W3[(16 * n + l % 16) * 16 + 4 * (l / 16) + j];
         }
-        L.w1b[n][l] = a; L.w3b[n][l] = c; L.w3t[n][l] = d;
+        L.w1b[n][l] = a; L.w3b[n][l] = c;
+        if (T) T->w3t[n][l] = d;
     }
     for (int e = threadIdx.x; e < 16 * 64; e += blockDim.x) {
         const int kk = e / 256, n = (e / 64) % 4, l = e % 64;
@@ -101,12 +113,12 @@ MPM_DEV void mlp_stage_weights(MlpLds& L, const MlpBufs& M, bool backward) {
             b[j] = W2[(16 * n + l % 16) * 64 + 16 * kk + 4 * (l / 16) + j];
         }
         L.w2b[kk][n][l] = a;
-        if (backward) L.w2t[kk][n][l] = b;
+        if (T) T->w2t[kk][n][l] = b;
     }
     for (int e = threadIdx.x; e < 64; e += blockDim.x) { L.b1[e] = M.params[MLP_B1 + e]; L.b2[e] = M.params[MLP_B2 + e]; }
     for (int e = threadIdx.x; e < 16; e += blockDim.x) {
         L.b3[e] = M.params[MLP_B3 + e];
-        L.mu_x[e] = M.norm[e]; L.sd_x[e] = M.norm[16 + e]; L.mu_y[e] = M.norm[32 + e]; L.sd_y[e] = M.norm[48 + e];
+        L.mu_x[e] = M.norm[e]; L.isd_x[e] = 1.0f / M.norm[16 + e]; L.mu_y[e] = M.norm[32 + e]; L.sd_y[e] = M.norm[48 + e];
     }
 }
 
@@ -183,9 +195,10 @@ struct MlpTrainArgs {
 };
 __global__ __launch_bounds__(256) void k_mlp_train(MlpBufs M, MlpTrainArgs T) {
     __shared__ MlpLds L;
+    __shared__ MlpLdsBack LT;
     __shared__ MlpWave Wv[4];
     __shared__ float G[MLP_NPARAM];
-    mlp_stage_weights(L, M, true);
+    mlp_stage_weights(L, &LT, M);
     for (int e = threadIdx.x; e < MLP_NPARAM; e += blockDim.x) G[e] = 0.0f;
     __syncthreads();
     const int w = threadIdx.x >> 6, l = lane_id(), r = l % 16, q = l / 16;
@@ -203,7 +216,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpBufs M, MlpTrainArgs T) {
         // features: lane (r, q) loads 4 and normalises them
         for (int j = 0; j < 4; ++j) {
             const int f = 4 * q + j;
-            const float x = f < MLP_NIN ? (T.feats[s * 16 + f] - L.mu_x[f]) / L.sd_x[f] : 0.0f;
+            const float x = f < MLP_NIN ? (T.feats[s * 16 + f] - L.mu_x[f]) * L.isd_x[f] : 0.0f;
             S.xs[r * 16 + f] = to_bf16(x);
         }
         wave_sync();
@@ -241,7 +254,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpBufs M, MlpTrainArgs T) {
 #pragma unroll
         for (int n = 0; n < 4; ++n) {
             f4 c = {0.f, 0.f, 0.f, 0.f};
-            c = MFMA(dya, L.w3t[n][l], c);
+            c = MFMA(dya, LT.w3t[n][l], c);
             for (int i = 0; i < 4; ++i) c[i] = c[i] * gelu_grad(A.z2[n][i]);
             dz2[n] = c;
         }
@@ -267,7 +280,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpBufs M, MlpTrainArgs T) {
             for (int kk = 0; kk < 4; ++kk) {
                 bf4 da;
                 for (int j = 0; j < 4; ++j) da[j] = S.ts[r * 64 + 16 * kk + 4 * q + j];
-                c = MFMA(da, L.w2t[kk][n][l], c);
+                c = MFMA(da, LT.w2t[kk][n][l], c);
             }
             bf4 db;
             for (int i = 0; i < 4; ++i) { const float dz = c[i] * gelu_grad(A.z1[n][i]); db[i] = to_bf16(dz); gb1[n] += dz; }
@@ -321,8 +334,9 @@ __global__ void k_mlp_refresh_bf16(const float* params, short* wbf) {
 __global__ __launch_bounds__(256) void k_mlp_grad_fixed(MlpBufs M, const float* feats, const float* targs, uint32_t n,
                                                         float* out_y, float* loss_sum) {
     __shared__ MlpLds L;
+    __shared__ MlpLdsBack LT;
     __shared__ MlpWave Wv[4];
-    mlp_stage_weights(L, M, true);
+    mlp_stage_weights(L, &LT, M);
     __syncthreads();
     const int w = threadIdx.x >> 6, l = lane_id(), r = l % 16, q = l / 16;
     MlpWave& S = Wv[w];
@@ -331,7 +345,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad_fixed(MlpBufs M, const float* 
     const float inv_n = 1.0f / (float)n;
     for (int j = 0; j < 4; ++j) {
         const int f = 4 * q + j;
-        const float x = f < MLP_NIN ? (feats[(size_t)(tile * 16 + r) * 16 + f] - L.mu_x[f]) / L.sd_x[f] : 0.0f;
+        const float x = f < MLP_NIN ? (feats[(size_t)(tile * 16 + r) * 16 + f] - L.mu_x[f]) * L.isd_x[f] : 0.0f;
         S.xs[r * 16 + f] = to_bf16(x);
     }
     wave_sync();
@@ -365,7 +379,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad_fixed(MlpBufs M, const float* 
     f4 dz2[4];
     for (int n2 = 0; n2 < 4; ++n2) {
         f4 c = {0.f, 0.f, 0.f, 0.f};
-        c = MFMA(dya, L.w3t[n2][l], c);
+        c = MFMA(dya, LT.w3t[n2][l], c);
         for (int i = 0; i < 4; ++i) c[i] = c[i] * gelu_grad(A.z2[n2][i]);
         dz2[n2] = c;
     }
@@ -387,7 +401,7 @@ __global__ __launch_bounds__(256) void k_mlp_grad_fixed(MlpBufs M, const float* 
         for (int kk = 0; kk < 4; ++kk) {
             bf4 da;
             for (int j = 0; j < 4; ++j) da[j] = S.ts[r * 64 + 16 * kk + 4 * q + j];
-            c = MFMA(da, L.w2t[kk][n2][l], c);
+            c = MFMA(da, LT.w2t[kk][n2][l], c);
         }
         bf4 db;
         for (int i = 0; i < 4; ++i) { const float dz = c[i] * gelu_grad(A.z1[n2][i]); db[i] = to_bf16(dz); atomicAdd(&M.grads[MLP_B1 + 16 * n2 + r], dz); }
@@ -401,18 +415,52 @@ __global__ __launch_bounds__(256) void k_mlp_grad_fixed(MlpBufs M, const float* 
     if (l == 0) atomicAdd(loss_sum, loss);
 }
 
+// Singular values of F, largest first: square roots of the eigenvalues of C = F^T F by the cyclic
+// Jacobi eigenvalue method (the rotation of Golub & Van Loan, Matrix Computations, Alg. 8.5.1
+// sym.schur2), MLP_JACOBI_SWEEPS sweeps over the pivots (0,1), (0,2), (1,2), eigenvalues only (no
+// eigenvector accumulation), on the hardware reciprocal, square root and reciprocal square root:
+// a network input rounded to bfloat16 needs neither V nor correct rounding. Jacobi eigenvalues are
+// accurate to a few ulp of |C| also where eigenvalues nearly coincide (F near a rotation).
+// t = sign(tau) / (|tau| + sqrt(1 + tau^2)), tau = d / (2 a_pq), d = a_qq - a_pp, is evaluated as
+// 2 a_pq sign(d) / (|d| + sqrt(d^2 + 4 a_pq^2)), without dividing by the pivot: 1 / a_pq overflows
+// for a pivot below 2.9e-39, and with d = 0 that gives 0 * inf. A zero denominator (d and a_pq both
+// 0, or both below 1e-19) skips the rotation (t = 0).
+enum : int { MLP_JACOBI_SWEEPS = 4 };
+MPM_DEV void mlp_jacobi_rotate(float& app, float& aqq, float& apq, float& arp, float& arq) {
+    const float d = aqq - app;
+    const float den = __builtin_fabsf(d) + __builtin_amdgcn_sqrtf(__builtin_fmaf(d, d, (4.0f * apq) * apq));
+    const float t = den > 0.0f ? ((2.0f * apq) * __builtin_copysignf(1.0f, d)) * __builtin_amdgcn_rcpf(den) : 0.0f;
+    const float c = __builtin_amdgcn_rsqf(__builtin_fmaf(t, t, 1.0f)), sn = t * c;
+    app = __builtin_fmaf(-t, apq, app);
+    aqq = __builtin_fmaf(t, apq, aqq);
+    apq = 0.0f;
+    const float rp = arp, rq = arq;
+    arp = c * rp - sn * rq;
+    arq = sn * rp + c * rq;
+}
+MPM_DEV void mlp_singular_values(const m3& F, float& s0, float& s1, float& s2) {
+    float c00 = dot(F.c[0], F.c[0]), c11 = dot(F.c[1], F.c[1]), c22 = dot(F.c[2], F.c[2]);
+    float c01 = dot(F.c[0], F.c[1]), c02 = dot(F.c[0], F.c[2]), c12 = dot(F.c[1], F.c[2]);
+#pragma unroll
+    for (int sweep = 0; sweep < MLP_JACOBI_SWEEPS; ++sweep) {
+        mlp_jacobi_rotate(c00, c11, c01, c02, c12);   // pivot (0,1), other index 2
+        mlp_jacobi_rotate(c00, c22, c02, c01, c12);   // pivot (0,2), other index 1
+        mlp_jacobi_rotate(c11, c22, c12, c01, c02);   // pivot (1,2), other index 0
+    }
+    // a rounding-negative eigenvalue is 0; NaN (a non-finite F) stays NaN, so the prediction is dropped
+    float a = __builtin_amdgcn_sqrtf(c00 < 0.0f ? 0.0f : c00);
+    float b = __builtin_amdgcn_sqrtf(c11 < 0.0f ? 0.0f : c11);
+    float c = __builtin_amdgcn_sqrtf(c22 < 0.0f ? 0.0f : c22);
+    float t;
+    if (a < b) { t = a; a = b; b = t; }
+    if (b < c) { t = b; b = c; c = t; }
+    if (a < b) { t = a; a = b; b = t; }
+    s0 = a; s1 = b; s2 = c;
+}
+
 // Features of a particle at the start of a substep (see the header comment).
 MPM_DEV void mlp_features(const m3& Fe, float Jp, const m3& D, v3 dprev, float f[16]) {
-    float a[9], v[9], b[9], sig[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) { a[3 * c] = Fe.c[c].x; a[3 * c + 1] = Fe.c[c].y; a[3 * c + 2] = Fe.c[c].z; }
-    svd3<float>(a, v, b, sig);
-    // largest first
-    float s0 = sig[0], s1 = sig[1], s2 = sig[2], t;
-    if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
-    if (s1 < s2) { t = s1; s1 = s2; s2 = t; }
-    if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
-    f[0] = s0; f[1] = s1; f[2] = s2;
+    mlp_singular_values(Fe, f[0], f[1], f[2]);
     f[3] = Jp;
     for (int c = 0; c < 3; ++c) { f[4 + 3 * c] = D.c[c].x; f[5 + 3 * c] = D.c[c].y; f[6 + 3 * c] = D.c[c].z; }
     f[13] = dprev.x; f[14] = dprev.y; f[15] = dprev.z;
@@ -499,22 +547,71 @@ __global__ __launch_bounds__(256) void k_mlp_colstats(const float* feats, const 
     }
 }
 
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+// Four floats to bfloat16, round to nearest even (v_cvt_pk_bf16_f32), as the B operand of an MFMA.
+MPM_DEV bf4 bf16x4(float a, float b, float c, float d) {
+    const bf2v lo = __builtin_convertvector((f2v){a, b}, bf2v);
+    const bf2v hi = __builtin_convertvector((f2v){c, d}, bf2v);
+    const uint32_t u0 = __builtin_bit_cast(uint32_t, lo), u1 = __builtin_bit_cast(uint32_t, hi);
+    bf4 o;
+    o[0] = (short)(u0 & 0xFFFFu); o[1] = (short)(u0 >> 16); o[2] = (short)(u1 & 0xFFFFu); o[3] = (short)(u1 >> 16);
+    return o;
+}
+
+// Inference forward of 16 samples with every product transposed, Z^T = W^T X^T: the weight images
+// above are the A operands (A[row = unit][k] = W[k][unit] is the lane image w1b / w2b / w3b), the
+// activations the B operands. The D layout of one layer's output block (lane l: units 4(l/16)+i of
+// the block, sample l%16) is the B layout that the next layer reads (k = 4(l/16)+j, column l%16), so
+// the activations stay in registers from layer to layer. xb: lane l holds the standardised
+// features 4(l/16)..4(l/16)+3 of sample l%16. Returns outputs 4(l/16)..4(l/16)+3 of sample l%16.
+MPM_DEV f4 mlp_infer16(const MlpLds& L, bf4 xb) {
+    const int l = lane_id(), q = l / 16;
+    bf4 h[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        f4 c = {0.f, 0.f, 0.f, 0.f};
+        c = MFMA(L.w1b[n][l], xb, c);
+        const f4 b = *reinterpret_cast<const f4*>(&L.b1[16 * n + 4 * q]);
+        h[n] = bf16x4(gelu(c[0] + b[0]), gelu(c[1] + b[1]), gelu(c[2] + b[2]), gelu(c[3] + b[3]));
+    }
+    bf4 h2[4];
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) c = MFMA(L.w2b[kk][n][l], h[kk], c);
+        const f4 b = *reinterpret_cast<const f4*>(&L.b2[16 * n + 4 * q]);
+        h2[n] = bf16x4(gelu(c[0] + b[0]), gelu(c[1] + b[1]), gelu(c[2] + b[2]), gelu(c[3] + b[3]));
+    }
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) c = MFMA(L.w3b[kk][l], h2[kk], c);
+    const f4 b = *reinterpret_cast<const f4*>(&L.b3[4 * q]);
+    return c + b;
+}
+
 // Warm start of a PBMPM substep, per particle in `cur`: D += the predicted D change and dx += the
 // predicted dx change (de-standardised). A particle falls back to the plain method (prediction
 // dropped) when any standardised output exceeds `limit` in magnitude or is not finite, or when no
 // iteration reached it in the previous substep (dprev 0: no input to predict from); the count of
 // fallbacks goes to counters[C_FALLBACK].
-__global__ __launch_bounds__(256) void k_mlp_warmstart(MlpBufs M, Bufs B, uint32_t n, float limit, uint32_t* n_fallback) {
+// Per wave and 64 particles (lane l owns particle base + l): the features into xs (row l, bf16),
+// 4 tiles of 16 through mlp_infer16, the outputs into ys (row = particle, 12 floats: rows 48 B apart
+// keep the 16-byte writes and reads free of bank conflicts), then each lane applies its own row.
+struct MlpInferWave {
+    bf4 xs[64][4];            // [particle][features 4g..4g+3]
+    float ys[64][MLP_NOUT];
+};
+__global__ __launch_bounds__(256, 4) void k_mlp_warmstart(MlpBufs M, Bufs B, uint32_t n, float limit, uint32_t* n_fallback) {
     __shared__ MlpLds L;
-    __shared__ MlpWave Wv[4];
-    __shared__ float ys[4][64][17];
-    mlp_stage_weights(L, M, false);
+    __shared__ MlpInferWave Wv[4];
+    mlp_stage_weights(L, nullptr, M);
     __syncthreads();
     const int w = threadIdx.x >> 6, l = lane_id(), r = l % 16, q = l / 16;
-    MlpWave& S = Wv[w];
+    MlpInferWave& S = Wv[w];
     const uint32_t sd = B.stride;
     for (uint32_t base = (blockIdx.x * 4u + (uint32_t)w) * 64u; base < n; base += gridDim.x * 256u) {
-        // lane l owns particle base + l: compute its features
         const uint32_t i = base + (uint32_t)l;
         float f[16];
         m3 D = zero3();
@@ -526,19 +623,21 @@ __global__ __launch_bounds__(256) void k_mlp_warmstart(MlpBufs M, Bufs B, uint32
         } else {
             for (int c = 0; c < 16; ++c) f[c] = L.mu_x[c];
         }
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+            S.xs[l][g] = bf16x4((f[4 * g] - L.mu_x[4 * g]) * L.isd_x[4 * g], (f[4 * g + 1] - L.mu_x[4 * g + 1]) * L.isd_x[4 * g + 1],
+                                (f[4 * g + 2] - L.mu_x[4 * g + 2]) * L.isd_x[4 * g + 2], (f[4 * g + 3] - L.mu_x[4 * g + 3]) * L.isd_x[4 * g + 3]);
+        wave_sync();
         for (int t = 0; t < 4; ++t) {                     // 4 tiles of 16 particles
-            if (l / 16 == t) for (int c = 0; c < 16; ++c) S.xs[(l % 16) * 16 + c] = to_bf16(c < MLP_NIN ? (f[c] - L.mu_x[c]) / L.sd_x[c] : 0.0f);
-            wave_sync();
-            MlpAct A;
-            mlp_forward16(L, S.xs, S.h1s, S.h2s, A);
-            for (int i2 = 0; i2 < 4; ++i2) ys[w][16 * t + 4 * q + i2][r] = A.y[i2];
-            wave_sync();
+            const f4 y = mlp_infer16(L, S.xs[16 * t + r][q]);
+            if (q < MLP_NOUT / 4) *reinterpret_cast<f4*>(&S.ys[16 * t + r][4 * q]) = y;
         }
+        wave_sync();
         bool ok = true;
         if (i < n) {
             float dd[MLP_NOUT];
             for (int c = 0; c < MLP_NOUT; ++c) {
-                const float ys_c = ys[w][l][c];
+                const float ys_c = S.ys[l][c];
                 if (!(ys_c <= limit && ys_c >= -limit)) ok = false;
                 dd[c] = ys_c * L.sd_y[c] + L.mu_y[c];
             }
