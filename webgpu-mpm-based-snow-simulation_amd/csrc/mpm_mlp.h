@@ -415,52 +415,21 @@ __global__ __launch_bounds__(256) void k_mlp_grad_fixed(MlpBufs M, const float* 
     if (l == 0) atomicAdd(loss_sum, loss);
 }
 
-// Singular values of F, largest first: square roots of the eigenvalues of C = F^T F by the cyclic
-// Jacobi eigenvalue method (the rotation of Golub & Van Loan, Matrix Computations, Alg. 8.5.1
-// sym.schur2), MLP_JACOBI_SWEEPS sweeps over the pivots (0,1), (0,2), (1,2), eigenvalues only (no
-// eigenvector accumulation), on the hardware reciprocal, square root and reciprocal square root:
-// a network input rounded to bfloat16 needs neither V nor correct rounding. Jacobi eigenvalues are
-// accurate to a few ulp of |C| also where eigenvalues nearly coincide (F near a rotation).
-// t = sign(tau) / (|tau| + sqrt(1 + tau^2)), tau = d / (2 a_pq), d = a_qq - a_pp, is evaluated as
-// 2 a_pq sign(d) / (|d| + sqrt(d^2 + 4 a_pq^2)), without dividing by the pivot: 1 / a_pq overflows
-// for a pivot below 2.9e-39, and with d = 0 that gives 0 * inf. A zero denominator (d and a_pq both
-// 0, or both below 1e-19) skips the rotation (t = 0).
-enum : int { MLP_JACOBI_SWEEPS = 4 };
-MPM_DEV void mlp_jacobi_rotate(float& app, float& aqq, float& apq, float& arp, float& arq) {
-    const float d = aqq - app;
-    const float den = __builtin_fabsf(d) + __builtin_amdgcn_sqrtf(__builtin_fmaf(d, d, (4.0f * apq) * apq));
-    const float t = den > 0.0f ? ((2.0f * apq) * __builtin_copysignf(1.0f, d)) * __builtin_amdgcn_rcpf(den) : 0.0f;
-    const float c = __builtin_amdgcn_rsqf(__builtin_fmaf(t, t, 1.0f)), sn = t * c;
-    app = __builtin_fmaf(-t, apq, app);
-    aqq = __builtin_fmaf(t, apq, aqq);
-    apq = 0.0f;
-    const float rp = arp, rq = arq;
-    arp = c * rp - sn * rq;
-    arq = sn * rp + c * rq;
-}
-MPM_DEV void mlp_singular_values(const m3& F, float& s0, float& s1, float& s2) {
-    float c00 = dot(F.c[0], F.c[0]), c11 = dot(F.c[1], F.c[1]), c22 = dot(F.c[2], F.c[2]);
-    float c01 = dot(F.c[0], F.c[1]), c02 = dot(F.c[0], F.c[2]), c12 = dot(F.c[1], F.c[2]);
-#pragma unroll
-    for (int sweep = 0; sweep < MLP_JACOBI_SWEEPS; ++sweep) {
-        mlp_jacobi_rotate(c00, c11, c01, c02, c12);   // pivot (0,1), other index 2
-        mlp_jacobi_rotate(c00, c22, c02, c01, c12);   // pivot (0,2), other index 1
-        mlp_jacobi_rotate(c11, c22, c12, c01, c02);   // pivot (1,2), other index 0
-    }
-    // a rounding-negative eigenvalue is 0; NaN (a non-finite F) stays NaN, so the prediction is dropped
-    float a = __builtin_amdgcn_sqrtf(c00 < 0.0f ? 0.0f : c00);
-    float b = __builtin_amdgcn_sqrtf(c11 < 0.0f ? 0.0f : c11);
-    float c = __builtin_amdgcn_sqrtf(c22 < 0.0f ? 0.0f : c22);
-    float t;
-    if (a < b) { t = a; a = b; b = t; }
-    if (b < c) { t = b; b = c; c = t; }
-    if (a < b) { t = a; a = b; b = t; }
-    s0 = a; s1 = b; s2 = c;
-}
-
 // Features of a particle at the start of a substep (see the header comment).
+// The singular values are those of the exact 6-sweep Jacobi SVD (svd3, the SVD plasticity option's).
+// An eigenvalue-only Jacobi on the hardware rcp / sqrt / rsqrt costs a tenth as much and agrees to
+// ~1e-7 in a float32 emulation, but warm starts trained on its features ran unstably (end-of-run
+// error 3-25x that of the same network on these features, and varying from one training to the next).
 MPM_DEV void mlp_features(const m3& Fe, float Jp, const m3& D, v3 dprev, float f[16]) {
-    mlp_singular_values(Fe, f[0], f[1], f[2]);
+    float a[9], v[9], b[9], sig[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { a[3 * c] = Fe.c[c].x; a[3 * c + 1] = Fe.c[c].y; a[3 * c + 2] = Fe.c[c].z; }
+    svd3<float>(a, v, b, sig);
+    float s0 = sig[0], s1 = sig[1], s2 = sig[2], t;        // largest first
+    if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
+    if (s1 < s2) { t = s1; s1 = s2; s2 = t; }
+    if (s0 < s1) { t = s0; s0 = s1; s1 = t; }
+    f[0] = s0; f[1] = s1; f[2] = s2;
     f[3] = Jp;
     for (int c = 0; c < 3; ++c) { f[4 + 3 * c] = D.c[c].x; f[5 + 3 * c] = D.c[c].y; f[6 + 3 * c] = D.c[c].z; }
     f[13] = dprev.x; f[14] = dprev.y; f[15] = dprev.z;
