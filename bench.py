@@ -127,7 +127,9 @@ def learned_vs_plain(mpm, n, k=16, k_ref=64, substeps=600):
         iterations whose error is at most the warm start's) / k;
       * the per-iteration residual r_i = mean over substeps of max_particles
         |dx_i - dx_{i-1} - g dt^2 / k|_2 / domain along each method's own run at k, and the
-        iterations each needs to reach plain's r at k / 2."""
+        iterations each needs to reach plain's r at k / 2;
+      * the time per substep of each run, and the warm start's cost on one state: a run at k
+        after 100 substeps, 64 substeps with the warm start off, then on, three times (median)."""
     def make(seed):
         sc = mpm.scenes.scene("pbmpm_two_spheres", n_particles=n, seed=seed)
         sim = mpm.MpmSimulation(n, **sc["sim"])
@@ -190,6 +192,25 @@ def learned_vs_plain(mpm, n, k=16, k_ref=64, substeps=600):
 
     thr = float(r_plain[k // 2])
     it_p, it_l = iters_to(r_plain, thr), iters_to(r_learn, thr)
+
+    # Cost on one state: the runs above follow different trajectories whose passes cost different
+    # amounts, so the warm start's price is timed by switching it off and on along one run at k.
+    sim = make(1)
+    sim.set_iterations(k)
+    sim.set_learn_params(*net)
+    sim.step(1, 100)
+    same = {"learned": [], "plain": []}
+    for _ in range(3):
+        for on in (False, True):
+            sim.learn_enable(on)
+            sim.step(1, 8)
+            sim.prepare_graphs(1)
+            sim.sync()
+            t = time.perf_counter()
+            sim.step(1, 64)
+            sim.sync()
+            same["learned" if on else "plain"].append((time.perf_counter() - t) / 64 * 1e3)
+    sim.close()
     return {
         "k": k, "k_ref": k_ref, "substeps": substeps, "training_pairs": pairs,
         "train_loss": [loss_a, loss_b], "train_and_collect_s": round(t_train, 2),
@@ -198,6 +219,7 @@ def learned_vs_plain(mpm, n, k=16, k_ref=64, substeps=600):
         "iteration_ratio": (match[0] / k) if match else k_ref / k,
         "iteration_ratio_is_bound": not match,
         "ms_per_substep": {"learned": round(ms_learn, 4), "plain": round(ms_plain, 4), "plain_k_ref": round(ms_ref, 4)},
+        "ms_per_substep_same_state": {m: round(float(np.median(v)), 4) for m, v in same.items()},
         "residual_plain": [float(x) for x in r_plain[1:]], "residual_learned": [float(x) for x in r_learn[1:]],
         "residual_threshold": thr, "residual_iters_plain": it_p, "residual_iters_learned": it_l,
         "residual_iteration_ratio": (it_p / it_l) if (it_p and it_l) else None,
