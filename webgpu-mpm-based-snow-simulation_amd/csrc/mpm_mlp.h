@@ -575,7 +575,9 @@ struct MlpInferWave {
 __global__ __launch_bounds__(256, 4) void k_mlp_warmstart(MlpBufs M, Bufs B, uint32_t n, float limit, uint32_t* n_fallback) {
     __shared__ MlpLds L;
     __shared__ MlpInferWave Wv[4];
+    __shared__ uint32_t n_fb;                 // this workgroup's fallbacks: one global atomic at the end
     mlp_stage_weights(L, nullptr, M);
+    if (threadIdx.x == 0) n_fb = 0u;
     __syncthreads();
     const int w = threadIdx.x >> 6, l = lane_id(), r = l % 16, q = l / 16;
     MlpInferWave& S = Wv[w];
@@ -618,9 +620,11 @@ __global__ __launch_bounds__(256, 4) void k_mlp_warmstart(MlpBufs M, Bufs B, uin
             }
         }
         const uint64_t fb = __ballot(i < n && !ok);
-        if (l == 0 && fb) atomicAdd(n_fallback, (uint32_t)__popcll(fb));
+        if (l == 0 && fb) atomicAdd(&n_fb, (uint32_t)__popcll(fb));
         wave_sync();
     }
+    __syncthreads();
+    if (threadIdx.x == 0 && n_fb) atomicAdd(n_fallback, n_fb);
 }
 
 }  // namespace mpm
