@@ -77,14 +77,17 @@ struct MlpBufs {
 // The backward images (training only, MlpLdsBack):
 //   w3t[n][l]      = W3[16n + l%16][4(l/16)+j]                     (backward dH2 = dY W3^T)
 //   w2t[kk][n][l]  = W2[16n + l%16][16kk + 4(l/16)+j]              (backward dH1 = dZ2 W2^T)
-// isd_x = 1 / sd_x: features are standardised as (x - mu_x) * isd_x.
-struct MlpLds {
+// isd_x = 1 / sd_x: features are standardised as (x - mu_x) * isd_x. 16-byte aligned: inference
+// reads the biases four at a time (b1, b2, b3 start at multiples of 16 bytes).
+struct alignas(16) MlpLds {
     bf4 w1b[4][64];
     bf4 w2b[4][4][64];
     bf4 w3b[4][64];
     float b1[64], b2[64], b3[16];
     float mu_x[16], isd_x[16], mu_y[16], sd_y[16];
 };
+static_assert(offsetof(MlpLds, b1) % 16 == 0 && offsetof(MlpLds, b2) % 16 == 0 && offsetof(MlpLds, b3) % 16 == 0,
+              "bias rows are read as 16-byte vectors");
 struct MlpLdsBack {
     bf4 w3t[4][64];
     bf4 w2t[4][4][64];
@@ -568,7 +571,7 @@ MPM_DEV f4 mlp_infer16(const MlpLds& L, bf4 xb) {
 // Per wave and 64 particles (lane l owns particle base + l): the features into xs (row l, bf16),
 // 4 tiles of 16 through mlp_infer16, the outputs into ys (row = particle, 12 floats: rows 48 B apart
 // keep the 16-byte writes and reads free of bank conflicts), then each lane applies its own row.
-struct MlpInferWave {
+struct alignas(16) MlpInferWave {  // ys rows are written 16 bytes at a time
     bf4 xs[64][4];            // [particle][features 4g..4g+3]
     float ys[64][MLP_NOUT];
 };
